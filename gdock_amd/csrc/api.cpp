@@ -1,0 +1,425 @@
+// C ABI of the gfx950 scoring / FCC library: contexts, complexes (device
+// resident receptor + positioned ligand), population scoring entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <string_view>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+#include "internal.h"
+
+using namespace gdock;
+
+namespace gdock {
+
+void set_error(gdock_ctx* ctx, const char* fmt, ...) {
+  if (!ctx) return;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  ctx->err = buf;
+}
+
+gdock_status hip_fail(gdock_ctx* ctx, hipError_t e, const char* what) {
+  set_error(ctx, "%s: %s", what, hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? GDOCK_ERR_OOM : GDOCK_ERR_DEVICE;
+}
+
+bool DeviceArena::reserve(size_t bytes, hipError_t* err) {
+  if (bytes <= cap) return true;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  const hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess) { *err = e; ptr = nullptr; return false; }
+  cap = bytes;
+  return true;
+}
+
+void DeviceArena::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+// Trimmed string of a fixed-size record field.
+std::string_view field(const char* f, size_t n) {
+  std::string_view s(f, strnlen(f, n));
+  while (!s.empty() && s.front() == ' ') s.remove_prefix(1);
+  while (!s.empty() && s.back() == ' ') s.remove_suffix(1);
+  return s;
+}
+
+// fitness.rs:129-138 atomic solvation parameter by element.
+double asp_of(const gdock_atom& a) {
+  const std::string_view e = field(a.element, sizeof a.element);
+  if (e == "C" || e == "S") return 0.012;
+  if (e == "N" || e == "O") return -0.160;
+  return 0.0;
+}
+
+// structure.rs:63-69: sequential sums divided by the count.
+void com_of(const gdock_atom* a, int32_t n, double* c) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int32_t i = 0; i < n; ++i) sx += a[i].x;
+  for (int32_t i = 0; i < n; ++i) sy += a[i].y;
+  for (int32_t i = 0; i < n; ++i) sz += a[i].z;
+  const double dn = static_cast<double>(n);
+  c[0] = sx / dn; c[1] = sy / dn; c[2] = sz / dn;
+}
+
+bool finite_atoms(const gdock_atom* a, int32_t n) {
+  for (int32_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i].x) || !std::isfinite(a[i].y) || !std::isfinite(a[i].z) ||
+        !std::isfinite(a[i].charge) || !std::isfinite(a[i].epsilon) || !std::isfinite(a[i].rmin2) ||
+        !std::isfinite(a[i].vdw_radius))
+      return false;
+  return true;
+}
+
+}  // namespace gdock
+
+extern "C" {
+
+const char* gdock_version(void) { return "gdock-mi355x 0.1 (gfx950)"; }
+
+gdock_status gdock_ctx_create(int device, gdock_ctx** out) {
+  if (!out) return GDOCK_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || device < 0 || device >= n) return GDOCK_ERR_DEVICE;
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return GDOCK_ERR_DEVICE;
+  auto* ctx = new gdock_ctx();
+  ctx->device = device;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete ctx; return GDOCK_ERR_DEVICE; }
+  *out = ctx;
+  return GDOCK_OK;
+}
+
+void gdock_ctx_destroy(gdock_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->staging.release();
+  release_fcc_scratch(ctx);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* gdock_last_error(const gdock_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+gdock_status gdock_read_pdb(const char* text, size_t len, gdock_atom** atoms, int32_t** model_offsets,
+                            int32_t* n_models) {
+  if (!text || !atoms || !model_offsets || !n_models) return GDOCK_ERR_INVALID_ARG;
+  std::vector<gdock_atom> a;
+  std::vector<int32_t> off;
+  read_pdb_text(std::string_view(text, len), &a, &off);
+  *atoms = static_cast<gdock_atom*>(std::malloc(std::max<size_t>(1, a.size()) * sizeof(gdock_atom)));
+  *model_offsets = static_cast<int32_t*>(std::malloc(off.size() * sizeof(int32_t)));
+  if (!*atoms || !*model_offsets) return GDOCK_ERR_OOM;
+  if (!a.empty()) std::memcpy(*atoms, a.data(), a.size() * sizeof(gdock_atom));
+  std::memcpy(*model_offsets, off.data(), off.size() * sizeof(int32_t));
+  *n_models = static_cast<int32_t>(off.size()) - 1;
+  return GDOCK_OK;
+}
+
+void gdock_free(void* p) { std::free(p); }
+
+gdock_status gdock_position_ligand(const gdock_atom* rec, int32_t n_rec, gdock_atom* lig, int32_t n_lig) {
+  if ((!rec && n_rec) || (!lig && n_lig) || n_rec < 0 || n_lig < 0) return GDOCK_ERR_INVALID_ARG;
+  double r[3], l[3];
+  com_of(rec, n_rec, r);
+  com_of(lig, n_lig, l);
+  for (int32_t i = 0; i < n_lig; ++i) { lig[i].x -= l[0]; lig[i].y -= l[1]; lig[i].z -= l[2]; }
+  for (int32_t i = 0; i < n_lig; ++i) { lig[i].x += r[0]; lig[i].y += r[1]; lig[i].z += r[2]; }
+  return GDOCK_OK;
+}
+
+gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t n_rec, const gdock_atom* lig,
+                                  int32_t n_lig, const int32_t* pairs, int32_t n_pairs, gdock_complex** out) {
+  if (!ctx || !out) return GDOCK_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (n_rec < 0 || n_lig < 0 || n_pairs < 0 || (n_rec && !rec) || (n_lig && !lig) || (n_pairs && !pairs)) {
+    set_error(ctx, "complex_create: bad sizes or null arrays");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (n_rec > kMaxLdsReceptor) {
+    set_error(ctx, "complex_create: %d receptor atoms exceeds the supported %d", n_rec, kMaxLdsReceptor);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (!finite_atoms(rec, n_rec) || !finite_atoms(lig, n_lig)) {
+    set_error(ctx, "complex_create: non-finite coordinate or parameter");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  auto* cx = new gdock_complex();
+  cx->rec.assign(rec, rec + n_rec);
+  cx->lig.assign(lig, lig + n_lig);
+
+  // Receptor permutation: group atoms by (rmin2, epsilon) class so that the
+  // kernel updates its per-lane Lennard-Jones factors once per class.
+  std::map<std::pair<double, double>, int> cls_of;
+  for (auto& a : cx->rec) cls_of.emplace(std::make_pair(a.rmin2, a.epsilon), 0);
+  int nc = 0;
+  for (auto& kv : cls_of) kv.second = nc++;
+  std::vector<int32_t> perm(n_rec);
+  for (int32_t i = 0; i < n_rec; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+    return cls_of[{cx->rec[a].rmin2, cx->rec[a].epsilon}] < cls_of[{cx->rec[b].rmin2, cx->rec[b].epsilon}];
+  });
+  std::vector<int32_t> class_start(nc + 1, 0);
+  std::vector<double> class_rmin(nc), class_eps(nc);
+  for (auto& kv : cls_of) { class_rmin[kv.second] = kv.first.first; class_eps[kv.second] = kv.first.second; }
+  for (auto& a : cx->rec) class_start[cls_of[{a.rmin2, a.epsilon}] + 1]++;
+  for (int c = 0; c < nc; ++c) class_start[c + 1] += class_start[c];
+
+  double origin[3] = {0, 0, 0}, lcom[3] = {0, 0, 0};
+  if (n_rec) com_of(rec, n_rec, origin);
+  if (n_lig) com_of(lig, n_lig, lcom);
+  std::unordered_set<int32_t> lig_serials;
+  for (auto& a : cx->lig) lig_serials.insert(a.serial);
+
+  std::vector<float4> xyzq(n_rec);
+  std::vector<double> rx(n_rec), ry(n_rec), rz(n_rec), rasp(n_rec), rrad(n_rec);
+  std::vector<int2> meta(n_rec);
+  double rec_ext = 0.0;
+  int has_coll = 0;
+  for (int32_t j = 0; j < n_rec; ++j) {
+    const gdock_atom& a = cx->rec[perm[j]];
+    const double px = a.x - origin[0], py = a.y - origin[1], pz = a.z - origin[2];
+    xyzq[j] = make_float4(static_cast<float>(px), static_cast<float>(py), static_cast<float>(pz),
+                          static_cast<float>(a.charge));
+    rec_ext = std::max({rec_ext, std::fabs(px), std::fabs(py), std::fabs(pz)});
+    rx[j] = a.x; ry[j] = a.y; rz[j] = a.z;
+    rasp[j] = asp_of(a);
+    rrad[j] = a.vdw_radius;
+    const int c = lig_serials.count(a.serial) ? 1 : 0;
+    meta[j] = make_int2(a.serial, c);
+    has_coll |= c;
+  }
+  std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
+  std::vector<int32_t> lser(n_lig);
+  double lig_r = 0.0;
+  for (int32_t i = 0; i < n_lig; ++i) {
+    const gdock_atom& a = cx->lig[i];
+    lx[i] = a.x; ly[i] = a.y; lz[i] = a.z;
+    lrm[i] = a.rmin2; leps[i] = a.epsilon; lq[i] = a.charge;
+    lasp[i] = asp_of(a); lrad[i] = a.vdw_radius; lser[i] = a.serial;
+    const double dx = a.x - lcom[0], dy = a.y - lcom[1], dz = a.z - lcom[2];
+    lig_r = std::max(lig_r, std::sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  const double com_off = std::max({std::fabs(lcom[0] - origin[0]), std::fabs(lcom[1] - origin[1]),
+                                   std::fabs(lcom[2] - origin[2])});
+
+  // restraints.rs:44-77: first "CA" with the residue number, receptor and ligand.
+  std::vector<double> rr_xyz;
+  std::vector<int32_t> rl_idx;
+  for (int32_t p = 0; p < n_pairs; ++p) {
+    int32_t ri = -1, li = -1;
+    for (int32_t i = 0; i < n_rec && ri < 0; ++i)
+      if (field(rec[i].name, sizeof rec[i].name) == "CA" && rec[i].resseq == pairs[2 * p]) ri = i;
+    for (int32_t i = 0; i < n_lig && li < 0; ++i)
+      if (field(lig[i].name, sizeof lig[i].name) == "CA" && lig[i].resseq == pairs[2 * p + 1]) li = i;
+    if (ri < 0 || li < 0) continue;  // dropped with a warning in the reference
+    rr_xyz.insert(rr_xyz.end(), {rec[ri].x, rec[ri].y, rec[ri].z});
+    rl_idx.push_back(li);
+  }
+  cx->n_restr = static_cast<int32_t>(rl_idx.size());
+
+  Layout L;
+  const size_t o_xyzq = L.add<float4>(n_rec), o_rx = L.add<double>(n_rec), o_ry = L.add<double>(n_rec),
+               o_rz = L.add<double>(n_rec), o_rasp = L.add<double>(n_rec), o_rrad = L.add<double>(n_rec),
+               o_meta = L.add<int2>(n_rec), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
+               o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
+               o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
+               o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
+               o_lser = L.add<int32_t>(n_lig), o_rrx = L.add<double>(rr_xyz.size()),
+               o_rli = L.add<int32_t>(rl_idx.size());
+  cx->contacts_layout_begin = L.size;
+  std::vector<uint8_t> host;
+  build_contact_tables(cx, &L, &host);  // FCC residue tables appended to the same arena
+  std::vector<uint8_t> blob(L.size, 0);
+  auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(blob.data() + off, src, bytes); };
+  put(o_xyzq, xyzq.data(), xyzq.size() * sizeof(float4));
+  put(o_rx, rx.data(), n_rec * 8); put(o_ry, ry.data(), n_rec * 8); put(o_rz, rz.data(), n_rec * 8);
+  put(o_rasp, rasp.data(), n_rec * 8); put(o_rrad, rrad.data(), n_rec * 8);
+  put(o_meta, meta.data(), n_rec * sizeof(int2));
+  put(o_cs, class_start.data(), (nc + 1) * 4); put(o_crm, class_rmin.data(), nc * 8); put(o_ceps, class_eps.data(), nc * 8);
+  put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
+  put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
+  put(o_lasp, lasp.data(), n_lig * 8); put(o_lrad, lrad.data(), n_lig * 8); put(o_lser, lser.data(), n_lig * 4);
+  put(o_rrx, rr_xyz.data(), rr_xyz.size() * 8); put(o_rli, rl_idx.data(), rl_idx.size() * 4);
+  if (!host.empty()) std::memcpy(blob.data() + cx->contacts_layout_begin, host.data(), host.size());
+
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipMalloc(&cx->mem, std::max<size_t>(L.size, 256));
+  if (e != hipSuccess) { delete cx; return hip_fail(ctx, e, "complex_create: hipMalloc"); }
+  cx->mem_bytes = std::max<size_t>(L.size, 256);
+  e = hipMemcpyAsync(cx->mem, blob.data(), L.size, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) { (void)hipFree(cx->mem); delete cx; return hip_fail(ctx, e, "complex_create: upload"); }
+
+  char* m = static_cast<char*>(cx->mem);
+  DevComplex& d = cx->dev;
+  d.rec_xyzq = reinterpret_cast<const float4*>(m + o_xyzq);
+  d.rec_x = reinterpret_cast<const double*>(m + o_rx);
+  d.rec_y = reinterpret_cast<const double*>(m + o_ry);
+  d.rec_z = reinterpret_cast<const double*>(m + o_rz);
+  d.rec_asp = reinterpret_cast<const double*>(m + o_rasp);
+  d.rec_radius = reinterpret_cast<const double*>(m + o_rrad);
+  d.rec_meta = reinterpret_cast<const int2*>(m + o_meta);
+  d.class_start = reinterpret_cast<const int32_t*>(m + o_cs);
+  d.class_rmin2 = reinterpret_cast<const double*>(m + o_crm);
+  d.class_eps = reinterpret_cast<const double*>(m + o_ceps);
+  d.lig_x = reinterpret_cast<const double*>(m + o_lx);
+  d.lig_y = reinterpret_cast<const double*>(m + o_ly);
+  d.lig_z = reinterpret_cast<const double*>(m + o_lz);
+  d.lig_rmin2 = reinterpret_cast<const double*>(m + o_lrm);
+  d.lig_eps = reinterpret_cast<const double*>(m + o_leps);
+  d.lig_q = reinterpret_cast<const double*>(m + o_lq);
+  d.lig_asp = reinterpret_cast<const double*>(m + o_lasp);
+  d.lig_radius = reinterpret_cast<const double*>(m + o_lrad);
+  d.lig_serial = reinterpret_cast<const int32_t*>(m + o_lser);
+  d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
+  d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
+  d.n_rec = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
+  d.has_serial_collision = has_coll;
+  for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }
+  d.rec_extent = rec_ext;
+  d.lig_extent = lig_r + com_off;
+  bind_contact_tables(cx);
+  *out = cx;
+  return GDOCK_OK;
+}
+
+void gdock_complex_destroy(gdock_ctx* ctx, gdock_complex* cx) {
+  if (!cx) return;
+  if (ctx) { (void)hipSetDevice(ctx->device); (void)hipStreamSynchronize(ctx->stream); }
+  if (cx->mem) (void)hipFree(cx->mem);
+  delete cx;
+}
+
+int32_t gdock_complex_n_restraints(const gdock_complex* cx) { return cx ? cx->n_restr : -1; }
+
+gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses) {
+  if (n < 0 || (n && (!genes || !poses))) return GDOCK_ERR_INVALID_ARG;
+  for (int32_t i = 0; i < n; ++i) {
+    const double* g = genes + 6 * i;
+    for (int k = 0; k < 6; ++k)
+      if (!std::isfinite(g[k])) return GDOCK_ERR_INVALID_ARG;
+    double* p = poses + kPoseDoubles * i;
+    for (int k = 0; k < 6; ++k) p[k] = g[k];
+    // glibc cos/sin, as the reference's f64::cos/sin (structure.rs:96-103)
+    p[6] = std::cos(g[0]); p[7] = std::sin(g[0]);
+    p[8] = std::cos(g[1]); p[9] = std::sin(g[1]);
+    p[10] = std::cos(g[2]); p[11] = std::sin(g[2]);
+  }
+  return GDOCK_OK;
+}
+
+gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const double* d_poses, int32_t n,
+                                      const gdock_weights* w, double* d_out, int32_t* d_nsat, void* stream) {
+  if (!ctx || !cx || !w || n < 0 || (n && (!d_poses || !d_out))) return GDOCK_ERR_INVALID_ARG;
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const hipError_t e = launch_score(cx->dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat, s);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device: launch");
+}
+
+gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                               const gdock_weights* w, double* out, int32_t* n_satisfied) {
+  if (!ctx || !cx || !w || n < 0 || (n && (!genes || !out))) {
+    if (ctx) set_error(ctx, "score_poses: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (n == 0) return GDOCK_OK;
+  std::vector<double> poses(static_cast<size_t>(n) * kPoseDoubles);
+  if (gdock_prepare_poses(genes, n, poses.data()) != GDOCK_OK) {
+    set_error(ctx, "score_poses: non-finite gene");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: set device");
+  const size_t pb = poses.size() * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8, nb = static_cast<size_t>(n) * 4;
+  const size_t o_out = (pb + 255) & ~static_cast<size_t>(255), o_ns = (o_out + ob + 255) & ~static_cast<size_t>(255);
+  if (!ctx->staging.reserve(o_ns + nb, &e)) return hip_fail(ctx, e, "score_poses: staging");
+  char* base = static_cast<char*>(ctx->staging.ptr);
+  e = hipMemcpyAsync(base, poses.data(), pb, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: upload");
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  e = launch_score(cx->dev, reinterpret_cast<double*>(base), nullptr, nullptr, n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: launch");
+  e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses: download");
+}
+
+gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double* lig_xyz, int32_t n,
+                                const gdock_weights* w, double* out, int32_t* n_satisfied) {
+  if (!ctx || !cx || !w || n < 0 || (n && (!lig_xyz || !out))) return GDOCK_ERR_INVALID_ARG;
+  if (n == 0) return GDOCK_OK;
+  const int32_t nl = cx->dev.n_lig;
+  const size_t ncoord = static_cast<size_t>(n) * nl * 3;
+  std::vector<double> extent(n, 0.0);
+  for (int32_t p = 0; p < n; ++p)
+    for (int32_t i = 0; i < nl; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const double v = lig_xyz[(static_cast<size_t>(p) * nl + i) * 3 + k];
+        if (!std::isfinite(v)) { set_error(ctx, "score_coords: non-finite coordinate"); return GDOCK_ERR_INVALID_ARG; }
+        extent[p] = std::max(extent[p], std::fabs(v - cx->dev.origin[k]));
+      }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: set device");
+  const size_t cb = ncoord * 8, eb = static_cast<size_t>(n) * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8,
+               nb = static_cast<size_t>(n) * 4;
+  auto al = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+  const size_t o_ext = al(cb), o_out = al(o_ext + eb), o_ns = al(o_out + ob);
+  if (!ctx->staging.reserve(o_ns + nb, &e)) return hip_fail(ctx, e, "score_coords: staging");
+  char* base = static_cast<char*>(ctx->staging.ptr);
+  e = hipMemcpyAsync(base, lig_xyz, cb, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_ext, extent.data(), eb, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: upload");
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  e = launch_score(cx->dev, nullptr, reinterpret_cast<double*>(base), reinterpret_cast<double*>(base + o_ext), n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: launch");
+  e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_coords: download");
+}
+
+gdock_status gdock_eval_fitness(gdock_ctx* ctx, gdock_complex* cx, gdock_chromosome* ch, int32_t n,
+                                const gdock_weights* w) {
+  if (!ctx || !cx || !w || n < 0 || (n && !ch)) return GDOCK_ERR_INVALID_ARG;
+  std::vector<double> genes(static_cast<size_t>(n) * 6), out(static_cast<size_t>(n) * kOutDoubles);
+  for (int32_t i = 0; i < n; ++i) std::memcpy(&genes[6 * i], ch[i].genes, 6 * sizeof(double));
+  const gdock_status s = gdock_score_poses(ctx, cx, genes.data(), n, w, out.data(), nullptr);
+  if (s != GDOCK_OK) return s;
+  for (int32_t i = 0; i < n; ++i) {  // chromosome.rs:111-127
+    const double* o = &out[kOutDoubles * i];
+    ch[i].vdw = o[0]; ch[i].elec = o[1]; ch[i].desolv = o[2]; ch[i].air = o[3];
+    ch[i].restraint_penalty = o[4]; ch[i].fitness = o[5];
+  }
+  return GDOCK_OK;
+}
+
+}  // extern "C"

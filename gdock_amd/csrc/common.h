@@ -1,0 +1,68 @@
+// Shared host/device definitions of the gfx950 scoring and FCC library.
+#pragma once
+#include <cstdint>
+
+namespace gdock {
+
+// constants.rs:22-72, fitness.rs:129-138,286
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kCoulomb = 332.0636;
+constexpr double kAirK = 10.0;
+constexpr double kAirUpper = 7.0;
+
+// Exact FP64 squared-distance forms of the reference's sqrt-based predicates
+// (a correctly rounded sqrt(d2) compared with the cutoff).  Each equality is
+// checked by brute force around the boundary in the test suite.
+constexpr double kD2ElecMinExcl = 1.0000000000000002;    // dist > 1.0  <=> d2 >  this
+constexpr double kD2DesolvIncl = 63.99999999999999;      // dist < 8.0  <=> d2 <= this
+constexpr double kD2VdwIncl = 143.99999999999997;        // dist < 12.0 <=> d2 <= this
+constexpr double kD2ElecMaxIncl = 224.99999999999997;    // dist < 15.0 <=> d2 <= this
+constexpr double kD2RestraintIncl = 49.0;                // dist <= 7.0 <=> d2 <= this
+constexpr double kD2ContactIncl = 25.000000000000004;    // dist <= 5.0 <=> d2 <= this
+
+constexpr int kPoseDoubles = 12;  // genes[6], cos/sin alpha, beta, gamma
+constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, fitness
+
+constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup
+constexpr int kMaxAtomsPerLane = 8;
+constexpr int kMaxLdsReceptor = 16000;  // receptor burial counters kept in LDS
+constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away
+
+// Device view of one complex (kernel argument, plain pointers).
+struct DevComplex {
+  // receptor, permuted so that atoms of one (rmin2, epsilon) class are contiguous
+  const float4* rec_xyzq;      // x, y, z relative to origin (FP32), charge
+  const double* rec_x;         // FP64 original coordinates (permuted)
+  const double* rec_y;
+  const double* rec_z;
+  const double* rec_asp;       // atomic solvation parameter (0 = skipped)
+  const double* rec_radius;
+  const int2* rec_meta;        // {serial, 1 if the serial also occurs in the ligand}
+  const int32_t* class_start;  // n_class + 1
+  const double* class_rmin2;
+  const double* class_eps;
+  // ligand (positioned), original order
+  const double* lig_x;
+  const double* lig_y;
+  const double* lig_z;
+  const double* lig_rmin2;
+  const double* lig_eps;
+  const double* lig_q;
+  const double* lig_asp;
+  const double* lig_radius;
+  const int32_t* lig_serial;
+  // restraints: receptor CA coordinates (FP64) and ligand CA index
+  const double* restr_rec_xyz;  // n_restr x 3
+  const int32_t* restr_lig;     // n_restr
+  int32_t n_rec, n_lig, n_class, n_restr;
+  int32_t has_serial_collision;
+  int32_t pad_;
+  double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
+  double origin[3];  // FP32 frame origin
+  double rec_extent;   // max |rec - origin| component
+  double lig_extent;   // max |lig - com| (2-norm) + max |com - origin| component
+};
+
+struct Weights { double vdw, elec, desolv, air; };
+
+}  // namespace gdock

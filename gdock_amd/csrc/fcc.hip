@@ -1,0 +1,643 @@
+// FCC clustering on gfx950: residue contact bitsets, pairwise overlap with
+// AND + popcount, threshold graph, greedy clustering (clustering.rs:62-293).
+//
+// Contacts: one workgroup per pose; ligand heavy atoms (FP64 pose transform,
+// then FP32) sit in lanes, receptor heavy atoms are wave-uniform scalar loads
+// walked residue by residue.  The 5 A test is FP32 against a guard band with
+// an FP64 re-decision inside the band, so every contact bit equals the FP64
+// reference's decision.  Bit b = receptor residue * n_lig_res + ligand
+// residue; the residue-pair string of a bit is reported by gdock_contact_pair.
+//
+// Overlap: bitsets are compacted to the 32-bit words that are set in any
+// pose and stored word-major, so a wave reads 16 row words with one scalar
+// load and its 64 column words with one coalesced vector load; each (pair,
+// word) costs v_and_b32 + v_bcnt_u32_b32.  The neighbour decisions
+// fcc >= cutoff and fcc_v >= cutoff * strictness are exact integer compares
+// against per-set-size minima that the host derives with the reference's own
+// FP64 division, and are written as two bit matrices: A (row i = nbr(i)) and
+// its transpose.  Greedy clustering runs in one workgroup with incremental
+// unused-neighbour counts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+#include "internal.h"
+
+namespace gdock {
+
+// ---------------------------------------------------------------- tables
+void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host) {
+  cx->con = DevContacts{};
+  cx->con_offsets.clear();
+  cx->rec_res_key.clear();
+  cx->lig_res_key.clear();
+  cx->con_unsupported.clear();
+  auto heavy = [](const gdock_atom& a) { return field(a.element, sizeof a.element) != "H"; };
+  std::map<std::pair<char, int16_t>, std::vector<int32_t>> rres, lres;
+  std::set<char> rch, lch;
+  for (int32_t i = 0; i < static_cast<int32_t>(cx->rec.size()); ++i)
+    if (heavy(cx->rec[i])) { rres[{cx->rec[i].chainid, cx->rec[i].resseq}].push_back(i); rch.insert(cx->rec[i].chainid); }
+  for (int32_t i = 0; i < static_cast<int32_t>(cx->lig.size()); ++i)
+    if (heavy(cx->lig[i])) { lres[{cx->lig[i].chainid, cx->lig[i].resseq}].push_back(i); lch.insert(cx->lig[i].chainid); }
+  // The device path covers one receptor chain against one other ligand chain
+  // (the layout every docking run uses).  Same-chain keys would merge
+  // receptor and ligand residues and extra chains add receptor-receptor or
+  // ligand-ligand candidates; those topologies are rejected by gdock_contacts.
+  if (rch.size() > 1 || lch.size() > 1 || (!rch.empty() && !lch.empty() && *rch.begin() == *lch.begin())) {
+    cx->con_unsupported = "contacts: device path needs one receptor chain and one different ligand chain";
+    return;
+  }
+  std::vector<float4> rxyz;
+  std::vector<double> rx, ry, rz;
+  std::vector<int32_t> rstart{0}, latom, lresi;
+  double origin[3] = {0, 0, 0};
+  if (!cx->rec.empty()) {  // same origin as the scoring tables (receptor COM)
+    double sx = 0, sy = 0, sz = 0;
+    for (auto& a : cx->rec) sx += a.x;
+    for (auto& a : cx->rec) sy += a.y;
+    for (auto& a : cx->rec) sz += a.z;
+    const double n = static_cast<double>(cx->rec.size());
+    origin[0] = sx / n; origin[1] = sy / n; origin[2] = sz / n;
+  }
+  for (auto& kv : rres) {
+    cx->rec_res_key.push_back(kv.first);
+    for (int32_t i : kv.second) {
+      const gdock_atom& a = cx->rec[i];
+      rxyz.push_back(make_float4(static_cast<float>(a.x - origin[0]), static_cast<float>(a.y - origin[1]),
+                                 static_cast<float>(a.z - origin[2]), 0.0f));
+      rx.push_back(a.x); ry.push_back(a.y); rz.push_back(a.z);
+    }
+    rstart.push_back(static_cast<int32_t>(rxyz.size()));
+  }
+  int32_t li = 0;
+  for (auto& kv : lres) {
+    cx->lig_res_key.push_back(kv.first);
+    for (int32_t i : kv.second) { latom.push_back(i); lresi.push_back(li); }
+    ++li;
+  }
+  DevContacts& c = cx->con;
+  c.n_rec_heavy = static_cast<int32_t>(rxyz.size());
+  c.n_lig_heavy = static_cast<int32_t>(latom.size());
+  c.n_rec_res = static_cast<int32_t>(cx->rec_res_key.size());
+  c.n_lig_res = static_cast<int32_t>(cx->lig_res_key.size());
+  const int64_t bits = static_cast<int64_t>(c.n_rec_res) * c.n_lig_res;
+  c.words64 = static_cast<int32_t>((bits + 63) / 64);
+  if (c.words64 * 8 > 60 * 1024) {
+    cx->con_unsupported = "contacts: residue-pair bitset exceeds the 60 KiB workgroup budget";
+    c.words64 = 0;
+    return;
+  }
+  c.supported = 1;
+  const size_t begin = cx->contacts_layout_begin;
+  const size_t o1 = L->add<float4>(rxyz.size()), o2 = L->add<double>(rx.size()), o3 = L->add<double>(ry.size()),
+               o4 = L->add<double>(rz.size()), o5 = L->add<int32_t>(rstart.size()),
+               o6 = L->add<int32_t>(latom.size()), o7 = L->add<int32_t>(lresi.size());
+  host->assign(L->size - begin, 0);
+  auto put = [&](size_t off, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(host->data() + (off - begin), src, bytes);
+  };
+  put(o1, rxyz.data(), rxyz.size() * sizeof(float4));
+  put(o2, rx.data(), rx.size() * 8); put(o3, ry.data(), ry.size() * 8); put(o4, rz.data(), rz.size() * 8);
+  put(o5, rstart.data(), rstart.size() * 4); put(o6, latom.data(), latom.size() * 4); put(o7, lresi.data(), lresi.size() * 4);
+  cx->con_offsets = {o1, o2, o3, o4, o5, o6, o7};
+}
+
+void bind_contact_tables(gdock_complex* cx) {
+  if (!cx->con.supported) return;
+  char* m = static_cast<char*>(cx->mem);
+  const auto& o = cx->con_offsets;
+  DevContacts& c = cx->con;
+  c.rec_xyz = reinterpret_cast<const float4*>(m + o[0]);
+  c.rec_x = reinterpret_cast<const double*>(m + o[1]);
+  c.rec_y = reinterpret_cast<const double*>(m + o[2]);
+  c.rec_z = reinterpret_cast<const double*>(m + o[3]);
+  c.rec_res_start = reinterpret_cast<const int32_t*>(m + o[4]);
+  c.lig_atom = reinterpret_cast<const int32_t*>(m + o[5]);
+  c.lig_res = reinterpret_cast<const int32_t*>(m + o[6]);
+}
+
+void release_fcc_scratch(gdock_ctx* ctx) { ctx->fcc.release(); }
+
+// ---------------------------------------------------------------- device helpers
+struct PoseT {
+  double ca, sa, cb, sb, m01, m02, m11, m12, cg, sg, dx, dy, dz;
+};
+
+__device__ __forceinline__ PoseT pose_of(const double* __restrict__ p) {
+  PoseT c;
+  c.dx = p[3]; c.dy = p[4]; c.dz = p[5];
+  c.ca = p[6]; c.sa = p[7]; c.cb = p[8]; c.sb = p[9]; c.cg = p[10]; c.sg = p[11];
+  c.m01 = (-c.sa * c.cg) + ((c.ca * c.sb) * c.sg);
+  c.m02 = (c.sa * c.sg) + ((c.ca * c.sb) * c.cg);
+  c.m11 = (c.ca * c.cg) + ((c.sa * c.sb) * c.sg);
+  c.m12 = (-c.ca * c.sg) + ((c.sa * c.sb) * c.cg);
+  return c;
+}
+
+// structure.rs:80-131 operation order (file built with -ffp-contract=off)
+__device__ __forceinline__ void pose_atom(const DevComplex& cx, const PoseT& pc, int l, double& X, double& Y, double& Z) {
+  const double x = cx.lig_x[l] - cx.com[0];
+  const double y = cx.lig_y[l] - cx.com[1];
+  const double z = cx.lig_z[l] - cx.com[2];
+  const double nx = ((x * pc.ca) * pc.cb + y * pc.m01) + z * pc.m02;
+  const double ny = ((x * pc.sa) * pc.cb + y * pc.m11) + z * pc.m12;
+  const double nz = ((-x) * pc.sb + (y * pc.cb) * pc.sg) + (z * pc.cb) * pc.cg;
+  X = (nx + cx.com[0]) + pc.dx;
+  Y = (ny + cx.com[1]) + pc.dy;
+  Z = (nz + cx.com[2]) + pc.dz;
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ void band(double T, double X, float& lo, float& hi) {
+  const double u = 5.9604644775390625e-08;
+  const double s = sqrt(T) * 1.01 + 0.01;
+  const double e = 2.0 * u * X + u * s;
+  const double b = 2.0 * (3.4641016151377544 * s * e * (1.0 + 2.0 * u) + 3.0 * e * e + 3.01 * u * T * 1.01) + 1e-9;
+  float l = static_cast<float>(T - b), h = static_cast<float>(T + b);
+  if (static_cast<double>(l) > T - b) l = nextafterf(l, -INFINITY);
+  if (static_cast<double>(h) < T + b) h = nextafterf(h, INFINITY);
+  lo = uniform_f(l);
+  hi = uniform_f(h);
+}
+
+// ---------------------------------------------------------------- K6 contacts
+template <int K>
+__global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, const DevContacts ct,
+                                                       const double* __restrict__ poses, double d2_incl,
+                                                       unsigned long long* __restrict__ bits_out,
+                                                       int32_t* __restrict__ sizes) {
+  extern __shared__ uint32_t s_bits[];
+  __shared__ int32_t s_cnt[4];
+  const int pose = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int words32 = 2 * ct.words64;
+  for (int i = tid; i < words32; i += 256) s_bits[i] = 0u;
+  const PoseT pc = pose_of(poses + static_cast<size_t>(kPoseDoubles) * pose);
+  const double X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
+  float lo, hi;
+  band(d2_incl, X, lo, hi);
+  __syncthreads();
+
+  for (int base = 0; base < ct.n_lig_heavy; base += 256 * K) {
+    float lx[K], ly[K], lz[K];
+    int32_t lres[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int h = base + tid + 256 * k;
+      if (h < ct.n_lig_heavy) {
+        double px, py, pz;
+        pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
+        lx[k] = static_cast<float>(px - cx.origin[0]);
+        ly[k] = static_cast<float>(py - cx.origin[1]);
+        lz[k] = static_cast<float>(pz - cx.origin[2]);
+        lres[k] = ct.lig_res[h];
+      } else {
+        lx[k] = ly[k] = lz[k] = kPadCoord;
+        lres[k] = 0;
+      }
+    }
+    for (int ri = 0; ri < ct.n_rec_res; ++ri) {
+      bool hit[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) hit[k] = false;
+      const int j1 = ct.rec_res_start[ri + 1];
+      for (int j = ct.rec_res_start[ri]; j < j1; ++j) {
+        const float4 r = ct.rec_xyz[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float dx = r.x - lx[k], dy = r.y - ly[k], dz = r.z - lz[k];
+          const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+          const bool a = d2 < hi, b = d2 <= lo;
+          bool m = a;
+          if (__any(a != b)) {
+            const int h = base + tid + 256 * k;
+            if (a != b && h < ct.n_lig_heavy) {  // clustering.rs:98-99 in FP64
+              double px, py, pz;
+              pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
+              const double ex = ct.rec_x[j] - px, ey = ct.rec_y[j] - py, ez = ct.rec_z[j] - pz;
+              m = ex * ex + ey * ey + ez * ez <= d2_incl;
+            }
+          }
+          hit[k] = hit[k] || m;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (hit[k]) {
+          const uint32_t bit = static_cast<uint32_t>(ri) * static_cast<uint32_t>(ct.n_lig_res) + static_cast<uint32_t>(lres[k]);
+          atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+        }
+    }
+  }
+  __syncthreads();
+  int cnt = 0;
+  unsigned long long* dst = bits_out + static_cast<size_t>(pose) * ct.words64;
+  for (int w = tid; w < ct.words64; w += 256) {
+    const unsigned long long v = static_cast<unsigned long long>(s_bits[2 * w]) |
+                                 (static_cast<unsigned long long>(s_bits[2 * w + 1]) << 32);
+    dst[w] = v;
+    cnt += __popcll(v);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  if (lane == 0) s_cnt[wave] = cnt;
+  __syncthreads();
+  if (tid == 0) sizes[pose] = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+}
+
+// ---------------------------------------------------------------- K7 overlap
+// OR of all poses' words (32-bit view): union[w].
+__global__ void union_kernel(const uint32_t* __restrict__ bits, int n, int words32, uint32_t* __restrict__ uni) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words32) return;
+  uint32_t u = 0;
+  for (int p = 0; p < n; ++p) u |= bits[static_cast<size_t>(p) * words32 + w];
+  uni[w] = u;
+}
+
+// Word-major compacted copy: T[k * n_pad + p] = bits[p][col[k]].
+__global__ void compact_kernel(const uint32_t* __restrict__ bits, int n, int n_pad, int words32,
+                               const int32_t* __restrict__ col, int wc, uint32_t* __restrict__ T) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (p >= n_pad || k >= wc) return;
+  T[static_cast<size_t>(k) * n_pad + p] = p < n ? bits[static_cast<size_t>(p) * words32 + col[k]] : 0u;
+}
+
+constexpr int kRowsPerWave = 16;
+
+// Tile: 64 rows (4 waves x 16 rows) x 64 columns (one per lane).  Writes the
+// words A[i][jblk] (bits j with j in nbr(i)) and AT[i][jblk] (bits j with i in
+// nbr(j)) for the tile's rows.
+__global__ __launch_bounds__(256) void overlap_kernel(const uint32_t* __restrict__ T, int n, int n_pad, int wc,
+                                                      const int32_t* __restrict__ sizes,
+                                                      const int32_t* __restrict__ cmin_a,
+                                                      const int32_t* __restrict__ cmin_b, int zero_ok,
+                                                      unsigned long long* __restrict__ A,
+                                                      unsigned long long* __restrict__ AT, int nwords) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jblk = blockIdx.x, iblk = blockIdx.y;
+  const int j = jblk * 64 + lane;
+  const int i0 = iblk * 64 + wave * kRowsPerWave;
+  if (i0 >= n) return;  // whole wave past the last row
+  uint32_t c[kRowsPerWave];
+#pragma unroll
+  for (int r = 0; r < kRowsPerWave; ++r) c[r] = 0u;
+  const uint32_t* colp = T + j;
+  const uint32_t* rowp = T + i0;
+  for (int k = 0; k < wc; ++k) {
+    const uint32_t xj = colp[static_cast<size_t>(k) * n_pad];
+    const uint32_t* rk = rowp + static_cast<size_t>(k) * n_pad;
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) c[r] += __popc(rk[r] & xj);
+  }
+  const int sj = j < n ? sizes[j] : 0;
+  const int cbj = j < n && sj > 0 ? cmin_b[sj] : INT_MAX;
+  const int caj = j < n && sj > 0 ? cmin_a[sj] : INT_MAX;
+#pragma unroll
+  for (int r = 0; r < kRowsPerWave; ++r) {
+    const int i = i0 + r;
+    if (i >= n) break;
+    const int si = sizes[i];
+    bool fwd, bwd;  // j in nbr(i); i in nbr(j)   (clustering.rs:169-177)
+    if (si == 0 || sj == 0) {
+      fwd = bwd = zero_ok && j < n;
+    } else {
+      const int c_ = static_cast<int>(c[r]);
+      fwd = c_ >= cmin_a[si] && c_ >= cbj;
+      bwd = c_ >= caj && c_ >= cmin_b[si];
+    }
+    const unsigned long long wf = __ballot(fwd && j < n);
+    const unsigned long long wb = __ballot(bwd && j < n);
+    if (lane == 0) {
+      A[static_cast<size_t>(i) * nwords + jblk] = wf;
+      AT[static_cast<size_t>(i) * nwords + jblk] = wb;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- K8 greedy
+// clustering.rs:183-254 in one workgroup.  count[a] = unused neighbours of a
+// (self included).  Each step takes the unused element with the largest
+// count (lowest index on ties), forms {centre} + its unused neighbours (in
+// ascending order), marks them used and decrements the counts of every
+// element that has a new member as neighbour (rows of AT).  Cluster sizes are
+// non-increasing along the sequence, so the loop stops as soon as no element
+// can reach min_size (count + 1 < min_size).
+constexpr int kGreedyThreads = 1024;
+constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count + 1 bit used flag per element
+
+__global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned long long* __restrict__ A,
+                                                                const unsigned long long* __restrict__ AT,
+                                                                int n, int nwords, long long min_size,
+                                                                int64_t* __restrict__ out_center,
+                                                                int64_t* __restrict__ out_size,
+                                                                int64_t* __restrict__ out_offset,
+                                                                int64_t* __restrict__ out_members,
+                                                                int32_t* __restrict__ out_n) {
+  extern __shared__ unsigned long long s_used[];  // nwords, then count[n]
+  int32_t* s_count = reinterpret_cast<int32_t*>(s_used + nwords);
+  __shared__ unsigned long long s_key[kGreedyThreads / 64];
+  __shared__ int32_t s_scan[kGreedyThreads / 64];
+  __shared__ unsigned long long s_best;
+  __shared__ int64_t s_total;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int a = tid; a < n; a += kGreedyThreads) {
+    int c = 0;
+    for (int w = 0; w < nwords; ++w) c += __popcll(A[static_cast<size_t>(a) * nwords + w]);
+    s_count[a] = c;
+  }
+  for (int w = tid; w < nwords; w += kGreedyThreads) s_used[w] = 0ull;
+  __syncthreads();
+  int32_t nclu = 0;
+  int64_t written = 0;
+  for (;;) {
+    // argmax over unused of (count, -index)
+    unsigned long long key = 0ull;
+    for (int a = tid; a < n; a += kGreedyThreads) {
+      if ((s_used[a >> 6] >> (a & 63)) & 1ull) continue;
+      const unsigned long long k = (static_cast<unsigned long long>(static_cast<uint32_t>(s_count[a]) + 1u) << 32) |
+                                   static_cast<unsigned long long>(0xFFFFFFFFu - static_cast<uint32_t>(a));
+      key = k > key ? k : key;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(key, off);
+      key = o > key ? o : key;
+    }
+    if (lane == 0) s_key[wave] = key;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long b = 0ull;
+      for (int i = 0; i < kGreedyThreads / 64; ++i) b = s_key[i] > b ? s_key[i] : b;
+      s_best = b;
+    }
+    __syncthreads();
+    const unsigned long long best = s_best;
+    if (best == 0ull) break;  // every element used
+    const long long best_count = static_cast<long long>(best >> 32) - 1;
+    if (min_size > 1 && best_count + 1 < min_size) break;
+    const int center = static_cast<int>(0xFFFFFFFFu - static_cast<uint32_t>(best & 0xFFFFFFFFull));
+    // members = centre + its unused neighbours, ascending (one word per thread)
+    const unsigned long long* row = A + static_cast<size_t>(center) * nwords;
+    unsigned long long mine = 0ull;
+    if (tid < nwords) {
+      mine = row[tid] & ~s_used[tid];
+      if (tid == (center >> 6)) mine |= 1ull << (center & 63);
+    }
+    const int local = __popcll(mine);
+    int incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) s_scan[wave] = incl;
+    __syncthreads();
+    int wbase = 0;
+    for (int i = 0; i < wave; ++i) wbase += s_scan[i];
+    if (tid == kGreedyThreads - 1) s_total = wbase + incl;
+    int pos = wbase + incl - local;
+    for (unsigned long long m = mine; m; m &= m - 1)
+      out_members[written + pos++] = static_cast<int64_t>(tid) * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+    if (tid < nwords) s_used[tid] |= mine;
+    __syncthreads();
+    const int64_t size = s_total;
+    // decrement the counts of elements that have a new member as neighbour
+    for (int64_t q = tid; q < size * nwords; q += kGreedyThreads) {
+      const int64_t u = out_members[written + q / nwords];
+      const int w = static_cast<int>(q % nwords);
+      for (unsigned long long m = AT[static_cast<size_t>(u) * nwords + w]; m; m &= m - 1)
+        atomicSub(&s_count[w * 64 + (__ffsll(static_cast<long long>(m)) - 1)], 1);
+    }
+    if (tid == 0) {
+      out_center[nclu] = center;
+      out_size[nclu] = size;
+      out_offset[nclu] = written;
+    }
+    ++nclu;
+    written += size;
+    __syncthreads();
+  }
+  if (tid == 0) *out_n = nclu;
+}
+
+template <int K>
+hipError_t launch_contacts_k(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
+                             unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
+  hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), static_cast<size_t>(cx->con.words64) * 8, s, cx->dev,
+                     cx->con, d_poses, d2_incl, d_bits, d_sizes);
+  return hipGetLastError();
+}
+
+hipError_t launch_contacts(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
+                           unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int k = (cx->con.n_lig_heavy + 255) / 256;
+  k = k < 1 ? 1 : (k > 8 ? 8 : k);
+  switch (k) {
+    case 1: return launch_contacts_k<1>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 2: return launch_contacts_k<2>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 3: return launch_contacts_k<3>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 4: return launch_contacts_k<4>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 5: return launch_contacts_k<5>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 6: return launch_contacts_k<6>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 7: return launch_contacts_k<7>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    default: return launch_contacts_k<8>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+  }
+}
+
+// Largest d2 with sqrt(d2) <= c (correctly rounded sqrt), i.e. the exact
+// FP64 form of clustering.rs:99 `dist <= contact_distance`.
+double contact_d2_threshold(double c) {
+  double t = c * c;
+  while (std::sqrt(t) > c) t = std::nextafter(t, -INFINITY);
+  while (std::sqrt(std::nextafter(t, INFINITY)) <= c) t = std::nextafter(t, INFINITY);
+  return t;
+}
+
+// Smallest c with (double)c / (double)s >= cut, for s = 1..max (clustering.rs:127-129,171-174).
+void fcc_minima(double cut, int max_size, std::vector<int32_t>* out) {
+  out->assign(static_cast<size_t>(max_size) + 1, INT_MAX);
+  for (int s = 1; s <= max_size; ++s) {
+    int c = static_cast<int>(std::floor(cut * s)) - 2;
+    if (c < 0) c = 0;
+    while (c <= s && !(static_cast<double>(c) / static_cast<double>(s) >= cut)) ++c;
+    (*out)[s] = c <= s ? c : INT_MAX;
+  }
+}
+
+}  // namespace gdock
+
+using namespace gdock;
+
+extern "C" {
+
+int32_t gdock_contact_words(const gdock_complex* cx) { return cx && cx->con.supported ? cx->con.words64 : 0; }
+
+int32_t gdock_contact_candidates(const gdock_complex* cx) {
+  return cx && cx->con.supported ? cx->con.n_rec_res * cx->con.n_lig_res : 0;
+}
+
+gdock_status gdock_contact_pair(const gdock_complex* cx, int32_t bit, char* chain_a, int16_t* res_a, char* chain_b,
+                                int16_t* res_b) {
+  if (!cx || !cx->con.supported || bit < 0 || bit >= cx->con.n_rec_res * cx->con.n_lig_res) return GDOCK_ERR_INVALID_ARG;
+  const auto& r = cx->rec_res_key[bit / cx->con.n_lig_res];
+  const auto& l = cx->lig_res_key[bit % cx->con.n_lig_res];
+  const bool rf = r.first < l.first;  // smaller chain first (clustering.rs:101-105)
+  *chain_a = rf ? r.first : l.first; *res_a = rf ? r.second : l.second;
+  *chain_b = rf ? l.first : r.first; *res_b = rf ? l.second : r.second;
+  return GDOCK_OK;
+}
+
+static gdock_status upload_poses(gdock_ctx* ctx, const double* genes, int32_t n, double* d_poses) {
+  std::vector<double> poses(static_cast<size_t>(n) * kPoseDoubles);
+  if (gdock_prepare_poses(genes, n, poses.data()) != GDOCK_OK) {
+    set_error(ctx, "non-finite gene");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipMemcpyAsync(d_poses, poses.data(), poses.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // `poses` is freed on return
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "upload poses");
+}
+
+gdock_status gdock_contacts(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n, double contact_distance,
+                            uint64_t* bitsets, int32_t* set_sizes) {
+  if (!ctx || !cx || n < 0 || (n && (!genes || !bitsets))) return GDOCK_ERR_INVALID_ARG;
+  if (!cx->con.supported) { set_error(ctx, "%s", cx->con_unsupported.c_str()); return GDOCK_ERR_INVALID_ARG; }
+  if (!(contact_distance >= 0.0) || !std::isfinite(contact_distance)) { set_error(ctx, "contacts: bad cutoff"); return GDOCK_ERR_INVALID_ARG; }
+  if (n == 0) return GDOCK_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "contacts: set device");
+  Layout L;
+  const size_t o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
+               o_b = L.add<uint64_t>(static_cast<size_t>(n) * cx->con.words64), o_s = L.add<int32_t>(n);
+  if (!ctx->fcc.reserve(L.size, &e)) return hip_fail(ctx, e, "contacts: scratch");
+  char* m = static_cast<char*>(ctx->fcc.ptr);
+  gdock_status st = upload_poses(ctx, genes, n, reinterpret_cast<double*>(m + o_p));
+  if (st != GDOCK_OK) return st;
+  e = launch_contacts(cx, reinterpret_cast<double*>(m + o_p), n, contact_d2_threshold(contact_distance),
+                      reinterpret_cast<unsigned long long*>(m + o_b), reinterpret_cast<int32_t*>(m + o_s), ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "contacts: launch");
+  e = hipMemcpyAsync(bitsets, m + o_b, static_cast<size_t>(n) * cx->con.words64 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && set_sizes) e = hipMemcpyAsync(set_sizes, m + o_s, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "contacts: download");
+}
+
+gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                                      const gdock_clustering_config* cfg, gdock_cluster_result* out, int64_t* members,
+                                      int32_t* n_clusters) {
+  if (!ctx || !cx || !cfg || !n_clusters || n < 0 || (n && (!genes || !out || !members))) return GDOCK_ERR_INVALID_ARG;
+  *n_clusters = 0;
+  if (n == 0) return GDOCK_OK;  // clustering.rs:269-271
+  if (!cx->con.supported) { set_error(ctx, "%s", cx->con_unsupported.c_str()); return GDOCK_ERR_INVALID_ARG; }
+  if (n > kGreedyMaxN) { set_error(ctx, "cluster_structures: %d structures exceeds the supported %d", n, kGreedyMaxN); return GDOCK_ERR_INVALID_ARG; }
+  if (!std::isfinite(cfg->contact_distance) || cfg->contact_distance < 0.0 || !std::isfinite(cfg->fcc_cutoff) ||
+      !std::isfinite(cfg->strictness)) {
+    set_error(ctx, "cluster_structures: bad config");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: set device");
+  const int W64 = cx->con.words64, W32 = 2 * W64;
+  const int n_pad = (n + 63) & ~63;
+  const int nwords = (n + 63) / 64;
+  const int max_set = cx->con.n_rec_res * cx->con.n_lig_res;
+  Layout L;
+  const size_t o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
+               o_b = L.add<uint64_t>(static_cast<size_t>(n) * W64), o_s = L.add<int32_t>(n_pad),
+               o_u = L.add<uint32_t>(W32), o_col = L.add<int32_t>(W32),
+               o_t = L.add<uint32_t>(static_cast<size_t>(W32) * n_pad), o_ca = L.add<int32_t>(max_set + 1),
+               o_cb = L.add<int32_t>(max_set + 1), o_A = L.add<uint64_t>(static_cast<size_t>(n) * nwords),
+               o_AT = L.add<uint64_t>(static_cast<size_t>(n) * nwords), o_cc = L.add<int64_t>(n), o_cs = L.add<int64_t>(n),
+               o_co = L.add<int64_t>(n), o_cm = L.add<int64_t>(n), o_cn = L.add<int32_t>(1);
+  if (!ctx->fcc.reserve(L.size, &e)) return hip_fail(ctx, e, "cluster: scratch");
+  char* m = static_cast<char*>(ctx->fcc.ptr);
+  auto P = [&](size_t o) { return static_cast<void*>(m + o); };
+  gdock_status st = upload_poses(ctx, genes, n, static_cast<double*>(P(o_p)));
+  if (st != GDOCK_OK) return st;
+  e = launch_contacts(cx, static_cast<double*>(P(o_p)), n, contact_d2_threshold(cfg->contact_distance),
+                      static_cast<unsigned long long*>(P(o_b)), static_cast<int32_t*>(P(o_s)), ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: contacts");
+  hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256), dim3(256), 0, ctx->stream,
+                     static_cast<const uint32_t*>(P(o_b)), n, W32, static_cast<uint32_t*>(P(o_u)));
+  std::vector<uint32_t> uni(W32);
+  e = hipMemcpyAsync(uni.data(), P(o_u), W32 * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
+  std::vector<int32_t> col;
+  for (int w = 0; w < W32; ++w) if (uni[w]) col.push_back(w);
+  const int wc = static_cast<int>(col.size());
+  std::vector<int32_t> ca, cb;
+  fcc_minima(cfg->fcc_cutoff, max_set, &ca);
+  fcc_minima(cfg->fcc_cutoff * cfg->strictness, max_set, &cb);
+  const int zero_ok = (0.0 >= cfg->fcc_cutoff && 0.0 >= cfg->fcc_cutoff * cfg->strictness) ? 1 : 0;
+  e = hipMemcpyAsync(P(o_col), col.data(), std::max(1, wc) * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(P(o_ca), ca.data(), ca.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(P(o_cb), cb.data(), cb.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: tables");
+  if (wc > 0)
+    hipLaunchKernelGGL(compact_kernel, dim3(n_pad / 256 + 1, wc), dim3(256), 0, ctx->stream,
+                       static_cast<const uint32_t*>(P(o_b)), n, n_pad, W32, static_cast<const int32_t*>(P(o_col)), wc,
+                       static_cast<uint32_t*>(P(o_t)));
+  hipLaunchKernelGGL(overlap_kernel, dim3(nwords, nwords), dim3(256), 0, ctx->stream,
+                     static_cast<const uint32_t*>(P(o_t)), n, n_pad, wc, static_cast<const int32_t*>(P(o_s)),
+                     static_cast<const int32_t*>(P(o_ca)), static_cast<const int32_t*>(P(o_cb)), zero_ok,
+                     static_cast<unsigned long long*>(P(o_A)), static_cast<unsigned long long*>(P(o_AT)), nwords);
+  hipLaunchKernelGGL(greedy_kernel, dim3(1), dim3(kGreedyThreads), static_cast<size_t>(nwords) * 8 + static_cast<size_t>(n) * 4,
+                     ctx->stream, static_cast<const unsigned long long*>(P(o_A)),
+                     static_cast<const unsigned long long*>(P(o_AT)), n, nwords,
+                     static_cast<long long>(cfg->min_cluster_size), static_cast<int64_t*>(P(o_cc)),
+                     static_cast<int64_t*>(P(o_cs)), static_cast<int64_t*>(P(o_co)), static_cast<int64_t*>(P(o_cm)),
+                     static_cast<int32_t*>(P(o_cn)));
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: launch");
+  int32_t nclu = 0;
+  e = hipMemcpyAsync(&nclu, P(o_cn), 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: result");
+  std::vector<int64_t> cc(nclu), cs(nclu), co(nclu);
+  if (nclu > 0) {
+    e = hipMemcpyAsync(cc.data(), P(o_cc), nclu * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cs.data(), P(o_cs), nclu * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(co.data(), P(o_co), nclu * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cluster: result");
+  }
+  // clustering.rs:250-251 stable sort by size (descending), then 288-292 filter
+  std::vector<int> order(nclu);
+  for (int i = 0; i < nclu; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cs[a] > cs[b]; });
+  std::vector<int64_t> all_members(static_cast<size_t>(co.empty() ? 0 : co.back() + cs.back()));
+  if (!all_members.empty()) {
+    e = hipMemcpy(all_members.data(), P(o_cm), all_members.size() * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cluster: members");
+  }
+  int32_t k = 0;
+  int64_t off = 0;
+  for (int i : order) {
+    if (cs[i] < cfg->min_cluster_size) continue;
+    out[k].center_idx = cc[i];
+    out[k].size = cs[i];
+    out[k].members_offset = off;
+    std::memcpy(members + off, all_members.data() + co[i], cs[i] * 8);
+    off += cs[i];
+    ++k;
+  }
+  *n_clusters = k;
+  return GDOCK_OK;
+}
+
+}  // extern "C"

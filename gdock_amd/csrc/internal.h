@@ -1,0 +1,88 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+
+namespace gdock {
+
+struct DeviceArena {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t bytes, hipError_t* err);
+  void release();
+};
+
+// Device tables of the inter-chain residue contact candidates of a complex.
+struct DevContacts {
+  const float4* rec_xyz;      // receptor heavy atoms grouped by residue (FP32, origin frame)
+  const double* rec_x;        // FP64 coordinates of the same atoms
+  const double* rec_y;
+  const double* rec_z;
+  const int32_t* rec_res_start;  // n_rec_res + 1
+  const int32_t* lig_atom;    // ligand heavy atom -> ligand atom index
+  const int32_t* lig_res;     // ligand heavy atom -> ligand residue index
+  int32_t n_rec_heavy, n_lig_heavy, n_rec_res, n_lig_res;
+  int32_t words64;            // uint64 words per pose bitset (n_rec_res * n_lig_res bits)
+  int32_t supported;          // 0: topology outside the device contact path
+};
+
+}  // namespace gdock
+
+struct gdock_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  gdock::DeviceArena staging;   // per-call poses / outputs
+  gdock::DeviceArena fcc;       // FCC bitsets, overlap and cluster scratch
+  void* pinned = nullptr;       // pinned host staging
+  size_t pinned_cap = 0;
+};
+
+struct gdock_complex {
+  gdock::DevComplex dev{};
+  gdock::DevContacts con{};
+  void* mem = nullptr;
+  size_t mem_bytes = 0;
+  std::vector<gdock_atom> rec, lig;
+  int32_t n_restr = 0;
+  // contact tables (host)
+  size_t contacts_layout_begin = 0;
+  std::vector<size_t> con_offsets;
+  std::vector<std::pair<char, int16_t>> rec_res_key, lig_res_key;
+  std::string con_unsupported;
+};
+
+namespace gdock {
+
+void set_error(gdock_ctx* ctx, const char* fmt, ...);
+gdock_status hip_fail(gdock_ctx* ctx, hipError_t e, const char* what);
+std::string_view field(const char* f, size_t n);
+
+void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::vector<int32_t>* offsets);
+
+hipError_t launch_score(const DevComplex& cx, const double* d_poses, const double* d_coords,
+                        const double* d_extent, int n_poses, const Weights& w, double* d_out,
+                        int32_t* d_nsat, hipStream_t stream);
+
+// Host-side bump layout of one device allocation (256-byte aligned pieces).
+struct Layout {
+  size_t size = 0;
+  template <class T>
+  size_t add(size_t count) {
+    const size_t off = (size + 255) & ~static_cast<size_t>(255);
+    size = off + count * sizeof(T);
+    return off;
+  }
+};
+void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host);
+void bind_contact_tables(gdock_complex* cx);
+void release_fcc_scratch(gdock_ctx* ctx);
+
+}  // namespace gdock

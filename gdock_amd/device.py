@@ -1,0 +1,160 @@
+"""Device context and device-resident complexes (gdock_ctx / gdock_complex)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import constants
+from ._lib import (ChromosomeC, ClusterResultC, ClusteringConfigC, Weights, check, lib, ptr)
+from .structure import Molecule
+
+_contexts: dict[int, "Context"] = {}
+
+
+class Context:
+    """One device, one stream, reusable staging memory."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().gdock_ctx_create(device, C.byref(h))
+        check(st, what=f"gdock_ctx_create(device={device})")
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gdock_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+def weights_c(w: constants.EnergyWeights | None) -> Weights:
+    w = w or constants.EnergyWeights()
+    return Weights(w.vdw, w.elec, w.desolv, w.air)
+
+
+class Complex:
+    """Receptor + positioned ligand + resolved restraints, resident on the device."""
+
+    def __init__(self, receptor: Molecule, ligand: Molecule, restraint_pairs=(), ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.receptor, self.ligand = receptor, ligand
+        pairs = np.ascontiguousarray(np.array(list(restraint_pairs), dtype=np.int32).reshape(-1, 2))
+        self._pairs = pairs
+        h = C.c_void_p()
+        st = lib().gdock_complex_create(self.ctx.handle, receptor.atoms.ctypes.data, len(receptor),
+                                        ligand.atoms.ctypes.data, len(ligand), ptr(pairs, C.c_int32),
+                                        len(pairs), C.byref(h))
+        check(st, self.ctx.handle, "gdock_complex_create")
+        self.handle = h
+
+    @property
+    def n_restraints(self) -> int:
+        return lib().gdock_complex_n_restraints(self.handle)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().gdock_complex_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scoring -----------------------------------------------------------
+    def score_poses(self, genes: np.ndarray, weights: constants.EnergyWeights | None = None):
+        """genes (n, 6) -> (out (n, 6) = vdw, elec, desolv, air, restraint_penalty, fitness; n_satisfied)."""
+        g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros((len(g), 6))
+        ns = np.zeros(len(g), np.int32)
+        w = weights_c(weights)
+        check(lib().gdock_score_poses(self.ctx.handle, self.handle, ptr(g), len(g), C.byref(w), ptr(out),
+                                      ptr(ns, C.c_int32)), self.ctx.handle, "gdock_score_poses")
+        return out, ns
+
+    def score_coords(self, lig_xyz: np.ndarray, weights: constants.EnergyWeights | None = None):
+        """Ligand coordinates (n, n_lig, 3) scored as given (commands/score.rs)."""
+        x = np.ascontiguousarray(lig_xyz, dtype=np.float64).reshape(-1, len(self.ligand), 3)
+        out = np.zeros((len(x), 6))
+        ns = np.zeros(len(x), np.int32)
+        w = weights_c(weights)
+        check(lib().gdock_score_coords(self.ctx.handle, self.handle, ptr(x), len(x), C.byref(w), ptr(out),
+                                       ptr(ns, C.c_int32)), self.ctx.handle, "gdock_score_coords")
+        return out, ns
+
+    def eval_fitness(self, chromosomes, weights: constants.EnergyWeights | None = None) -> None:
+        arr = (ChromosomeC * len(chromosomes))()
+        for i, c in enumerate(chromosomes):
+            arr[i].genes[:] = list(c.genes)
+        w = weights_c(weights)
+        check(lib().gdock_eval_fitness(self.ctx.handle, self.handle, arr, len(chromosomes), C.byref(w)),
+              self.ctx.handle, "gdock_eval_fitness")
+        for i, c in enumerate(chromosomes):
+            a = arr[i]
+            c.fitness, c.vdw, c.elec, c.desolv, c.air = a.fitness, a.vdw, a.elec, a.desolv, a.air
+            c.restraint_penalty = a.restraint_penalty
+
+    # -- FCC -----------------------------------------------------------------
+    @property
+    def contact_words(self) -> int:
+        return lib().gdock_contact_words(self.handle)
+
+    def contact_pair(self, bit: int) -> tuple[str, int, str, int]:
+        ca, cb = C.create_string_buffer(1), C.create_string_buffer(1)
+        ra, rb = C.c_int16(), C.c_int16()
+        check(lib().gdock_contact_pair(self.handle, bit, ca, C.byref(ra), cb, C.byref(rb)), what="contact_pair")
+        return ca.raw.decode(), ra.value, cb.raw.decode(), rb.value
+
+    def contacts(self, genes: np.ndarray, contact_distance: float = 5.0):
+        """(bitsets (n, words) uint64, set sizes (n,))."""
+        g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
+        w = self.contact_words
+        bits = np.zeros((len(g), max(w, 1)), np.uint64)
+        sizes = np.zeros(len(g), np.int32)
+        check(lib().gdock_contacts(self.ctx.handle, self.handle, ptr(g), len(g), contact_distance,
+                                   ptr(bits, C.c_uint64), ptr(sizes, C.c_int32)), self.ctx.handle, "gdock_contacts")
+        return bits[:, :w], sizes
+
+    def contact_set(self, bitset: np.ndarray) -> set[str]:
+        """Decode one pose's bitset into the reference's contact strings."""
+        out = set()
+        for wi, word in enumerate(bitset.tolist()):
+            while word:
+                b = (word & -word).bit_length() - 1
+                word &= word - 1
+                ca, ra, cb, rb = self.contact_pair(wi * 64 + b)
+                out.add(f"{ca} {ra} {cb} {rb}")
+        return out
+
+    def cluster(self, genes: np.ndarray, config=None):
+        from .clustering import ClusterResult, ClusteringConfig
+        cfg = config or ClusteringConfig()
+        g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
+        n = len(g)
+        res = (ClusterResultC * max(n, 1))()
+        members = np.zeros(max(n, 1), np.int64)
+        nc = C.c_int32()
+        cc = ClusteringConfigC(cfg.contact_distance, cfg.fcc_cutoff, cfg.strictness, cfg.min_cluster_size)
+        check(lib().gdock_cluster_structures(self.ctx.handle, self.handle, ptr(g), n, C.byref(cc), res,
+                                             ptr(members, C.c_int64), C.byref(nc)), self.ctx.handle,
+              "gdock_cluster_structures")
+        out = []
+        for k in range(nc.value):
+            r = res[k]
+            m = members[r.members_offset:r.members_offset + r.size].tolist()
+            out.append(ClusterResult(r.center_idx, m, r.size))
+        return out

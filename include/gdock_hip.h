@@ -1,0 +1,161 @@
+/*
+ * gdock_hip.h — C ABI of the MI355X (gfx950) population-scoring and FCC
+ * clustering library.  Plain pointers and sizes only; no HIP or torch types
+ * cross this boundary ("stream" arguments are hipStream_t passed as void*).
+ *
+ * Every entry point returns a gdock_status; on failure the context's
+ * gdock_last_error() holds a message.  Nothing aborts across the ABI.
+ * Host arrays passed in are copied; output arrays are caller-allocated.
+ * A context is not thread safe: use one per host thread (or lock around it).
+ *
+ * Reference interfaces replaced (gdock 2.0.0-rc.2, Rust):
+ *   Population::eval_fitness ............ src/population.rs:48-61
+ *   Chromosome::fitness / apply_genes ... src/chromosome.rs:79-128
+ *   fitness::{vdw,elec,desolv,air}_energy, satisfaction_ratio
+ *                                         src/fitness.rs:86,122,205,251,284
+ *   restraints::create_restraints_from_pairs  src/restraints.rs:44-77
+ *   clustering::{calculate_contacts, cluster_structures}
+ *                                         src/clustering.rs:62,265
+ *   structure::read_pdb (+ toppar)  ..... src/structure.rs:187-300
+ *   utils::position_ligand .............. src/utils.rs:77-95
+ */
+#ifndef GDOCK_HIP_H
+#define GDOCK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gdock_status {
+  GDOCK_OK = 0,
+  GDOCK_ERR_INVALID_ARG = 1, /* bad size, null pointer, NaN gene, unknown handle */
+  GDOCK_ERR_OOM = 2,         /* device allocation failed / over the memory budget */
+  GDOCK_ERR_DEVICE = 3,      /* HIP runtime error */
+  GDOCK_ERR_BUSY = 4,        /* admission refused: no slot or memory right now */
+  GDOCK_ERR_NOT_FOUND = 5    /* unknown job id */
+} gdock_status;
+
+/* One parameterized PDB ATOM record; mirrors structure::Atom
+ * (src/structure.rs:28-49).  Strings are trimmed and NUL-terminated. */
+typedef struct gdock_atom {
+  double x, y, z;     /* Angstrom, FP64 */
+  double charge;      /* e */
+  double vdw_radius;  /* Angstrom (H 1.2, C 1.7, N 1.55, O 1.52, else 1.0) */
+  double epsilon;     /* kcal/mol, <= 0 */
+  double rmin2;       /* Angstrom */
+  int32_t serial;
+  int16_t resseq;
+  char chainid, altloc, icode;
+  char name[5];       /* atom name, e.g. "CA" */
+  char resname[5];
+  char element[5];    /* inferred from name[0] when the PDB column is blank */
+} gdock_atom;
+
+/* constants::EnergyWeights (src/constants.rs:28-34); defaults 0.4/0.05/3.4/1.0 */
+typedef struct gdock_weights { double vdw, elec, desolv, air; } gdock_weights;
+
+/* chromosome::Chromosome (src/chromosome.rs:12-24): genes = alpha, beta,
+ * gamma (rad), dx, dy, dz (Angstrom).  eval_fitness fills the energy fields. */
+typedef struct gdock_chromosome {
+  double genes[6];
+  double fitness, vdw, elec, desolv, air, restraint_penalty, rmsd, fnat;
+} gdock_chromosome;
+
+/* clustering::ClusteringConfig (src/clustering.rs:15-35); defaults 5.0/0.60/0.75/4 */
+typedef struct gdock_clustering_config {
+  double contact_distance, fcc_cutoff, strictness;
+  int64_t min_cluster_size;
+} gdock_clustering_config;
+
+/* clustering::ClusterResult (src/clustering.rs:39-46); members live in a
+ * caller array at [members_offset, members_offset + size). */
+typedef struct gdock_cluster_result {
+  int64_t center_idx, size, members_offset;
+} gdock_cluster_result;
+
+typedef struct gdock_ctx gdock_ctx;         /* one device + stream + scratch */
+typedef struct gdock_complex gdock_complex; /* receptor + positioned ligand, device resident */
+
+const char* gdock_version(void);
+gdock_status gdock_ctx_create(int device, gdock_ctx** out);
+void gdock_ctx_destroy(gdock_ctx* ctx);
+const char* gdock_last_error(const gdock_ctx* ctx);
+
+/* ---- input formats (host) ------------------------------------------------ */
+/* structure::read_pdb over text (src/structure.rs:187-300): parses ATOM
+ * records, infers elements and radii and looks up the OPLS parameters
+ * (src/toppar.rs).  *atoms holds every model back to back; model m spans
+ * [(*model_offsets)[m], (*model_offsets)[m+1]).  Free both with gdock_free. */
+gdock_status gdock_read_pdb(const char* text, size_t len, gdock_atom** atoms,
+                            int32_t** model_offsets, int32_t* n_models);
+void gdock_free(void* p);
+/* utils::position_ligand (src/utils.rs:77-95), in place on lig. */
+gdock_status gdock_position_ligand(const gdock_atom* rec, int32_t n_rec, gdock_atom* lig,
+                                   int32_t n_lig);
+
+/* ---- complexes ------------------------------------------------------------- */
+/* Uploads receptor and (already positioned) ligand and resolves restraint
+ * residue pairs (receptor resseq, ligand resseq) to their first "CA" atoms as
+ * restraints::create_restraints_from_pairs does; unresolved pairs are dropped
+ * (src/restraints.rs:68-73).  n_restraints() reports how many were kept. */
+gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t n_rec,
+                                  const gdock_atom* lig, int32_t n_lig,
+                                  const int32_t* restraint_pairs, int32_t n_pairs,
+                                  gdock_complex** out);
+void gdock_complex_destroy(gdock_ctx* ctx, gdock_complex* cx);
+int32_t gdock_complex_n_restraints(const gdock_complex* cx);
+
+/* ---- population scoring (the drop-in for Population::eval_fitness) ------- */
+/* Scores every chromosome in place: vdw, elec, desolv, air, restraint_penalty
+ * and fitness, as Chromosome::fitness (src/chromosome.rs:85-128, normal mode).
+ * Synchronous. */
+gdock_status gdock_eval_fitness(gdock_ctx* ctx, gdock_complex* cx, gdock_chromosome* chromosomes,
+                                int32_t n, const gdock_weights* w);
+
+/* Flat form.  genes: n x 6.  out: n x 6 = {vdw, elec, desolv, air,
+ * restraint_penalty, fitness}.  n_satisfied (optional): satisfied restraints. */
+gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                               const gdock_weights* w, double* out, int32_t* n_satisfied);
+
+/* commands::score (src/commands/score.rs:32-49): scores ligand coordinates as
+ * given (no transform) for n models of the complex's ligand topology.
+ * lig_xyz: n x n_lig x 3 FP64; restraints resolve as for the complex. */
+gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double* lig_xyz, int32_t n,
+                                const gdock_weights* w, double* out, int32_t* n_satisfied);
+
+/* Device-resident form for callers that keep populations in HBM.  A pose
+ * record is 12 doubles: the 6 genes, then cos/sin of alpha, beta, gamma as
+ * gdock_prepare_poses computes them on the host (glibc, like the reference).
+ * d_poses, d_out (n x 6) and d_nsat (n) are device pointers; the launch is
+ * queued on `stream` (hipStream_t, 0 = the context's stream) and not waited on. */
+gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses);
+gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const double* d_poses,
+                                      int32_t n, const gdock_weights* w, double* d_out,
+                                      int32_t* d_nsat, void* stream);
+
+/* ---- FCC clustering ---------------------------------------------------------- */
+/* Contact bitsets (clustering::calculate_contacts, src/clustering.rs:62-115) of
+ * the complexes receptor + (ligand moved by genes i).  Bit b of a pose's set
+ * stands for the residue pair gdock_contact_pair(cx, b).  bitsets: n x
+ * gdock_contact_words(cx) uint64; set_sizes (optional): n. */
+int32_t gdock_contact_words(const gdock_complex* cx);
+int32_t gdock_contact_candidates(const gdock_complex* cx);
+gdock_status gdock_contact_pair(const gdock_complex* cx, int32_t bit, char* chain_a, int16_t* res_a,
+                                char* chain_b, int16_t* res_b);
+gdock_status gdock_contacts(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                            double contact_distance, uint64_t* bitsets, int32_t* set_sizes);
+
+/* clustering::cluster_structures (src/clustering.rs:265-293) over the
+ * complexes receptor + (ligand moved by genes i), i < n.  out: up to n
+ * results in the reference's order; members: up to n indices. */
+gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                                      const gdock_clustering_config* cfg, gdock_cluster_result* out,
+                                      int64_t* members, int32_t* n_clusters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDOCK_HIP_H */

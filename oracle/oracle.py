@@ -1,0 +1,286 @@
+"""ctypes access to the FP64 oracle (test infrastructure only).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the product package.  Records use the same 80-byte layout as
+gdock_atom so that the same numpy arrays feed both sides.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libgdock_oracle.so"
+
+ATOM_DTYPE = np.dtype(
+    [("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("charge", "<f8"), ("vdw_radius", "<f8"), ("epsilon", "<f8"),
+     ("rmin2", "<f8"), ("serial", "<i4"), ("resseq", "<i2"), ("chainid", "S1"), ("altloc", "S1"), ("icode", "S1"),
+     ("name", "S5"), ("resname", "S5"), ("element", "S5")], align=True)
+
+_lib = None
+P, D, I32, I64 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        sig = {
+            "or_pdb_count_models": (C.c_int, [C.c_char_p]),
+            "or_pdb_model": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int]),
+            "or_position_ligand": (None, [P, C.c_int, P, C.c_int, P]),
+            "or_apply_genes": (None, [P, C.c_int, D, P]),
+            "or_center_of_mass": (None, [P, C.c_int, D]),
+            "or_vdw_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+            "or_elec_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+            "or_desolv_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+            "or_air_energy": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+            "or_satisfaction_ratio": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+            "or_restraints_resolved": (C.c_int, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+            "or_score_poses": (None, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_int, D, C.c_int, C.c_int, D, I32]),
+            "or_burial_counts": (None, [P, C.c_int, P, C.c_int, D, I32, I32]),
+            "or_contacts": (C.c_int, [P, C.c_int, C.c_double, I32, C.c_int]),
+            "or_calculate_fcc": (None, [I32, C.c_int, I32, C.c_int, D]),
+            "or_cluster_elements": (C.c_int, [C.POINTER(C.c_uint8), C.c_int, C.c_int, I64, I64, I64]),
+            "or_cluster_poses": (C.c_int, [P, C.c_int, P, C.c_int, D, C.c_int, D, C.c_int, I64, I64, I64, I32]),
+            "or_ga_run": (P, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "or_ga_free": (None, [P]),
+            "or_ga_generations": (C.c_int, [P]),
+            "or_ga_converged_at": (C.c_int, [P]),
+            "or_ga_trace": (None, [P, D, D, I32]),
+            "or_ga_hof": (C.c_int, [P, D, D, I32]),
+            "or_ga_clusters": (C.c_int, [P, I64, I64]),
+            "or_ga_selected": (C.c_int, [P, I64, I64]),
+            "or_hof_new": (P, [C.c_int]),
+            "or_hof_try_add": (C.c_int, [P, D, C.c_double, C.c_int]),
+            "or_hof_entries": (C.c_int, [P, D]),
+            "or_hof_angular_difference": (C.c_double, [C.c_double, C.c_double]),
+            "or_hof_free": (None, [P]),
+            "or_rng_stream": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
+            "or_evolve": (None, [D, D, C.c_int, C.c_uint64, D, D]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _v(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _pairs(pairs):
+    return np.ascontiguousarray(np.array(list(pairs), np.int32).reshape(-1, 2))
+
+
+def read_pdb_text(text: str) -> list[np.ndarray]:
+    raw = text.encode()
+    L = lib()
+    out = []
+    for m in range(L.or_pdb_count_models(raw)):
+        n = L.or_pdb_model(raw, m, None, 0)
+        a = np.zeros(n, ATOM_DTYPE)
+        L.or_pdb_model(raw, m, _v(a), n)
+        out.append(a)
+    return out
+
+
+def position_ligand(rec: np.ndarray, lig: np.ndarray) -> np.ndarray:
+    out = np.zeros_like(lig)
+    lib().or_position_ligand(_v(rec), len(rec), _v(lig), len(lig), _v(out))
+    return out
+
+
+def apply_genes(lig: np.ndarray, genes) -> np.ndarray:
+    g = np.ascontiguousarray(genes, np.float64)
+    out = np.zeros_like(lig)
+    lib().or_apply_genes(_v(lig), len(lig), _p(g), _v(out))
+    return out
+
+
+def center_of_mass(m: np.ndarray) -> tuple:
+    o = np.zeros(3)
+    lib().or_center_of_mass(_v(m), len(m), _p(o))
+    return tuple(o)
+
+
+def vdw_energy(r, l):
+    return lib().or_vdw_energy(_v(r), len(r), _v(l), len(l))
+
+
+def elec_energy(r, l):
+    return lib().or_elec_energy(_v(r), len(r), _v(l), len(l))
+
+
+def desolv_energy(r, l):
+    return lib().or_desolv_energy(_v(r), len(r), _v(l), len(l))
+
+
+def air_energy(pairs, r, l):
+    p = _pairs(pairs)
+    return lib().or_air_energy(_v(r), len(r), _v(l), len(l), _p(p, C.c_int32), len(p))
+
+
+def satisfaction_ratio(pairs, r, l):
+    p = _pairs(pairs)
+    return lib().or_satisfaction_ratio(_v(r), len(r), _v(l), len(l), _p(p, C.c_int32), len(p))
+
+
+def restraints_resolved(pairs, r, l):
+    p = _pairs(pairs)
+    return lib().or_restraints_resolved(_v(r), len(r), _v(l), len(l), _p(p, C.c_int32), len(p))
+
+
+def score_poses(rec, lig, pairs, genes, weights=(0.4, 0.05, 3.4, 1.0), transform=True, nthreads=8):
+    """Population::eval_fitness in FP64: (out (n, 6), n_satisfied (n,))."""
+    p = _pairs(pairs)
+    g = np.ascontiguousarray(genes, np.float64).reshape(-1, 6)
+    w = np.ascontiguousarray(weights, np.float64)
+    out = np.zeros((len(g), 6))
+    ns = np.zeros(len(g), np.int32)
+    lib().or_score_poses(_v(rec), len(rec), _v(lig), len(lig), _p(p, C.c_int32), len(p), _p(g), len(g), _p(w),
+                         1 if transform else 0, nthreads, _p(out), _p(ns, C.c_int32))
+    return out, ns
+
+
+def burial_counts(rec, lig, genes=None):
+    rc = np.zeros(len(rec), np.int32)
+    lc = np.zeros(len(lig), np.int32)
+    g = None if genes is None else np.ascontiguousarray(genes, np.float64)
+    lib().or_burial_counts(_v(rec), len(rec), _v(lig), len(lig), None if g is None else _p(g), _p(rc, C.c_int32),
+                           _p(lc, C.c_int32))
+    return rc, lc
+
+
+def contacts(mol: np.ndarray, cutoff=5.0) -> set[str]:
+    n = lib().or_contacts(_v(mol), len(mol), cutoff, None, 0)
+    buf = np.zeros((max(n, 1), 4), np.int32)
+    lib().or_contacts(_v(mol), len(mol), cutoff, _p(buf, C.c_int32), n)
+    return {f"{chr(a)} {b} {chr(c)} {d}" for a, b, c, d in buf[:n].tolist()}
+
+
+def calculate_fcc(x: set[str], y: set[str]):
+    def enc(s):
+        rows = []
+        for t in s:
+            ca, ra, cb, rb = t.split()
+            rows.append((ord(ca), int(ra), ord(cb), int(rb)))
+        return np.ascontiguousarray(np.array(rows, np.int32).reshape(-1, 4))
+    a, b = enc(x), enc(y)
+    o = np.zeros(2)
+    lib().or_calculate_fcc(_p(a, C.c_int32), len(a), _p(b, C.c_int32), len(b), _p(o))
+    return o[0], o[1]
+
+
+def cluster_elements(nb: np.ndarray, min_size: int):
+    n = len(nb)
+    m = np.ascontiguousarray(nb, np.uint8)
+    cen = np.zeros(max(n, 1), np.int64)
+    off = np.zeros(n + 1, np.int64)
+    mem = np.zeros(max(n, 1), np.int64)
+    k = lib().or_cluster_elements(_p(m, C.c_uint8), n, min_size, _p(cen, C.c_int64), _p(off, C.c_int64),
+                                  _p(mem, C.c_int64))
+    return [(int(cen[i]), mem[off[i]:off[i + 1]].tolist()) for i in range(k)]
+
+
+def cluster_poses(rec, lig, genes, config=(5.0, 0.60, 0.75, 4), nthreads=8):
+    """cluster_structures over receptor + (lig moved by genes i): ([(center, members)], set sizes)."""
+    g = np.ascontiguousarray(genes, np.float64).reshape(-1, 6)
+    n = len(g)
+    cfg = np.ascontiguousarray(config, np.float64)
+    cen = np.zeros(max(n, 1), np.int64)
+    off = np.zeros(n + 1, np.int64)
+    mem = np.zeros(max(n, 1), np.int64)
+    sizes = np.zeros(max(n, 1), np.int32)
+    k = lib().or_cluster_poses(_v(rec), len(rec), _v(lig), len(lig), _p(g), n, _p(cfg), nthreads,
+                               _p(cen, C.c_int64), _p(off, C.c_int64), _p(mem, C.c_int64), _p(sizes, C.c_int32))
+    return [(int(cen[i]), mem[off[i]:off[i + 1]].tolist()) for i in range(k)], sizes[:n]
+
+
+class GaRun:
+    """Full GA run (run.rs:361-733 restated) with its per-generation trace."""
+
+    def __init__(self, rec, lig_positioned, pairs, weights=(0.4, 0.05, 3.4, 1.0), seed=42, pop_size=150,
+                 max_gen=250, cluster=True, nthreads=8):
+        p = _pairs(pairs)
+        w = np.ascontiguousarray(weights, np.float64)
+        L = lib()
+        self.h = L.or_ga_run(_v(rec), len(rec), _v(lig_positioned), len(lig_positioned), _p(p, C.c_int32), len(p),
+                             _p(w), seed, pop_size, max_gen, 1 if cluster else 0, nthreads)
+        self.generations = L.or_ga_generations(self.h)
+        self.converged_at = L.or_ga_converged_at(self.h)
+        g = self.generations
+        self.genes = np.zeros((g, pop_size, 6))
+        self.fitness = np.zeros((g, pop_size))
+        self.best_idx = np.zeros(g, np.int32)
+        L.or_ga_trace(self.h, _p(self.genes), _p(self.fitness), _p(self.best_idx, C.c_int32))
+        nh = L.or_ga_hof(self.h, None, None, None)
+        self.hof_genes = np.zeros((nh, 6))
+        self.hof_fitness = np.zeros(nh)
+        self.hof_generation = np.zeros(nh, np.int32)
+        L.or_ga_hof(self.h, _p(self.hof_genes), _p(self.hof_fitness), _p(self.hof_generation, C.c_int32))
+        nc = L.or_ga_clusters(self.h, None, None)
+        self.cluster_center = np.zeros(nc, np.int64)
+        self.cluster_size = np.zeros(nc, np.int64)
+        L.or_ga_clusters(self.h, _p(self.cluster_center, C.c_int64), _p(self.cluster_size, C.c_int64))
+        si = np.zeros(5, np.int64)
+        ss = np.zeros(5, np.int64)
+        ns = L.or_ga_selected(self.h, _p(si, C.c_int64), _p(ss, C.c_int64))
+        self.selected = list(zip(si[:ns].tolist(), ss[:ns].tolist()))
+        L.or_ga_free(self.h)
+        self.h = None
+
+
+def rng_stream(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint64)
+    lib().or_rng_stream(seed, n, _p(out, C.c_uint64))
+    return out
+
+
+def evolve(genes, fitness, seed):
+    g = np.ascontiguousarray(genes, np.float64).reshape(-1, 6)
+    f = np.ascontiguousarray(fitness, np.float64)
+    og = np.zeros_like(g)
+    of = np.zeros_like(f)
+    lib().or_evolve(_p(g), _p(f), len(f), seed, _p(og), _p(of))
+    return og, of
+
+
+class Hof:
+    def __init__(self, max_size=500):
+        self.h = lib().or_hof_new(max_size)
+
+    def try_add(self, genes, fitness, gen):
+        g = np.ascontiguousarray(genes, np.float64)
+        return bool(lib().or_hof_try_add(self.h, _p(g), fitness, gen))
+
+    def fitness(self):
+        n = lib().or_hof_entries(self.h, None)
+        o = np.zeros(n)
+        lib().or_hof_entries(self.h, _p(o))
+        return o
+
+    def __del__(self):
+        try:
+            lib().or_hof_free(self.h)
+        except Exception:
+            pass
+
+
+def angular_difference(a, b):
+    return lib().or_hof_angular_difference(a, b)

@@ -1,0 +1,184 @@
+"""GPU parity of population scoring against the FP64 oracle.
+
+Bar: every component within max(1e-5 |ref|, 1e-3 kcal/mol) (helpers.within_tol);
+satisfied-restraint counts and desolvation burial decisions exact."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle.oracle as orc
+from gdock_amd import _lib, constants, structure, synth
+from gdock_amd.device import Complex
+from gdock_amd.population import Chromosome, Population
+from gdock_amd.restraints import Restraint
+from helpers import RESTRAINTS_2OOB, assert_energies_close, mol, read_text, test_atom as A
+
+pytestmark = pytest.mark.gpu
+W = (0.4, 0.05, 3.4, 1.0)
+
+
+def _pair(text_r, text_l, pairs=()):
+    rec = structure.read_pdb_text(text_r)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(text_l)[0])
+    return rec, lig, Complex(rec, lig, pairs)
+
+
+def _check_poses(rec, lig, cx, pairs, genes, what):
+    got, ns = cx.score_poses(genes)
+    ref, rns = orc.score_poses(rec.atoms, lig.atoms, pairs, genes, W)
+    assert_energies_close(got, ref, what)
+    assert np.array_equal(ns, rns), what
+    return got, ref
+
+
+def test_reference_unit_pairs():
+    """fitness.rs:357-607 set-ups, scored as given on the device."""
+    cases = [
+        (mol(A(0, 0, 0)), mol(A(20, 0, 0, serial=2))),
+        (mol(A(0, 0, 0)), mol(A(5, 0, 0, serial=2))),
+        (mol(A(0, 0, 0)), mol(A(1, 0, 0, serial=2))),
+        (mol(A(0, 0, 0, charge=1)), mol(A(5, 0, 0, charge=-1, serial=2))),
+        (mol(A(0, 0, 0, charge=1)), mol(A(5, 0, 0, charge=1, serial=2))),
+        (mol(A(0, 0, 0, element="C")), mol(*[A(3 * math.cos(i), 3 * math.sin(i), 0, serial=i + 2) for i in range(15)])),
+        (mol(A(0, 0, 0, element="O")), mol(*[A(3 * math.cos(i), 3 * math.sin(i), 0, serial=i + 2) for i in range(15)])),
+        (mol(A(0, 0, 0, serial=7)), mol(A(4, 0, 0, serial=7))),  # serial skip
+    ]
+    for k, (r, l) in enumerate(cases):
+        cx = Complex(structure.Molecule(r), structure.Molecule(l))
+        got, _ = cx.score_coords(np.stack([l["x"], l["y"], l["z"]], 1)[None])
+        ref = [orc.vdw_energy(r, l), orc.elec_energy(r, l), orc.desolv_energy(r, l)]
+        assert_energies_close(got[0, :3], np.array(ref), f"case {k}")
+    # closed forms through the device path
+    cx = Complex(structure.Molecule(mol(A(0, 0, 0))), structure.Molecule(mol(A(5, 0, 0, serial=2))))
+    got, _ = cx.score_coords(np.array([[[5.0, 0, 0]], [[3.0, 0, 0]], [[15.0, 0, 0]]]))
+    assert got[0, 0] == pytest.approx(-0.04098164762097422, rel=1e-6)
+    assert got[1, 0] == pytest.approx(-0.07744006564546031, rel=1e-6)
+    assert got[2, 0] == 0.0
+
+
+def test_restraints_on_device():
+    rec = structure.Molecule(mol(A(0, 0, 0, resseq=1), A(0, 0, 0, resseq=2, serial=3)))
+    lig = structure.Molecule(mol(A(5, 0, 0, resseq=10, chainid="B", serial=4), A(15, 0, 0, resseq=11, chainid="B", serial=5)))
+    cx = Complex(rec, lig, [(1, 10), (2, 11), (99, 10)])
+    assert cx.n_restraints == 2
+    got, ns = cx.score_coords(lig.xyz[None])
+    assert got[0, 3] == 640.0 and ns[0] == 1 and got[0, 4] == 1.0  # (1 - 0.5) * 2
+
+
+def test_2oob_ga_and_docking_poses():
+    rec, lig, cx = _pair(read_text("2oob_A.pdb"), read_text("2oob_B.pdb"), RESTRAINTS_2OOB)
+    assert cx.n_restraints == 6
+    genes = np.concatenate([synth.ga_genes(7, 96), synth.random_genes(8, 96, radius=20.0)])
+    _check_poses(rec, lig, cx, RESTRAINTS_2OOB, genes, "2oob")
+
+
+def test_1h590_100_models_scored_as_given():
+    """commands/score.rs on data/1h590*: untransformed models, serial collisions
+    (both files number atoms from 1), no element column, OXT / ILE CD1."""
+    rec = structure.read_pdb_text(read_text("1h590A_0A.pdb"))[0]
+    models = structure.read_pdb_text(read_text("1h590B_0A_poses.pdb.gz"))
+    cx = Complex(rec, models[0], [(10, 5), (20, 30)])
+    xyz = np.stack([m.xyz for m in models])
+    got, ns = cx.score_coords(xyz)
+    ref = np.zeros((len(models), 6))
+    rns = np.zeros(len(models), np.int32)
+    for i, m in enumerate(models):
+        o, n = orc.score_poses(rec.atoms, m.atoms, [(10, 5), (20, 30)], np.zeros((1, 6)), W, transform=False)
+        ref[i], rns[i] = o[0], n[0]
+    assert_energies_close(got, ref, "1h590")
+    assert np.array_equal(ns, rns)
+
+
+@pytest.mark.parametrize("seed,nr,nl", [(11, 700, 300), (12, 2000, 1500), (13, 900, 2600)])
+def test_synthetic_complexes(seed, nr, nl):
+    c = synth.make_complex(seed, rec_atoms=nr, lig_atoms=nl, n_restraints=8)
+    rec, lig, cx = _pair(c.receptor_pdb, c.ligand_pdb, c.restraints)
+    genes = np.concatenate([synth.random_genes(seed, 24, 20.0), synth.ga_genes(seed, 8)])
+    _check_poses(rec, lig, cx, c.restraints, genes, f"synthetic {nr}x{nl}")
+
+
+def test_cutoff_boundaries_exact():
+    """Pairs placed exactly on and one ulp around every cutoff go through the
+    FP64 re-decision and must match the reference's decisions."""
+    cuts = [1.0, 8.0, 12.0, 15.0]
+    lig_pts = []
+    for c in cuts:
+        for k in (-2, -1, 0, 1, 2):
+            x = c
+            for _ in range(abs(k)):
+                x = math.nextafter(x, math.inf if k > 0 else -math.inf)
+            lig_pts.append(x)
+    rec = structure.Molecule(mol(A(0.0, 0.0, 0.0, charge=0.5, element="O", serial=1)))
+    lig = structure.Molecule(mol(*[A(x, 0.0, 0.0, charge=-0.5, element="C", serial=2 + i, chainid="B")
+                                   for i, x in enumerate(lig_pts)]))
+    cx = Complex(rec, lig)
+    got, _ = cx.score_coords(lig.xyz[None])
+    ref = [orc.vdw_energy(rec.atoms, lig.atoms), orc.elec_energy(rec.atoms, lig.atoms),
+           orc.desolv_energy(rec.atoms, lig.atoms)]
+    assert_energies_close(got[0, :3], np.array(ref), "boundaries")
+    # and one atom at a time, so a single flip cannot hide in the sum
+    for i in range(len(lig_pts)):
+        l1 = structure.Molecule(lig.atoms[i:i + 1].copy())
+        cx1 = Complex(rec, l1)
+        g1, _ = cx1.score_coords(l1.xyz[None])
+        r1 = [orc.vdw_energy(rec.atoms, l1.atoms), orc.elec_energy(rec.atoms, l1.atoms),
+              orc.desolv_energy(rec.atoms, l1.atoms)]
+        assert_energies_close(g1[0, :3], np.array(r1), f"boundary atom {i} at {lig_pts[i]!r}")
+        assert (g1[0, 1] == 0.0) == (r1[1] == 0.0)
+
+
+def test_alone_equals_in_batch():
+    c = synth.make_complex(21, rec_atoms=800, lig_atoms=600)
+    rec, lig, cx = _pair(c.receptor_pdb, c.ligand_pdb, c.restraints)
+    genes = synth.random_genes(21, 200)
+    full, _ = cx.score_poses(genes)
+    for i in (0, 17, 199):
+        one, _ = cx.score_poses(genes[i:i + 1])
+        assert np.array_equal(one[0], full[i])
+
+
+def test_eval_fitness_drop_in():
+    rec, lig, cx = _pair(read_text("2oob_A.pdb"), read_text("2oob_B.pdb"), RESTRAINTS_2OOB)
+    genes = synth.ga_genes(3, 20)
+    chromos = [Chromosome(list(g)) for g in genes]
+    pop = Population(chromos, rec, lig, [Restraint(r, l) for r, l in RESTRAINTS_2OOB], constants.EnergyWeights())
+    pop.eval_fitness()
+    out, _ = cx.score_poses(genes)
+    for c, o in zip(chromos, out):
+        assert (c.vdw, c.elec, c.desolv, c.air, c.restraint_penalty, c.fitness) == tuple(o)
+
+
+def test_invalid_inputs_rejected():
+    rec, lig, cx = _pair(read_text("2oob_A.pdb"), read_text("2oob_B.pdb"))
+    g = synth.ga_genes(1, 4)
+    g[2, 1] = np.nan
+    with pytest.raises(_lib.GdockError) as e:
+        cx.score_poses(g)
+    assert e.value.status == _lib.INVALID_ARG
+    out, ns = cx.score_poses(np.zeros((0, 6)))
+    assert out.shape == (0, 6)
+
+
+def test_empty_partners():
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    cx = Complex(rec, structure.Molecule(mol(A(1e3, 0, 0, serial=9999))))
+    got, _ = cx.score_poses(np.zeros((2, 6)))
+    ref, _ = orc.score_poses(rec.atoms, mol(A(1e3, 0, 0, serial=9999)), [], np.zeros((2, 6)), W)
+    assert_energies_close(got, ref, "far ligand")
+
+
+def test_full_size_batch_properties():
+    """4096 poses of the 2000 x 1500 workload: finite, deterministic across two
+    launches, and exact on a sample against the oracle."""
+    cfg = synth.CONFIGS["batched_scoring"]
+    c = synth.make_complex(cfg["seed"], rec_atoms=cfg["rec_atoms"], lig_atoms=cfg["lig_atoms"], n_restraints=6)
+    rec, lig, cx = _pair(c.receptor_pdb, c.ligand_pdb, c.restraints)
+    genes = synth.random_genes(cfg["seed"], cfg["n_poses"])
+    a, na = cx.score_poses(genes)
+    b, nb = cx.score_poses(genes)
+    assert np.isfinite(a).all() and np.array_equal(a, b) and np.array_equal(na, nb)
+    idx = np.linspace(0, len(genes) - 1, 24).astype(int)
+    ref, rns = orc.score_poses(rec.atoms, lig.atoms, c.restraints, genes[idx], W)
+    assert_energies_close(a[idx], ref, "4096-pose sample")
+    assert np.array_equal(na[idx], rns)

@@ -1,0 +1,115 @@
+"""Host-side checks of the product library (no GPU calls): the C ABI is
+exported, PDB parsing + parameterization and ligand positioning match the
+oracle bit for bit, pose records carry glibc trig, the synthetic generator is
+deterministic."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle.oracle as orc
+from gdock_amd import _lib, structure, synth
+from helpers import read_text
+
+ROOT = Path(__file__).resolve().parent.parent
+FIELDS = ["x", "y", "z", "charge", "vdw_radius", "epsilon", "rmin2", "serial", "resseq", "chainid", "name",
+          "resname", "element"]
+
+
+def header_functions():
+    src = (ROOT / "include" / "gdock_hip.h").read_text()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gdock_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_library_fails_loudly_when_missing(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "nope.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(ImportError):
+        _lib.lib()
+
+
+@pytest.mark.parametrize("name", ["2oob_A.pdb", "2oob_B.pdb", "2oob.pdb", "1h590A_0A.pdb", "1h590B_0A_poses.pdb.gz"])
+def test_pdb_parse_matches_oracle(name):
+    text = read_text(name)
+    mine = structure.read_pdb_text(text)
+    ref = orc.read_pdb_text(text)
+    assert len(mine) == len(ref)
+    for a, b in zip(mine, ref):
+        assert len(a) == len(b)
+        for f in FIELDS:
+            assert np.array_equal(a.atoms[f], b[f]), f
+
+
+def test_pdb_parse_edge_cases():
+    lines = [
+        "MODEL        1",
+        "ATOM      1  CA  ALA A   1       1.000   2.000   3.000",                  # 54 chars, no occupancy
+        "ATOM      2  CB  ALA A   1       1.000   2.000   x.000  1.00  0.00      C",  # bad z: skipped
+        "HETATM    3  O   HOH A   2       0.000   0.000   0.000  1.00  0.00           O",
+        "ATOM      4  OXT ALA A   1       0.000   0.000   0.000  1.00  0.00           O",
+        "ATOM      5 HB1  ALA A   1       0.000   0.000   0.000  1.00  0.00",            # element from name
+        "ENDMDL",
+        "ATOM      6  N   ALA A   1       0.000   0.000   0.000  1.00  0.00           N",  # after ENDMDL, no MODEL: dropped
+    ]
+    text = "\n".join(lines) + "\n"
+    mine = structure.read_pdb_text(text)
+    ref = orc.read_pdb_text(text)
+    assert [len(m) for m in mine] == [len(r) for r in ref] == [3]
+    for f in FIELDS:
+        assert np.array_equal(mine[0].atoms[f], ref[0][f]), f
+
+
+def test_position_ligand_matches_oracle():
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    lig = structure.read_pdb_text(read_text("2oob_B.pdb"))[0]
+    mine = structure.position_ligand(rec, lig)
+    ref = orc.position_ligand(rec.atoms, lig.atoms)
+    for f in ("x", "y", "z"):
+        assert np.array_equal(mine.atoms[f], ref[f])
+
+
+def test_prepare_poses_trig_is_glibc():
+    g = synth.random_genes(5, 64)
+    poses = np.zeros((64, 12))
+    assert _lib.lib().gdock_prepare_poses(_lib.ptr(g), 64, _lib.ptr(poses)) == 0
+    assert np.array_equal(poses[:, :6], g)
+    for i in range(64):
+        a, b, c = g[i, :3]
+        assert tuple(poses[i, 6:]) == (math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(c), math.sin(c))
+    bad = g.copy()
+    bad[3, 4] = np.nan
+    assert _lib.lib().gdock_prepare_poses(_lib.ptr(bad), 64, _lib.ptr(poses)) == _lib.INVALID_ARG
+
+
+def test_synthetic_complex_deterministic_and_sized():
+    a = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
+    b = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
+    assert a.receptor_pdb == b.receptor_pdb and a.restraints == b.restraints
+    rec = structure.read_pdb_text(a.receptor_pdb)[0]
+    lig = structure.read_pdb_text(a.ligand_pdb)[0]
+    assert (len(rec), len(lig)) == (2000, 1500)
+    assert set(rec.atoms["chainid"]) == {b"A"} and set(lig.atoms["chainid"]) == {b"B"}
+    assert lig.atoms["serial"][0] == 2001
+    assert np.all(rec.atoms["epsilon"] < 0)  # every template atom is parameterized
+    r0 = synth.make_complex(0, rec_res=60, lig_res=76)
+    assert len(r0.restraints) == 6
+    assert orc.restraints_resolved(r0.restraints, orc.read_pdb_text(r0.receptor_pdb)[0],
+                                   orc.read_pdb_text(r0.ligand_pdb)[0]) == 6
+
+
+def test_random_genes_are_reference_euler_angles():
+    g = synth.random_genes(1, 256)
+    assert np.all((g[:, :3] >= 0) & (g[:, :3] < 2 * np.pi))
+    assert np.all(np.linalg.norm(g[:, 3:], axis=1) <= 20.0)
