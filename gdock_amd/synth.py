@@ -67,10 +67,15 @@ class Chain:
 
 def make_chain(rng: np.random.Generator, chain: str, n_res: int | None = None, n_atoms: int | None = None,
                radius: float | None = None) -> Chain:
-    if n_res is None:
-        n_res = int(np.ceil(n_atoms / MEAN_HEAVY)) + 4
+    if n_res is None:  # draw residues until the templates hold n_atoms heavy atoms
+        resnames, total = [], 0
+        while total < n_atoms:
+            resnames.append(RESIDUES[int(rng.integers(0, len(RESIDUES)))])
+            total += len(HEAVY_TEMPLATES[resnames[-1]])
+        n_res = len(resnames)
+    else:
+        resnames = [RESIDUES[i] for i in rng.integers(0, len(RESIDUES), size=n_res)]
     R = globule_radius(n_res) if radius is None else radius
-    resnames = [RESIDUES[i] for i in rng.integers(0, len(RESIDUES), size=n_res)]
     ca = np.zeros((n_res, 3))
     ca[0] = _unit(rng) * R * 0.5 * rng.random()
     for i in range(1, n_res):
