@@ -117,6 +117,7 @@ void gdock_ctx_destroy(gdock_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->staging.release();
+  ctx->redo.release();
   release_fcc_scratch(ctx);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
@@ -160,7 +161,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     set_error(ctx, "complex_create: bad sizes or null arrays");
     return GDOCK_ERR_INVALID_ARG;
   }
-  if (n_rec > kMaxLdsReceptor) {
+  if (n_rec + 4 * 64 > kMaxLdsReceptor) {
     set_error(ctx, "complex_create: %d receptor atoms exceeds the supported %d", n_rec, kMaxLdsReceptor);
     return GDOCK_ERR_INVALID_ARG;
   }
@@ -173,21 +174,25 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   cx->lig.assign(lig, lig + n_lig);
 
   // Receptor permutation: group atoms by (rmin2, epsilon) class so that the
-  // kernel updates its per-lane Lennard-Jones factors once per class.
+  // kernel updates its per-lane Lennard-Jones factors once per class.  Each
+  // class segment is padded to a multiple of 4 with inert atoms (far away, no
+  // charge, no solvation parameter) so the kernel reads receptor atoms 4 at a
+  // time (one 64-byte scalar load) and class boundaries fall on those groups.
   std::map<std::pair<double, double>, int> cls_of;
   for (auto& a : cx->rec) cls_of.emplace(std::make_pair(a.rmin2, a.epsilon), 0);
   int nc = 0;
   for (auto& kv : cls_of) kv.second = nc++;
-  std::vector<int32_t> perm(n_rec);
-  for (int32_t i = 0; i < n_rec; ++i) perm[i] = i;
-  std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
-    return cls_of[{cx->rec[a].rmin2, cx->rec[a].epsilon}] < cls_of[{cx->rec[b].rmin2, cx->rec[b].epsilon}];
-  });
-  std::vector<int32_t> class_start(nc + 1, 0);
+  std::vector<std::vector<int32_t>> members(nc);
+  for (int32_t i = 0; i < n_rec; ++i) members[cls_of[{cx->rec[i].rmin2, cx->rec[i].epsilon}]].push_back(i);
+  std::vector<int32_t> class_start(nc + 1, 0), perm;  // perm[j] = receptor atom or -1 (padding)
   std::vector<double> class_rmin(nc), class_eps(nc);
   for (auto& kv : cls_of) { class_rmin[kv.second] = kv.first.first; class_eps[kv.second] = kv.first.second; }
-  for (auto& a : cx->rec) class_start[cls_of[{a.rmin2, a.epsilon}] + 1]++;
-  for (int c = 0; c < nc; ++c) class_start[c + 1] += class_start[c];
+  for (int c = 0; c < nc; ++c) {
+    perm.insert(perm.end(), members[c].begin(), members[c].end());
+    while (perm.size() % kRecGroup) perm.push_back(-1);
+    class_start[c + 1] = static_cast<int32_t>(perm.size());
+  }
+  const int32_t n_pad = static_cast<int32_t>(perm.size());
 
   double origin[3] = {0, 0, 0}, lcom[3] = {0, 0, 0};
   if (n_rec) com_of(rec, n_rec, origin);
@@ -195,12 +200,17 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   std::unordered_set<int32_t> lig_serials;
   for (auto& a : cx->lig) lig_serials.insert(a.serial);
 
-  std::vector<float4> xyzq(n_rec);
-  std::vector<double> rx(n_rec), ry(n_rec), rz(n_rec), rasp(n_rec), rrad(n_rec);
-  std::vector<int2> meta(n_rec);
+  std::vector<float4> xyzq(n_pad);
+  std::vector<double> rx(n_pad), ry(n_pad), rz(n_pad), rasp(n_pad, 0.0), rrad(n_pad, 0.0);
+  std::vector<int2> meta(n_pad, make_int2(0, 0));
   double rec_ext = 0.0;
   int has_coll = 0;
-  for (int32_t j = 0; j < n_rec; ++j) {
+  for (int32_t j = 0; j < n_pad; ++j) {
+    if (perm[j] < 0) {
+      xyzq[j] = make_float4(kPadRecCoord, kPadRecCoord, kPadRecCoord, 0.0f);
+      rx[j] = ry[j] = rz[j] = static_cast<double>(kPadRecCoord);
+      continue;
+    }
     const gdock_atom& a = cx->rec[perm[j]];
     const double px = a.x - origin[0], py = a.y - origin[1], pz = a.z - origin[2];
     xyzq[j] = make_float4(static_cast<float>(px), static_cast<float>(py), static_cast<float>(pz),
@@ -243,23 +253,23 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   cx->n_restr = static_cast<int32_t>(rl_idx.size());
 
   Layout L;
-  const size_t o_xyzq = L.add<float4>(n_rec), o_rx = L.add<double>(n_rec), o_ry = L.add<double>(n_rec),
-               o_rz = L.add<double>(n_rec), o_rasp = L.add<double>(n_rec), o_rrad = L.add<double>(n_rec),
-               o_meta = L.add<int2>(n_rec), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
+  const size_t o_xyzq = L.add<float4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
+               o_rz = L.add<double>(n_pad), o_rasp = L.add<double>(n_pad), o_rrad = L.add<double>(n_pad),
+               o_meta = L.add<int2>(n_pad), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
                o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
                o_lser = L.add<int32_t>(n_lig), o_rrx = L.add<double>(rr_xyz.size()),
-               o_rli = L.add<int32_t>(rl_idx.size());
+               o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1);
   cx->contacts_layout_begin = L.size;
   std::vector<uint8_t> host;
   build_contact_tables(cx, &L, &host);  // FCC residue tables appended to the same arena
   std::vector<uint8_t> blob(L.size, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(blob.data() + off, src, bytes); };
   put(o_xyzq, xyzq.data(), xyzq.size() * sizeof(float4));
-  put(o_rx, rx.data(), n_rec * 8); put(o_ry, ry.data(), n_rec * 8); put(o_rz, rz.data(), n_rec * 8);
-  put(o_rasp, rasp.data(), n_rec * 8); put(o_rrad, rrad.data(), n_rec * 8);
-  put(o_meta, meta.data(), n_rec * sizeof(int2));
+  put(o_rx, rx.data(), n_pad * 8); put(o_ry, ry.data(), n_pad * 8); put(o_rz, rz.data(), n_pad * 8);
+  put(o_rasp, rasp.data(), n_pad * 8); put(o_rrad, rrad.data(), n_pad * 8);
+  put(o_meta, meta.data(), n_pad * sizeof(int2));
   put(o_cs, class_start.data(), (nc + 1) * 4); put(o_crm, class_rmin.data(), nc * 8); put(o_ceps, class_eps.data(), nc * 8);
   put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
   put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
@@ -271,10 +281,6 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   if (e == hipSuccess) e = hipMalloc(&cx->mem, std::max<size_t>(L.size, 256));
   if (e != hipSuccess) { delete cx; return hip_fail(ctx, e, "complex_create: hipMalloc"); }
   cx->mem_bytes = std::max<size_t>(L.size, 256);
-  e = hipMemcpyAsync(cx->mem, blob.data(), L.size, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) { (void)hipFree(cx->mem); delete cx; return hip_fail(ctx, e, "complex_create: upload"); }
-
   char* m = static_cast<char*>(cx->mem);
   DevComplex& d = cx->dev;
   d.rec_xyzq = reinterpret_cast<const float4*>(m + o_xyzq);
@@ -298,12 +304,17 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.lig_serial = reinterpret_cast<const int32_t*>(m + o_lser);
   d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
-  d.n_rec = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
+  d.n_rec = n_pad; d.n_rec_atoms = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
   d.has_serial_collision = has_coll;
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }
   d.rec_extent = rec_ext;
   d.lig_extent = lig_r + com_off;
   bind_contact_tables(cx);
+  cx->d_dev = reinterpret_cast<const DevComplex*>(m + o_dev);
+  std::memcpy(blob.data() + o_dev, &cx->dev, sizeof(DevComplex));
+  e = hipMemcpyAsync(cx->mem, blob.data(), L.size, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) { (void)hipFree(cx->mem); delete cx; return hip_fail(ctx, e, "complex_create: upload"); }
   *out = cx;
   return GDOCK_OK;
 }
@@ -338,7 +349,10 @@ gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const d
   if (!ctx || !cx || !w || n < 0 || (n && (!d_poses || !d_out))) return GDOCK_ERR_INVALID_ARG;
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  const hipError_t e = launch_score(cx->dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat, s);
+  hipError_t e = hipSuccess;
+  if (!ctx->redo.reserve(static_cast<size_t>(std::max(n, 1)) * 4, &e)) return hip_fail(ctx, e, "score_poses_device: scratch");
+  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat,
+                   static_cast<int32_t*>(ctx->redo.ptr), s);
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device: launch");
 }
 
@@ -357,14 +371,16 @@ gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* 
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: set device");
   const size_t pb = poses.size() * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8, nb = static_cast<size_t>(n) * 4;
-  const size_t o_out = (pb + 255) & ~static_cast<size_t>(255), o_ns = (o_out + ob + 255) & ~static_cast<size_t>(255);
-  if (!ctx->staging.reserve(o_ns + nb, &e)) return hip_fail(ctx, e, "score_poses: staging");
+  const size_t o_out = (pb + 255) & ~static_cast<size_t>(255), o_ns = (o_out + ob + 255) & ~static_cast<size_t>(255),
+               o_rd = (o_ns + nb + 255) & ~static_cast<size_t>(255);
+  if (!ctx->staging.reserve(o_rd + nb, &e)) return hip_fail(ctx, e, "score_poses: staging");
   char* base = static_cast<char*>(ctx->staging.ptr);
   e = hipMemcpyAsync(base, poses.data(), pb, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: upload");
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
-  e = launch_score(cx->dev, reinterpret_cast<double*>(base), nullptr, nullptr, n, ww,
-                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), ctx->stream);
+  e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(base), nullptr, nullptr, n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns),
+                   reinterpret_cast<int32_t*>(base + o_rd), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: launch");
   e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);
@@ -391,15 +407,16 @@ gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double*
   const size_t cb = ncoord * 8, eb = static_cast<size_t>(n) * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8,
                nb = static_cast<size_t>(n) * 4;
   auto al = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
-  const size_t o_ext = al(cb), o_out = al(o_ext + eb), o_ns = al(o_out + ob);
-  if (!ctx->staging.reserve(o_ns + nb, &e)) return hip_fail(ctx, e, "score_coords: staging");
+  const size_t o_ext = al(cb), o_out = al(o_ext + eb), o_ns = al(o_out + ob), o_rd = al(o_ns + nb);
+  if (!ctx->staging.reserve(o_rd + nb, &e)) return hip_fail(ctx, e, "score_coords: staging");
   char* base = static_cast<char*>(ctx->staging.ptr);
   e = hipMemcpyAsync(base, lig_xyz, cb, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(base + o_ext, extent.data(), eb, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: upload");
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
-  e = launch_score(cx->dev, nullptr, reinterpret_cast<double*>(base), reinterpret_cast<double*>(base + o_ext), n, ww,
-                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), ctx->stream);
+  e = launch_score(cx->dev, cx->d_dev, nullptr, reinterpret_cast<double*>(base), reinterpret_cast<double*>(base + o_ext), n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns),
+                   reinterpret_cast<int32_t*>(base + o_rd), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: launch");
   e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);

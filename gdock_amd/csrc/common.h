@@ -26,7 +26,9 @@ constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, 
 constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup
 constexpr int kMaxAtomsPerLane = 8;
 constexpr int kMaxLdsReceptor = 16000;  // receptor burial counters kept in LDS
-constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away
+constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away ...
+constexpr float kPadRecCoord = -1.0e4f;  // ... on the other side from padding receptor atoms
+constexpr int kRecGroup = 4;             // receptor atoms per scalar load
 
 // Device view of one complex (kernel argument, plain pointers).
 struct DevComplex {
@@ -54,9 +56,10 @@ struct DevComplex {
   // restraints: receptor CA coordinates (FP64) and ligand CA index
   const double* restr_rec_xyz;  // n_restr x 3
   const int32_t* restr_lig;     // n_restr
-  int32_t n_rec, n_lig, n_class, n_restr;
+  int32_t n_rec;        // receptor slots including class padding (multiple of kRecGroup)
+  int32_t n_rec_atoms;  // real receptor atoms
+  int32_t n_lig, n_class, n_restr;
   int32_t has_serial_collision;
-  int32_t pad_;
   double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
   double origin[3];  // FP32 frame origin
   double rec_extent;   // max |rec - origin| component

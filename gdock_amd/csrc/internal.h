@@ -41,12 +41,14 @@ struct gdock_ctx {
   std::string err;
   gdock::DeviceArena staging;   // per-call poses / outputs
   gdock::DeviceArena fcc;       // FCC bitsets, overlap and cluster scratch
+  gdock::DeviceArena redo;      // per-pose exact-rescore flags of the device-resident entry
   void* pinned = nullptr;       // pinned host staging
   size_t pinned_cap = 0;
 };
 
 struct gdock_complex {
-  gdock::DevComplex dev{};
+  gdock::DevComplex dev{};                  // host copy (sizes, pointers)
+  const gdock::DevComplex* d_dev = nullptr;  // the same struct in device memory
   gdock::DevContacts con{};
   void* mem = nullptr;
   size_t mem_bytes = 0;
@@ -67,9 +69,11 @@ std::string_view field(const char* f, size_t n);
 
 void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::vector<int32_t>* offsets);
 
-hipError_t launch_score(const DevComplex& cx, const double* d_poses, const double* d_coords,
+// Scores n_poses poses: the fast kernel, then the exact kernel for any pose
+// whose settle buffer overflowed.  d_redo: n_poses int32 of device scratch.
+hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         const double* d_extent, int n_poses, const Weights& w, double* d_out,
-                        int32_t* d_nsat, hipStream_t stream);
+                        int32_t* d_nsat, int32_t* d_redo, hipStream_t stream);
 
 // Host-side bump layout of one device allocation (256-byte aligned pieces).
 struct Layout {

@@ -10,14 +10,20 @@
 // Per pair (fitness.rs:57-105,122-184,284-302): soft-core LJ, Coulomb with
 // eps = r, and the two desolvation burial counts.  Pair math is FP32.  Every
 // cutoff decision (d > 1, d < 8, d < 12, d < 15) is taken in FP32 against a
-// guard band derived from the coordinate magnitudes; a pair inside a band is
-// decided again in FP64 from the original coordinates with the reference's
-// operation order, so decisions match the FP64 reference exactly.  Energies
-// are accumulated per lane in FP32 over 64-atom receptor tiles, then in FP64;
-// the cross-lane and cross-wave reductions are FP64.  Desolvation burial
-// counts are integers; the per-atom desolvation and AIR restraint terms and
-// the weighted fitness are computed in FP64 in the reference's order.
+// guard band derived from the coordinate magnitudes.  A pair inside a band is
+// rare; the wave records it as an event in LDS and, once per ligand chunk,
+// decides it again in FP64 from the original coordinates with the reference's
+// operation order and applies the difference, so every decision matches the
+// FP64 reference.  (A wave whose event buffer overflows flags its pose, and
+// the exact variant of the kernel, launched right after, rescores that pose
+// deciding such pairs inline.)  Energies are accumulated per lane in FP32 over
+// 64-atom receptor tiles, then in FP64; the cross-lane and cross-wave
+// reductions are FP64.  Desolvation burial counts are integers; the per-atom
+// desolvation and AIR restraint terms and the weighted fitness are computed
+// in FP64 in the reference's order.
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "common.h"
 
@@ -82,15 +88,209 @@ __device__ __forceinline__ void guard_band(double T, double X, float& lo, float&
   hi = uniform(h);
 }
 
+// One receptor atom r (FP32, wave-uniform) against the K ligand atoms of the
+// lane.  The fast path decides every cutoff in FP32 (d2 <= lo_T: below the
+// guard band of T; d2 > hi_1: beyond the elec lower cutoff).  A pair whose d2
+// lies inside a band, or at d <= ~1 A, flags the receptor atom; settle_pairs()
+// then re-decides those pairs in FP64 exactly as the reference does and adds
+// the difference to what the fast path accumulated.
+struct Bands { float lo1, hi1, lo64, hi64, lo144, hi144, lo225, hi225; };
+
 template <int K>
+struct LaneAtoms {
+  float x[K], y[K], z[K], eps[K], ir6[K], acv[K], ace[K];
+  int32_t cnt[K];
+};
+
+__device__ __forceinline__ float pair_d2(float rx, float ry, float rz, float lx, float ly, float lz) {
+  const float dx = rx - lx, dy = ry - ly, dz = rz - lz;
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Soft-core LJ shape (rho^2 - 2 rho, rho = R^6 / (d^6 + R^6/2)) and 1/d^2,
+// both from one reciprocal: 1 / (d^2 (d^6/R^6 + 1/2)).
+__device__ __forceinline__ void pair_terms(float d2, float ir6, float& tv, float& id2) {
+  const float d6 = (d2 * d2) * d2;
+  const float den = __builtin_fmaf(d6, ir6, 0.5f);
+  const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, d2, 1.0e-30f));
+  const float rho = rp * d2;
+  id2 = rp * den;
+  tv = (rho - 2.0f) * rho;
+}
+
+__device__ __forceinline__ bool in_band(float d2, const Bands& B) {
+  return ((d2 < B.hi1) != (d2 <= B.lo1)) | ((d2 < B.hi64) != (d2 <= B.lo64)) |
+         ((d2 < B.hi144) != (d2 <= B.lo144)) | ((d2 < B.hi225) != (d2 <= B.lo225));
+}
+
+// Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
+// 1/R^6 and sqrt(eps_r eps_l) in FP64 (fitness.rs:58-63), rounded once.
+__device__ __forceinline__ void class_factors(const DevComplex& cx, int c, int l, float& ir6, float& eps) {
+  const double R = cx.class_rmin2[c] + cx.lig_rmin2[l];
+  const double R2 = R * R, R6 = R2 * (R2 * R2);
+  ir6 = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
+  eps = static_cast<float>(sqrt(cx.class_eps[c] * cx.lig_eps[l]));
+}
+
+// The reference's decisions for one pair, in FP64 on the pose's coordinates.
+__device__ __forceinline__ void exact_flags(const DevComplex& cx, const PoseCoef& pc, const double* __restrict__ coords,
+                                            int pose, int l, int j, int rser, bool& x64, bool& x144, bool& x225) {
+  double px, py, pz;
+  ligand_atom(cx, pc, coords, pose, l, px, py, pz);
+  const double ex = cx.rec_x[j] - px, ey = cx.rec_y[j] - py, ez = cx.rec_z[j] - pz;
+  const double e2 = ex * ex + ey * ey + ez * ez;
+  x64 = e2 <= kD2DesolvIncl;
+  x144 = e2 > 0.0 && e2 <= kD2VdwIncl && cx.lig_serial[l] != rser;
+  x225 = e2 > kD2ElecMinExcl && e2 <= kD2ElecMaxIncl;
+}
+
+struct SettleEvent { int32_t j, k; uint64_t mask; };  // receptor slot, ligand slot, lanes
+constexpr int kEventsPerWave = 128;
+
+// Fast path of one receptor atom against the lane's K ligand atoms; returns
+// this wave's burial count of the receptor atom and flags band pairs.
+template <int K, bool COLL>
+__device__ __forceinline__ int32_t receptor_atom(const DevComplex& cx, int base, int j, const float4 r, const Bands& B,
+                                                 LaneAtoms<K>& a, int32_t& ambv) {
+  int rser = INT_MIN;  // INT_MIN: no ligand atom shares this receptor atom's serial
+  if (COLL) {
+    const int2 m = cx.rec_meta[j];
+    rser = m.y ? m.x : INT_MIN;
+  }
+  int32_t cj = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
+    const bool md = d2 <= B.lo64, mv0 = d2 <= B.lo144, me0 = d2 <= B.lo225, far1 = !(d2 <= B.lo1);
+    const bool amb = (far1 != !(d2 < B.hi1)) | (md != (d2 < B.hi64)) | (mv0 != (d2 < B.hi144)) |
+                     (me0 != (d2 < B.hi225));
+    ambv = amb ? 1 : ambv;
+    bool mv = mv0;
+    if (COLL) mv = mv && cx.lig_serial[base + threadIdx.x + kScoreThreads * k] != rser;  // fitness.rs:92
+    const bool me = me0 && far1;
+    float tv, id2;
+    pair_terms(d2, a.ir6[k], tv, id2);
+    a.acv[k] = __builtin_fmaf(mv ? tv : 0.0f, a.eps[k], a.acv[k]);
+    a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
+    a.cnt[k] += md ? 1 : 0;
+    cj += __popcll(__builtin_amdgcn_ballot_w64(d2 <= B.lo64));
+    // Pin this pair's results here (an empty asm, no instruction): otherwise
+    // the compiler sinks the math below the group's rare band-pair branch and
+    // keeps every pair's lane masks alive across it, spilling SGPRs.
+    asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+v"(ambv));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return cj;
+}
+
+// Band pairs of one receptor atom: recorded as events (fast kernel) or
+// settled on the spot in FP64 (exact kernel).
+template <int K, bool COLL, bool EXACT>
+__device__ __forceinline__ void band_pairs(const DevComplex& cx, const PoseCoef& pc, const double* __restrict__ coords,
+                                           int pose, int base, int j, const float4 r, const Bands& B,
+                                           LaneAtoms<K>& a, SettleEvent* __restrict__ ev, int& nev, int& overflow,
+                                           int32_t* __restrict__ s_rcnt) {
+  const int lane = threadIdx.x & 63;
+  int rser = INT_MIN;
+  if (COLL) {
+    const int2 m = cx.rec_meta[j];
+    rser = m.y ? m.x : INT_MIN;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
+    const int l = base + threadIdx.x + kScoreThreads * k;
+    const uint64_t am = __builtin_amdgcn_ballot_w64(in_band(d2, B) && l < cx.n_lig);
+    if (am == 0ull) continue;
+    if (!EXACT) {  // settled once per ligand chunk
+      if (nev < kEventsPerWave) {
+        if (lane == 0) ev[nev] = SettleEvent{j, k, am};
+        ++nev;
+      } else {
+        overflow = 1;
+      }
+      continue;
+    }
+    if ((am >> lane) & 1ull) {
+      bool x64, x144, x225;
+      exact_flags(cx, pc, coords, pose, l, j, rser, x64, x144, x225);
+      bool f144 = d2 <= B.lo144;
+      if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
+      const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
+      float tv, id2;
+      pair_terms(d2, a.ir6[k], tv, id2);
+      if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? tv : -tv, a.eps[k], a.acv[k]);
+      if (x225 != f225) a.ace[k] = __builtin_fmaf(x225 ? id2 : -id2, r.w, a.ace[k]);
+      const int dc = static_cast<int>(x64) - static_cast<int>(f64);
+      if (dc) {
+        a.cnt[k] += dc;
+        atomicAdd(&s_rcnt[j], dc);
+      }
+    }
+  }
+}
+
+// Settles the wave's recorded events: each flagged pair is decided in FP64
+// and the difference to the FP32 decisions is added to the FP64 sums, the
+// lane's burial count and the receptor atom's count.
+template <int K, bool COLL>
+__device__ __forceinline__ void settle_events(const DevComplex& cx, const double* __restrict__ poses,
+                                           const double* __restrict__ coords, int pose, int base, const Bands& B,
+                                           const SettleEvent* __restrict__ ev, int nev, LaneAtoms<K>& a,
+                                           double& sum_v, double& sum_e, int32_t* __restrict__ s_rcnt) {
+  const int lane = threadIdx.x & 63;
+  PoseCoef pc{};
+  if (!coords) pc = load_pose(poses + static_cast<size_t>(kPoseDoubles) * pose);
+  for (int e = 0; e < nev; ++e) {
+    const SettleEvent E = ev[e];
+    if (!((E.mask >> lane) & 1ull)) continue;
+    int c = 0;
+    while (cx.class_start[c + 1] <= E.j) ++c;
+    const float4 r = cx.rec_xyzq[E.j];
+    int rser = INT_MIN;
+    if (COLL) {
+      const int2 m = cx.rec_meta[E.j];
+      rser = m.y ? m.x : INT_MIN;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k != E.k) continue;
+      const int l = base + threadIdx.x + kScoreThreads * k;
+      float ir6, eps;
+      class_factors(cx, c, l, ir6, eps);
+      const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
+      bool x64, x144, x225;
+      exact_flags(cx, pc, coords, pose, l, E.j, rser, x64, x144, x225);
+      bool f144 = d2 <= B.lo144;
+      if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
+      const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
+      float tv, id2;
+      pair_terms(d2, ir6, tv, id2);
+      if (x144 != f144) sum_v += static_cast<double>(x144 ? tv : -tv) * static_cast<double>(eps);
+      if (x225 != f225)
+        sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
+      const int dc = static_cast<int>(x64) - static_cast<int>(f64);
+      if (dc) {
+        a.cnt[k] += dc;
+        atomicAdd(&s_rcnt[E.j], dc);
+      }
+    }
+  }
+}
+
+template <int K, bool COLL, bool EXACT>
 __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     const DevComplex cx, const double* __restrict__ poses, const double* __restrict__ coords,
     const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
-    int32_t* __restrict__ nsat_out) {
-  extern __shared__ int32_t s_rcnt[];  // n_rec burial counters, then reduction scratch
-  double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
+    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
   const int pose = blockIdx.x;
+  if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
+  extern __shared__ int32_t s_rcnt[];    // n_rec burial counters, then reduction scratch
+  double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
+  SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
+  __shared__ int32_t s_overflow;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  SettleEvent* my_ev = s_ev + wave * kEventsPerWave;
 
   PoseCoef pc{};
   double X;
@@ -100,125 +300,98 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     pc = load_pose(poses + static_cast<size_t>(kPoseDoubles) * pose);
     X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
   }
-  float lo1, hi1, lo64, hi64, lo144, hi144, lo225, hi225;
-  guard_band(1.0, X, lo1, hi1);
-  guard_band(64.0, X, lo64, hi64);
-  guard_band(144.0, X, lo144, hi144);
-  guard_band(225.0, X, lo225, hi225);
+  Bands B;
+  guard_band(1.0, X, B.lo1, B.hi1);
+  guard_band(64.0, X, B.lo64, B.hi64);
+  guard_band(144.0, X, B.lo144, B.hi144);
+  guard_band(225.0, X, B.lo225, B.hi225);
 
   for (int i = tid; i < cx.n_rec; i += kScoreThreads) s_rcnt[i] = 0;
+  if (tid == 0) s_overflow = 0;
   __syncthreads();
 
   double sum_v = 0.0, sum_e = 0.0, sum_d = 0.0;
-
+  int overflow = 0;
   for (int base = 0; base < cx.n_lig; base += kScoreThreads * K) {
-    float lx[K], ly[K], lz[K], epsij[K], invr6[K], acv[K], ace[K];
-    double adv[K], ade[K];
-    int32_t lcnt[K], lser[K];
+    LaneAtoms<K> a;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int l = base + tid + kScoreThreads * k;
       if (l < cx.n_lig) {
         double px, py, pz;
         ligand_atom(cx, pc, coords, pose, l, px, py, pz);
-        lx[k] = static_cast<float>(px - cx.origin[0]);
-        ly[k] = static_cast<float>(py - cx.origin[1]);
-        lz[k] = static_cast<float>(pz - cx.origin[2]);
-        lser[k] = cx.lig_serial[l];
+        a.x[k] = static_cast<float>(px - cx.origin[0]);
+        a.y[k] = static_cast<float>(py - cx.origin[1]);
+        a.z[k] = static_cast<float>(pz - cx.origin[2]);
       } else {
-        lx[k] = ly[k] = lz[k] = kPadCoord;
-        lser[k] = 0;
+        a.x[k] = a.y[k] = a.z[k] = kPadCoord;
       }
-      epsij[k] = 0.0f; invr6[k] = 0.0f; acv[k] = 0.0f; ace[k] = 0.0f;
-      adv[k] = 0.0; ade[k] = 0.0; lcnt[k] = 0;
+      a.eps[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
-
+    int nev = 0;
     int cls = -1, cend = 0;
-    for (int t0 = 0; t0 < cx.n_rec; t0 += 64) {
-      const int t1 = min(t0 + 64, cx.n_rec);
-      int32_t rc = 0;  // lane t: burial count of receptor atom t0 + t from this wave
-      for (int j = t0; j < t1; ++j) {
+    for (int g0 = 0; g0 < cx.n_rec; g0 += 64) {
+      const int g1 = min(g0 + 64, cx.n_rec);
+      int32_t rc = 0;  // lane t: burial count of receptor slot g0 + t from this wave
+      for (int j = g0; j < g1; j += kRecGroup) {
         if (j == cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
           do { ++cls; cend = cx.class_start[cls + 1]; } while (cend == j);
-          const double rmin_c = cx.class_rmin2[cls], eps_c = cx.class_eps[cls];
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             const int l = base + tid + kScoreThreads * k;
-            if (l < cx.n_lig) {
-              const double R = rmin_c + cx.lig_rmin2[l];
-              const double R2 = R * R, R6 = R2 * (R2 * R2);
-              invr6[k] = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
-              epsij[k] = static_cast<float>(sqrt(eps_c * cx.lig_eps[l]));
-            }
+            if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
           }
         }
-        const float4 r = cx.rec_xyzq[j];
-        int coll = 0, rser = 0;
-        if (cx.has_serial_collision) {
-          const int2 m = cx.rec_meta[j];
-          rser = m.x; coll = m.y;
+        const float4 r0 = cx.rec_xyzq[j], r1 = cx.rec_xyzq[j + 1], r2 = cx.rec_xyzq[j + 2], r3 = cx.rec_xyzq[j + 3];
+        int32_t ambv = 0;
+        const int t = lane - (j - g0);
+        const int32_t c0 = receptor_atom<K, COLL>(cx, base, j, r0, B, a, ambv);
+        rc = t == 0 ? c0 : rc;
+        const int32_t c1 = receptor_atom<K, COLL>(cx, base, j + 1, r1, B, a, ambv);
+        rc = t == 1 ? c1 : rc;
+        const int32_t c2 = receptor_atom<K, COLL>(cx, base, j + 2, r2, B, a, ambv);
+        rc = t == 2 ? c2 : rc;
+        const int32_t c3 = receptor_atom<K, COLL>(cx, base, j + 3, r3, B, a, ambv);
+        rc = t == 3 ? c3 : rc;
+        if (__builtin_amdgcn_ballot_w64(ambv != 0) != 0ull) {  // rare
+          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
+          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
+          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
+          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
         }
-        int32_t cj = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float dx = r.x - lx[k], dy = r.y - ly[k], dz = r.z - lz[k];
-          const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-          const bool a1 = d2 < hi1, a64 = d2 < hi64, a144 = d2 < hi144, a225 = d2 < hi225;
-          const bool b1 = d2 <= lo1, b64 = d2 <= lo64, b144 = d2 <= lo144, b225 = d2 <= lo225;
-          bool m1 = !a1, m64 = a64, m144 = a144, m225 = a225;
-          const bool amb = (a1 != b1) | (a64 != b64) | (a144 != b144) | (a225 != b225);
-          if (__any(amb)) {
-            if (amb && base + tid + kScoreThreads * k < cx.n_lig) {  // FP64 decision, structure.rs:312-317 order
-              double px, py, pz;
-              ligand_atom(cx, pc, coords, pose, base + tid + kScoreThreads * k, px, py, pz);
-              const double ex = cx.rec_x[j] - px, ey = cx.rec_y[j] - py, ez = cx.rec_z[j] - pz;
-              const double e2 = ex * ex + ey * ey + ez * ez;
-              m1 = e2 > kD2ElecMinExcl;
-              m64 = e2 <= kD2DesolvIncl;
-              m144 = e2 > 0.0 && e2 <= kD2VdwIncl;
-              m225 = e2 <= kD2ElecMaxIncl;
-            }
-          }
-          if (coll) m144 = m144 && (lser[k] != rser);  // fitness.rs:92 serial skip
-          // soft-core LJ (rho = R^6 / (d^6 + R^6/2)) and 1/d^2 from one reciprocal
-          const float d6 = (d2 * d2) * d2;
-          const float den = __builtin_fmaf(d6, invr6[k], 0.5f);
-          const float p = __builtin_fmaf(den, d2, 1.0e-30f);
-          const float rp = __builtin_amdgcn_rcpf(p);
-          const float rho = rp * d2;
-          const float id2 = rp * den;
-          const float tv = (rho - 2.0f) * rho;
-          acv[k] = __builtin_fmaf(m144 ? tv : 0.0f, epsij[k], acv[k]);
-          ace[k] = __builtin_fmaf((m1 && m225) ? id2 : 0.0f, r.w, ace[k]);
-          lcnt[k] += m64 ? 1 : 0;
-          cj += __popcll(__ballot(m64));
-        }
-        rc = lane == j - t0 ? cj : rc;
       }
-      if (t0 + lane < t1 && rc) atomicAdd(&s_rcnt[t0 + lane], rc);
+      if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane], rc);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        adv[k] += static_cast<double>(acv[k]);
-        ade[k] += static_cast<double>(ace[k]);
-        acv[k] = 0.0f; ace[k] = 0.0f;
+      for (int k = 0; k < K; ++k) {  // fold the tile's FP32 partial sums into FP64
+        const int l = base + tid + kScoreThreads * k;
+        if (l < cx.n_lig) {
+          sum_v += static_cast<double>(a.acv[k]);
+          sum_e += static_cast<double>(a.ace[k]) * (kCoulomb * cx.lig_q[l]);
+        }
+        a.acv[k] = 0.0f; a.ace[k] = 0.0f;
       }
     }
+    if (!EXACT && nev > 0)
+      settle_events<K, COLL>(cx, poses, coords, pose, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int l = base + tid + kScoreThreads * k;
       if (l < cx.n_lig) {
-        sum_v += adv[k];
-        sum_e += ade[k] * (kCoulomb * cx.lig_q[l]);
         const double asp = cx.lig_asp[l];
         if (fabs(asp) >= 0.001) {  // fitness.rs:165-181
-          const double burial = fmin(static_cast<double>(lcnt[k]) / 10.0, 1.0);
+          const double burial = fmin(static_cast<double>(a.cnt[k]) / 10.0, 1.0);
           const double rad = cx.lig_radius[l];
           sum_d -= asp * burial * rad * rad * 4.0 * kPI;
         }
       }
     }
   }
+  if (overflow && lane == 0) s_overflow = 1;
   __syncthreads();
+  if (!EXACT && s_overflow) {  // the exact variant rescores this pose
+    if (tid == 0) redo[pose] = 1;
+    return;
+  }
   for (int j = tid; j < cx.n_rec; j += kScoreThreads) {  // fitness.rs:141-162
     const double asp = cx.rec_asp[j];
     if (fabs(asp) >= 0.001) {
@@ -269,11 +442,13 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     o[4] = (1.0 - ratio) * n;  // chromosome.rs:117-118
     o[5] = w.vdw * vdw + w.elec * elec + w.desolv * desolv + w.air * air;  // chromosome.rs:121-124
     if (nsat_out) nsat_out[pose] = nsat;
+    if (!EXACT) redo[pose] = 0;
   }
 }
 
 size_t score_lds_bytes(const DevComplex& cx) {
-  return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 3 * (kScoreThreads / 64) * sizeof(double);
+  return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
+         (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent);
 }
 
 int score_atoms_per_lane(int n_lig) {
@@ -281,15 +456,30 @@ int score_atoms_per_lane(int n_lig) {
   return k < 1 ? 1 : (k > kMaxAtomsPerLane ? kMaxAtomsPerLane : k);
 }
 
-hipError_t launch_score(const DevComplex& cx, const double* d_poses, const double* d_coords,
-                        const double* d_extent, int n_poses, const Weights& w, double* d_out,
-                        int32_t* d_nsat, hipStream_t stream) {
+template <int K, bool COLL>
+static void launch_pair(const DevComplex& d_cx, const double* d_poses, const double* d_coords, const double* d_extent,
+                        int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, size_t lds,
+                        hipStream_t stream) {
+  hipLaunchKernelGGL((score_poses_kernel<K, COLL, false>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
+                     d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
+  hipLaunchKernelGGL((score_poses_kernel<K, COLL, true>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
+                     d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
+}
+
+// Fast kernel over all poses, then the exact kernel (a no-op for every pose
+// that did not overflow its settle buffer).  d_redo: n_poses int32 scratch.
+hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
+                        const double* d_extent, int n_poses, const Weights& w, double* d_out, int32_t* d_nsat,
+                        int32_t* d_redo, hipStream_t stream) {
   if (n_poses <= 0) return hipSuccess;
   const size_t lds = score_lds_bytes(cx);
-  const dim3 grid(n_poses), block(kScoreThreads);
+  const bool coll = cx.has_serial_collision != 0;
   switch (score_atoms_per_lane(cx.n_lig)) {
-#define GDOCK_CASE(KK) \
-  case KK: hipLaunchKernelGGL(score_poses_kernel<KK>, grid, block, lds, stream, cx, d_poses, d_coords, d_extent, w, d_out, d_nsat); break;
+#define GDOCK_CASE(KK)                                                                                            \
+  case KK:                                                                                                        \
+    if (coll) launch_pair<KK, true>(cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream); \
+    else launch_pair<KK, false>(cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);     \
+    break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)
 #undef GDOCK_CASE

@@ -182,3 +182,21 @@ def test_full_size_batch_properties():
     ref, rns = orc.score_poses(rec.atoms, lig.atoms, c.restraints, genes[idx], W)
     assert_energies_close(a[idx], ref, "4096-pose sample")
     assert np.array_equal(na[idx], rns)
+
+
+def test_lattice_cutoff_storm_uses_exact_path():
+    """Receptor and ligand on one integer lattice: thousands of pairs sit exactly
+    on d = 1, 8, 12 and 15, overflowing the per-wave settle buffer, so the pose
+    goes through the exact kernel.  Results must still match the reference."""
+    pts = [(x, y, z) for x in range(0, 16, 1) for y in range(0, 16, 2) for z in range(0, 16, 3)]
+    rec = structure.Molecule(mol(*[A(float(x), float(y), float(z), charge=0.5 if (x + y) % 2 else -0.5,
+                                     element="O" if z % 2 else "C", serial=i + 1, resseq=i // 8 + 1)
+                                   for i, (x, y, z) in enumerate(pts)]))
+    lig = structure.Molecule(mol(*[A(float(x) + 1.0, float(y) + 8.0, float(z), charge=-0.25 if x % 3 else 0.4,
+                                     element="N" if y % 4 else "C", serial=5000 + i, chainid="B",
+                                     resseq=i // 8 + 1) for i, (x, y, z) in enumerate(pts[:700])]))
+    cx = Complex(rec, lig)
+    got, _ = cx.score_coords(lig.xyz[None])
+    ref = [orc.vdw_energy(rec.atoms, lig.atoms), orc.elec_energy(rec.atoms, lig.atoms),
+           orc.desolv_energy(rec.atoms, lig.atoms)]
+    assert_energies_close(got[0, :3], np.array(ref), "lattice")
