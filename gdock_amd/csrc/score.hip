@@ -29,6 +29,18 @@
 
 namespace gdock {
 
+// Complex descriptors and receptor tables are read-only for a whole launch;
+// reading them through the constant address space lets the compiler use
+// scalar loads for every wave-uniform access.
+#define GDOCK_AS4 __attribute__((address_space(4)))
+using CxRef = const DevComplex GDOCK_AS4&;
+
+struct RecAtom { float x, y, z, w; };  // receptor x, y, z (FP32, origin frame), charge
+
+__device__ __forceinline__ RecAtom rec_atom(const float GDOCK_AS4* rf, int j) {
+  return RecAtom{rf[4 * j], rf[4 * j + 1], rf[4 * j + 2], rf[4 * j + 3]};
+}
+
 struct PoseCoef {
   double ca, sa, cb, sb, cg, sg;
   double m01, m02, m11, m12;  // the loop-invariant sums inside structure.rs:96-101
@@ -49,11 +61,10 @@ __device__ __forceinline__ PoseCoef load_pose(const double* __restrict__ p) {
 // FP64 pose of ligand atom l: rotate about the COM, then displace
 // (chromosome.rs:79-83 -> structure.rs:80-131).  This file is compiled with
 // -ffp-contract=off, so every product and sum rounds as in the reference.
-__device__ __forceinline__ void ligand_atom(const DevComplex& cx, const PoseCoef& pc,
-                                            const double* __restrict__ coords, int pose, int l,
+__device__ __forceinline__ void ligand_atom(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords, int l,
                                             double& X, double& Y, double& Z) {
-  if (coords) {
-    const double* p = coords + (static_cast<size_t>(pose) * cx.n_lig + l) * 3;
+  if (coords) {  // this pose's coordinates, n_lig x 3
+    const double* p = coords + static_cast<size_t>(l) * 3;
     X = p[0]; Y = p[1]; Z = p[2];
     return;
   }
@@ -125,7 +136,7 @@ __device__ __forceinline__ bool in_band(float d2, const Bands& B) {
 
 // Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
 // 1/R^6 and sqrt(eps_r eps_l) in FP64 (fitness.rs:58-63), rounded once.
-__device__ __forceinline__ void class_factors(const DevComplex& cx, int c, int l, float& ir6, float& eps) {
+__device__ __forceinline__ void class_factors(CxRef cx, int c, int l, float& ir6, float& eps) {
   const double R = cx.class_rmin2[c] + cx.lig_rmin2[l];
   const double R2 = R * R, R6 = R2 * (R2 * R2);
   ir6 = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
@@ -133,10 +144,10 @@ __device__ __forceinline__ void class_factors(const DevComplex& cx, int c, int l
 }
 
 // The reference's decisions for one pair, in FP64 on the pose's coordinates.
-__device__ __forceinline__ void exact_flags(const DevComplex& cx, const PoseCoef& pc, const double* __restrict__ coords,
-                                            int pose, int l, int j, int rser, bool& x64, bool& x144, bool& x225) {
+__device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords, int l,
+                                            int j, int rser, bool& x64, bool& x144, bool& x225) {
   double px, py, pz;
-  ligand_atom(cx, pc, coords, pose, l, px, py, pz);
+  ligand_atom(cx, pc, coords, l, px, py, pz);
   const double ex = cx.rec_x[j] - px, ey = cx.rec_y[j] - py, ez = cx.rec_z[j] - pz;
   const double e2 = ex * ex + ey * ey + ez * ez;
   x64 = e2 <= kD2DesolvIncl;
@@ -144,13 +155,13 @@ __device__ __forceinline__ void exact_flags(const DevComplex& cx, const PoseCoef
   x225 = e2 > kD2ElecMinExcl && e2 <= kD2ElecMaxIncl;
 }
 
-struct SettleEvent { int32_t j, k; uint64_t mask; };  // receptor slot, ligand slot, lanes
+struct SettleEvent { int32_t j, pad; uint64_t mask; };  // first receptor slot of a group, lanes
 constexpr int kEventsPerWave = 128;
 
 // Fast path of one receptor atom against the lane's K ligand atoms; returns
 // this wave's burial count of the receptor atom and flags band pairs.
 template <int K, bool COLL>
-__device__ __forceinline__ int32_t receptor_atom(const DevComplex& cx, int base, int j, const float4 r, const Bands& B,
+__device__ __forceinline__ int32_t receptor_atom(CxRef cx, int base, int j, const RecAtom r, const Bands& B,
                                                  LaneAtoms<K>& a, int32_t& ambv) {
   int rser = INT_MIN;  // INT_MIN: no ligand atom shares this receptor atom's serial
   if (COLL) {
@@ -178,7 +189,6 @@ __device__ __forceinline__ int32_t receptor_atom(const DevComplex& cx, int base,
     // the compiler sinks the math below the group's rare band-pair branch and
     // keeps every pair's lane masks alive across it, spilling SGPRs.
     asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+v"(ambv));
-    __builtin_amdgcn_sched_barrier(0);
   }
   return cj;
 }
@@ -186,8 +196,8 @@ __device__ __forceinline__ int32_t receptor_atom(const DevComplex& cx, int base,
 // Band pairs of one receptor atom: recorded as events (fast kernel) or
 // settled on the spot in FP64 (exact kernel).
 template <int K, bool COLL, bool EXACT>
-__device__ __forceinline__ void band_pairs(const DevComplex& cx, const PoseCoef& pc, const double* __restrict__ coords,
-                                           int pose, int base, int j, const float4 r, const Bands& B,
+__device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords,
+                                           int base, int j, const RecAtom r, const Bands& B,
                                            LaneAtoms<K>& a, SettleEvent* __restrict__ ev, int& nev, int& overflow,
                                            int32_t* __restrict__ s_rcnt) {
   const int lane = threadIdx.x & 63;
@@ -213,7 +223,7 @@ __device__ __forceinline__ void band_pairs(const DevComplex& cx, const PoseCoef&
     }
     if ((am >> lane) & 1ull) {
       bool x64, x144, x225;
-      exact_flags(cx, pc, coords, pose, l, j, rser, x64, x144, x225);
+      exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
       bool f144 = d2 <= B.lo144;
       if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
       const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
@@ -230,74 +240,88 @@ __device__ __forceinline__ void band_pairs(const DevComplex& cx, const PoseCoef&
   }
 }
 
-// Settles the wave's recorded events: each flagged pair is decided in FP64
-// and the difference to the FP32 decisions is added to the FP64 sums, the
-// lane's burial count and the receptor atom's count.
+// Settles the wave's recorded events.  An event names a group of kRecGroup
+// receptor atoms and the lanes that saw a band pair in it; each band pair is
+// decided in FP64 and the difference to the FP32 decisions is added to the
+// FP64 sums, the lane's burial count and the receptor atom's count.
 template <int K, bool COLL>
-__device__ __forceinline__ void settle_events(const DevComplex& cx, const double* __restrict__ poses,
-                                           const double* __restrict__ coords, int pose, int base, const Bands& B,
-                                           const SettleEvent* __restrict__ ev, int nev, LaneAtoms<K>& a,
-                                           double& sum_v, double& sum_e, int32_t* __restrict__ s_rcnt) {
+__device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict__ pose_rec,
+                                              const double* __restrict__ coords, int base, const Bands& B,
+                                              const SettleEvent* __restrict__ ev, int nev, LaneAtoms<K>& a,
+                                              double& sum_v, double& sum_e, int32_t* __restrict__ s_rcnt) {
   const int lane = threadIdx.x & 63;
-  PoseCoef pc{};
-  if (!coords) pc = load_pose(poses + static_cast<size_t>(kPoseDoubles) * pose);
+  const int32_t GDOCK_AS4* cstart = (const int32_t GDOCK_AS4*)cx.class_start;
+  const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
   for (int e = 0; e < nev; ++e) {
     const SettleEvent E = ev[e];
     if (!((E.mask >> lane) & 1ull)) continue;
     int c = 0;
-    while (cx.class_start[c + 1] <= E.j) ++c;
-    const float4 r = cx.rec_xyzq[E.j];
-    int rser = INT_MIN;
-    if (COLL) {
-      const int2 m = cx.rec_meta[E.j];
-      rser = m.y ? m.x : INT_MIN;
-    }
+    while (cstart[c + 1] <= E.j) ++c;  // groups never straddle classes
+    for (int j = E.j; j < E.j + kRecGroup; ++j) {
+      const RecAtom r = rec_atom(rq, j);
+      int rser = INT_MIN;
+      if (COLL) {
+        const int2 m = cx.rec_meta[j];
+        rser = m.y ? m.x : INT_MIN;
+      }
+#pragma unroll 1
+      for (int k = 0; k < K; ++k) {  // one copy of the FP64 path: pick slot k by value
+        float lx = a.x[0], ly = a.y[0], lz = a.z[0];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (k != E.k) continue;
-      const int l = base + threadIdx.x + kScoreThreads * k;
-      float ir6, eps;
-      class_factors(cx, c, l, ir6, eps);
-      const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
-      bool x64, x144, x225;
-      exact_flags(cx, pc, coords, pose, l, E.j, rser, x64, x144, x225);
-      bool f144 = d2 <= B.lo144;
-      if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
-      const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
-      float tv, id2;
-      pair_terms(d2, ir6, tv, id2);
-      if (x144 != f144) sum_v += static_cast<double>(x144 ? tv : -tv) * static_cast<double>(eps);
-      if (x225 != f225)
-        sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
-      const int dc = static_cast<int>(x64) - static_cast<int>(f64);
-      if (dc) {
-        a.cnt[k] += dc;
-        atomicAdd(&s_rcnt[E.j], dc);
+        for (int kk = 1; kk < K; ++kk)
+          if (kk == k) { lx = a.x[kk]; ly = a.y[kk]; lz = a.z[kk]; }
+        const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
+        const int l = base + threadIdx.x + kScoreThreads * k;
+        if (!in_band(d2, B) || l >= cx.n_lig) continue;
+        PoseCoef pc{};
+        if (!coords) pc = load_pose(pose_rec);
+        float ir6, eps;
+        class_factors(cx, c, l, ir6, eps);
+        bool x64, x144, x225;
+        exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
+        bool f144 = d2 <= B.lo144;
+        if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
+        const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
+        float tv, id2;
+        pair_terms(d2, ir6, tv, id2);
+        if (x144 != f144) sum_v += static_cast<double>(x144 ? tv : -tv) * static_cast<double>(eps);
+        if (x225 != f225)
+          sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
+        const int dc = static_cast<int>(x64) - static_cast<int>(f64);
+        if (dc) {
+#pragma unroll
+          for (int kk = 0; kk < K; ++kk) a.cnt[kk] += kk == k ? dc : 0;
+          atomicAdd(&s_rcnt[j], dc);
+        }
       }
     }
   }
 }
 
+// Scores one pose with the whole workgroup.  pose_rec: the pose record (12
+// doubles) or null; coords: this pose's ligand coordinates (n_lig x 3) when
+// scoring coordinates as given, with `extent` = their largest |x - origin|.
+// o: 6 outputs; *redo is set when the fast variant could not settle every
+// band pair (the exact variant then rescores the pose).
 template <int K, bool COLL, bool EXACT>
-__global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
-    const DevComplex cx, const double* __restrict__ poses, const double* __restrict__ coords,
-    const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
-    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
-  const int pose = blockIdx.x;
-  if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
-  extern __shared__ int32_t s_rcnt[];    // n_rec burial counters, then reduction scratch
+__device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ pose_rec,
+                                           const double* __restrict__ coords, double extent, const Weights& w,
+                                           double* __restrict__ o, int32_t* __restrict__ nsat_out,
+                                           int32_t* __restrict__ redo, int32_t* __restrict__ s_rcnt,
+                                           int32_t* s_overflow) {
   double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
   SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
-  __shared__ int32_t s_overflow;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SettleEvent* my_ev = s_ev + wave * kEventsPerWave;
+  const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
+  const int32_t GDOCK_AS4* cstart = (const int32_t GDOCK_AS4*)cx.class_start;
 
   PoseCoef pc{};
   double X;
   if (coords) {
-    X = fmax(cx.rec_extent, pose_extent[pose]);
+    X = fmax(cx.rec_extent, extent);
   } else {
-    pc = load_pose(poses + static_cast<size_t>(kPoseDoubles) * pose);
+    pc = load_pose(pose_rec);
     X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
   }
   Bands B;
@@ -307,7 +331,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
   guard_band(225.0, X, B.lo225, B.hi225);
 
   for (int i = tid; i < cx.n_rec; i += kScoreThreads) s_rcnt[i] = 0;
-  if (tid == 0) s_overflow = 0;
+  if (tid == 0) *s_overflow = 0;
   __syncthreads();
 
   double sum_v = 0.0, sum_e = 0.0, sum_d = 0.0;
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
       const int l = base + tid + kScoreThreads * k;
       if (l < cx.n_lig) {
         double px, py, pz;
-        ligand_atom(cx, pc, coords, pose, l, px, py, pz);
+        ligand_atom(cx, pc, coords, l, px, py, pz);
         a.x[k] = static_cast<float>(px - cx.origin[0]);
         a.y[k] = static_cast<float>(py - cx.origin[1]);
         a.z[k] = static_cast<float>(pz - cx.origin[2]);
@@ -335,14 +359,14 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
       int32_t rc = 0;  // lane t: burial count of receptor slot g0 + t from this wave
       for (int j = g0; j < g1; j += kRecGroup) {
         if (j == cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
-          do { ++cls; cend = cx.class_start[cls + 1]; } while (cend == j);
+          do { ++cls; cend = cstart[cls + 1]; } while (cend == j);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             const int l = base + tid + kScoreThreads * k;
             if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
           }
         }
-        const float4 r0 = cx.rec_xyzq[j], r1 = cx.rec_xyzq[j + 1], r2 = cx.rec_xyzq[j + 2], r3 = cx.rec_xyzq[j + 3];
+        const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
         int32_t ambv = 0;
         const int t = lane - (j - g0);
         const int32_t c0 = receptor_atom<K, COLL>(cx, base, j, r0, B, a, ambv);
@@ -353,11 +377,19 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
         rc = t == 2 ? c2 : rc;
         const int32_t c3 = receptor_atom<K, COLL>(cx, base, j + 3, r3, B, a, ambv);
         rc = t == 3 ? c3 : rc;
-        if (__builtin_amdgcn_ballot_w64(ambv != 0) != 0ull) {  // rare
-          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
-          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
-          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
-          band_pairs<K, COLL, EXACT>(cx, pc, coords, pose, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
+        const uint64_t amask = __builtin_amdgcn_ballot_w64(ambv != 0);
+        if (amask != 0ull) {  // rare: this group holds band pairs
+          if (EXACT) {
+            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
+          } else if (nev < kEventsPerWave) {  // settled once per ligand chunk
+            if (lane == 0) my_ev[nev] = SettleEvent{j, 0, amask};
+            ++nev;
+          } else {
+            overflow = 1;
+          }
         }
       }
       if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane], rc);
@@ -372,7 +404,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
       }
     }
     if (!EXACT && nev > 0)
-      settle_events<K, COLL>(cx, poses, coords, pose, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
+      settle_events<K, COLL>(cx, pose_rec, coords, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int l = base + tid + kScoreThreads * k;
@@ -386,10 +418,11 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
       }
     }
   }
-  if (overflow && lane == 0) s_overflow = 1;
+  if (overflow && lane == 0) *s_overflow = 1;
   __syncthreads();
-  if (!EXACT && s_overflow) {  // the exact variant rescores this pose
-    if (tid == 0) redo[pose] = 1;
+  if (!EXACT && *s_overflow) {  // the exact variant rescores this pose
+    if (tid == 0) *redo = 1;
+    __syncthreads();
     return;
   }
   for (int j = tid; j < cx.n_rec; j += kScoreThreads) {  // fitness.rs:141-162
@@ -423,7 +456,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     int32_t nsat = 0;
     for (int i = 0; i < cx.n_restr; ++i) {
       double px, py, pz;
-      ligand_atom(cx, pc, coords, pose, cx.restr_lig[i], px, py, pz);
+      ligand_atom(cx, pc, coords, cx.restr_lig[i], px, py, pz);
       const double ex = cx.restr_rec_xyz[3 * i] - px, ey = cx.restr_rec_xyz[3 * i + 1] - py,
                    ez = cx.restr_rec_xyz[3 * i + 2] - pz;
       const double e2 = ex * ex + ey * ey + ez * ez;
@@ -434,15 +467,61 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     }
     const double n = static_cast<double>(cx.n_restr);
     const double ratio = cx.n_restr > 0 ? static_cast<double>(nsat) / n : 0.0;
-    double* o = out + static_cast<size_t>(kOutDoubles) * pose;
     o[0] = vdw;
     o[1] = elec;
     o[2] = desolv;
     o[3] = air;
     o[4] = (1.0 - ratio) * n;  // chromosome.rs:117-118
     o[5] = w.vdw * vdw + w.elec * elec + w.desolv * desolv + w.air * air;  // chromosome.rs:121-124
-    if (nsat_out) nsat_out[pose] = nsat;
-    if (!EXACT) redo[pose] = 0;
+    if (nsat_out) *nsat_out = nsat;
+    if (!EXACT) *redo = 0;
+  }
+  __syncthreads();  // LDS is reused by the workgroup's next pose
+}
+
+template <int K, bool COLL, bool EXACT>
+__global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
+    const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
+    const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
+    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
+  const int pose = blockIdx.x;
+  if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
+  extern __shared__ int32_t s_lds[];
+  __shared__ int32_t s_overflow;
+  CxRef cx = *(const DevComplex GDOCK_AS4*)d_cx;
+  score_pose<K, COLL, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
+                             coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr,
+                             coords ? pose_extent[pose] : 0.0, w, out + static_cast<size_t>(kOutDoubles) * pose,
+                             nsat_out ? nsat_out + pose : nullptr, redo + pose, s_lds, &s_overflow);
+}
+
+// Persistent form: each workgroup takes (complex, pose) items from a shared
+// counter until the list is exhausted.  Items come longest first.
+template <int K, bool COLL, bool EXACT>
+__global__ __launch_bounds__(kScoreThreads) void score_items_kernel(
+    const DevComplex* __restrict__ complexes, const int2* __restrict__ items, int n_items,
+    int32_t* __restrict__ counter, const double* __restrict__ poses, const Weights w, double* __restrict__ out,
+    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
+  extern __shared__ int32_t s_lds[];
+  __shared__ int32_t s_overflow, s_item;
+  for (;;) {
+    if (EXACT) {  // one item per workgroup: only items whose settle buffer overflowed
+      if (blockIdx.x >= static_cast<unsigned>(n_items)) return;
+    } else {
+      if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+      __syncthreads();
+    }
+    const int it = EXACT ? static_cast<int>(blockIdx.x) : s_item;
+    __syncthreads();
+    if (it >= n_items) return;
+    const int2 I = items[it];  // {complex, global pose index}
+    if (!EXACT || redo[I.y] != 0) {
+      CxRef cx = ((const DevComplex GDOCK_AS4*)complexes)[I.x];
+      score_pose<K, COLL, EXACT>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, nullptr, 0.0, w,
+                                 out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds,
+                                 &s_overflow);
+    }
+    if (EXACT) return;
   }
 }
 
@@ -457,7 +536,7 @@ int score_atoms_per_lane(int n_lig) {
 }
 
 template <int K, bool COLL>
-static void launch_pair(const DevComplex& d_cx, const double* d_poses, const double* d_coords, const double* d_extent,
+static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, const double* d_extent,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, size_t lds,
                         hipStream_t stream) {
   hipLaunchKernelGGL((score_poses_kernel<K, COLL, false>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
@@ -477,8 +556,8 @@ hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const doub
   switch (score_atoms_per_lane(cx.n_lig)) {
 #define GDOCK_CASE(KK)                                                                                            \
   case KK:                                                                                                        \
-    if (coll) launch_pair<KK, true>(cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream); \
-    else launch_pair<KK, false>(cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);     \
+    if (coll) launch_pair<KK, true>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream); \
+    else launch_pair<KK, false>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);     \
     break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)
