@@ -34,18 +34,54 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="batched_scoring", choices=["batched_scoring", "uneven_batch"])
     ap.add_argument("--poses", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference kernel for PMC byte counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "score_poses_kernel_2000x1500x4096_traffic.txt"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the scoring kernel, from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
+    size); None when no such summary is present."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 30
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard "
               "across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven_batch":
+        bench_uneven(args)
+        return
     import torch
     from gdock_amd import _lib, structure, synth
     from gdock_amd.device import Complex, default_context
@@ -129,13 +165,92 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "score_poses_kernel",
             "flop_per_launch": flop,
         },
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(rec, lig, c.restraints, genes, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+def bench_uneven(args):
+    """64 complexes (receptor 500-8000, ligand 300-3000 atoms, seeds 100-163),
+    150 poses each (the GA's initial population: angles U[0, 2 pi], shifts
+    U[-50, 50] A).  Batched: one replay of the captured generation graph
+    (persistent kernel) per step.  Plain: the drop-in path one job after
+    another (gdock_score_poses per complex).  Both take genes from the host and
+    return results to the host; kernel-only times come from HIP events."""
+    import torch
+    from gdock_amd import _lib, structure, synth
+    from gdock_amd.device import Batch, Complex, default_context
+
+    per = 150
+    cs = synth.uneven_batch(64, 100)
+    ctx = default_context(0)
+    cxs, genes, pairs = [], [], 0.0
+    for i, c in enumerate(cs):
+        rec = structure.read_pdb_text(c.receptor_pdb)[0]
+        lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+        cxs.append(Complex(rec, lig, c.restraints, ctx))
+        genes.append(synth.ga_genes(100 + i, per))
+        pairs += float(len(rec)) * len(lig) * per
+    t0 = time.perf_counter()
+    b = Batch(cxs, [per] * len(cxs), ctx)
+    create_s = time.perf_counter() - t0
+    g = np.concatenate(genes)
+    for _ in range(args.warmup):
+        b.score(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, _ = b.score(g)
+    wall_b = (time.perf_counter() - t0) / args.steps
+    assert np.isfinite(out).all()
+    # kernel-only: graph replays on one stream, inputs already in HBM
+    stream = torch.cuda.Stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        b.launch(None, stream.cuda_stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    kern_b = ev0.elapsed_time(ev1) / args.steps * 1e-3
+    # plain drop-in, one job after another
+    for cx, gi in zip(cxs, genes):
+        cx.score_poses(gi)
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        for cx, gi in zip(cxs, genes):
+            cx.score_poses(gi)
+    wall_p = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    n = per * len(cxs)
+    flop = FLOP_PER_PAIR * pairs
+    plan = b.plan()
+    result = {
+        "metric": BASELINE_METRIC,
+        "value": n / wall_b,
+        "unit": "pose-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall_b * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded complexes and poses)",
+        "config": {"workload": f"uneven_batch: 64 complexes (seeds 100-163, receptor 500-8000 and ligand "
+                               f"300-3000 atoms), {per} GA-initial poses each, host genes in / results out",
+                   "parallelism": "single GPU"},
+        "plain_pose_evals_per_s": n / wall_p,
+        "speedup_vs_plain": wall_p / wall_b,
+        "kernel_ms": kern_b * 1e3,
+        "kernel_pose_evals_per_s": n / kern_b,
+        "pair_interactions_per_s": pairs / kern_b,
+        "batch_create_s": create_s,
+        "plan": {"grid": plan["grid"], "lds_bytes": plan["lds_bytes"],
+                 "lanes_k_histogram": {str(k): plan["lanes_k"].count(k) for k in sorted(set(plan["lanes_k"]))}},
+        "roofline": {"bound": "valu", "achieved": flop / kern_b / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": flop / kern_b / PEAK_FP32, "traffic": None, "kernel": "score_batch_kernel",
+                     "flop_per_launch": flop},
+    }
     print(json.dumps(result), flush=True)
 
 

@@ -68,6 +68,15 @@ def lib() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m gdock_amd.build` "
                           "(the scoring path has no CPU fallback)")
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 with the
+    # same soname as /opt/rocm's: once torch is loaded, this library binds to
+    # that copy, so torch streams and tensors can be handed to it.  Loaded the
+    # other way round, the process would hold two runtimes and torch's would
+    # find no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     sig = {
         "gdock_version": (C.c_char_p, []),
@@ -92,6 +101,13 @@ def lib() -> C.CDLL:
         "gdock_contacts": (C.c_int, [P, P, D, C.c_int32, C.c_double, C.POINTER(C.c_uint64), I32]),
         "gdock_cluster_structures": (C.c_int, [P, P, D, C.c_int32, C.POINTER(ClusteringConfigC),
                                                C.POINTER(ClusterResultC), C.POINTER(C.c_int64), I32]),
+        "gdock_batch_create": (C.c_int, [P, C.POINTER(P), C.c_int32, I32, C.POINTER(P)]),
+        "gdock_batch_destroy": (None, [P, P]),
+        "gdock_batch_n_poses": (C.c_int32, [P]),
+        "gdock_batch_plan": (C.c_int, [P, I32, I32, I32]),
+        "gdock_batch_score": (C.c_int, [P, P, D, C.POINTER(Weights), D, I32]),
+        "gdock_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "gdock_batch_launch": (C.c_int, [P, P, C.POINTER(Weights), P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -103,12 +119,8 @@ def lib() -> C.CDLL:
 
 
 def _register_optional(L: C.CDLL) -> None:
-    """Entry points of the batched, GA and serving layers."""
+    """Entry points of the GA and serving layers."""
     opt = {
-        "gdock_batch_create": (C.c_int, [P, C.POINTER(P), C.c_int32, I32, C.POINTER(P)]),
-        "gdock_batch_destroy": (None, [P, P]),
-        "gdock_batch_score": (C.c_int, [P, P, D, I32, C.POINTER(Weights), D, I32, C.c_int]),
-        "gdock_batch_plan": (C.c_int, [P, I32, I32]),
         "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.c_int32, C.c_int32, C.c_int, P, C.POINTER(P)]),
         "gdock_ga_free": (None, [P]),
         "gdock_ga_info": (C.c_int, [P, I32]),

@@ -13,8 +13,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libgdock_hip.so"
-SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip"]
-EXTRA_SOURCES = ["batch.hip", "ga.cpp", "serve.cpp"]
+SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip", "batch.cpp", "ga.cpp"]
+EXTRA_SOURCES = ["serve.cpp"]
 
 # -ffp-contract=off: FP64 sums and products round as in the reference (no
 # implicit FMA); the FP32 pair loop spells its FMAs out.  Packed FP32

@@ -11,7 +11,7 @@
 #include <map>
 #include <string>
 #include <string_view>
-#include <unordered_set>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -197,15 +197,17 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   double origin[3] = {0, 0, 0}, lcom[3] = {0, 0, 0};
   if (n_rec) com_of(rec, n_rec, origin);
   if (n_lig) com_of(lig, n_lig, lcom);
-  std::unordered_set<int32_t> lig_serials;
-  for (auto& a : cx->lig) lig_serials.insert(a.serial);
+  std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;
+  for (int32_t i = 0; i < n_lig; ++i) lig_by_serial[cx->lig[i].serial].push_back(i);
 
   std::vector<float4> xyzq(n_pad);
   std::vector<double> rx(n_pad), ry(n_pad), rz(n_pad), rasp(n_pad, 0.0), rrad(n_pad, 0.0);
   std::vector<int2> meta(n_pad, make_int2(0, 0));
   double rec_ext = 0.0;
-  int has_coll = 0;
+  std::vector<int4> coll;  // fitness.rs:92 pairs: same serial in receptor and ligand
+  int cls = 0;
   for (int32_t j = 0; j < n_pad; ++j) {
+    while (class_start[cls + 1] <= j) ++cls;
     if (perm[j] < 0) {
       xyzq[j] = make_float4(kPadRecCoord, kPadRecCoord, kPadRecCoord, 0.0f);
       rx[j] = ry[j] = rz[j] = static_cast<double>(kPadRecCoord);
@@ -219,9 +221,10 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     rx[j] = a.x; ry[j] = a.y; rz[j] = a.z;
     rasp[j] = asp_of(a);
     rrad[j] = a.vdw_radius;
-    const int c = lig_serials.count(a.serial) ? 1 : 0;
-    meta[j] = make_int2(a.serial, c);
-    has_coll |= c;
+    const auto hit = lig_by_serial.find(a.serial);
+    meta[j] = make_int2(a.serial, hit != lig_by_serial.end() ? 1 : 0);
+    if (hit != lig_by_serial.end())
+      for (int32_t l : hit->second) coll.push_back(make_int4(j, l, cls, 0));
   }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);
@@ -255,7 +258,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   Layout L;
   const size_t o_xyzq = L.add<float4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
                o_rz = L.add<double>(n_pad), o_rasp = L.add<double>(n_pad), o_rrad = L.add<double>(n_pad),
-               o_meta = L.add<int2>(n_pad), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
+               o_meta = L.add<int2>(n_pad), o_coll = L.add<int4>(coll.size()), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
                o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
@@ -270,6 +273,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_rx, rx.data(), n_pad * 8); put(o_ry, ry.data(), n_pad * 8); put(o_rz, rz.data(), n_pad * 8);
   put(o_rasp, rasp.data(), n_pad * 8); put(o_rrad, rrad.data(), n_pad * 8);
   put(o_meta, meta.data(), n_pad * sizeof(int2));
+  put(o_coll, coll.data(), coll.size() * sizeof(int4));
   put(o_cs, class_start.data(), (nc + 1) * 4); put(o_crm, class_rmin.data(), nc * 8); put(o_ceps, class_eps.data(), nc * 8);
   put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
   put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
@@ -290,6 +294,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.rec_asp = reinterpret_cast<const double*>(m + o_rasp);
   d.rec_radius = reinterpret_cast<const double*>(m + o_rrad);
   d.rec_meta = reinterpret_cast<const int2*>(m + o_meta);
+  d.coll_pairs = reinterpret_cast<const int4*>(m + o_coll);
   d.class_start = reinterpret_cast<const int32_t*>(m + o_cs);
   d.class_rmin2 = reinterpret_cast<const double*>(m + o_crm);
   d.class_eps = reinterpret_cast<const double*>(m + o_ceps);
@@ -305,7 +310,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
   d.n_rec = n_pad; d.n_rec_atoms = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
-  d.has_serial_collision = has_coll;
+  d.n_coll = static_cast<int32_t>(coll.size());
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }
   d.rec_extent = rec_ext;
   d.lig_extent = lig_r + com_off;

@@ -1,0 +1,334 @@
+// Batched scoring of many complexes in one persistent launch, replayed from a
+// captured HIP graph, with a measured choice of ligand atoms per lane.
+//
+// A batch is a fixed set of complexes with a fixed number of poses each (one
+// GA generation of every complex).  Its items (complex, pose) are ordered
+// longest first; one persistent kernel drains them from a device counter, and
+// a second kernel rescores the rare items whose settle buffer overflowed.
+// Both launches are captured once into a graph; every generation replays it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+#include "internal.h"
+
+using namespace gdock;
+
+struct gdock_batch {
+  gdock_ctx* ctx = nullptr;
+  std::vector<gdock_complex*> cx;
+  std::vector<int32_t> pose_off;  // n + 1
+  std::vector<int32_t> lanes_k;
+  std::vector<int2> items;
+  int32_t n_poses = 0;
+  int32_t max_n_rec = 0;
+  BatchLaunch L{};
+  void* mem = nullptr;  // device: table, items, n_items, counters, poses, out, nsat, redo
+  double* h_poses = nullptr;  // pinned host staging
+  double* h_out = nullptr;
+  int32_t* h_nsat = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Weights gw{};
+  bool have_graph = false;
+};
+
+namespace gdock {
+
+// Ligand atoms per lane, by measurement.  A ligand of n atoms is scored in
+// ceil(n / (256 K)) chunks of 256 K lanes-slots; each chunk walks the whole
+// receptor.  Candidates are the K of each chunk count (the smallest K that
+// covers the ligand in that many chunks); the fastest on a probe launch wins.
+// Decisions are cached per (ligand atoms, receptor size class) for the
+// process, so every complex of the same shape gets the same K (and so the
+// same summation order) whatever batch it is in.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, int> g_tuned;
+
+std::vector<int> lane_candidates(int n_lig) {
+  std::vector<int> ks;
+  for (int chunks = 1; chunks <= 16; ++chunks) {
+    const int k = (n_lig + kScoreThreads * chunks - 1) / (kScoreThreads * chunks);
+    if (k < 1) break;
+    if (k > kMaxAtomsPerLane) continue;
+    if (std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
+    if (ks.size() >= 3) break;
+  }
+  if (ks.empty()) ks.push_back(1);
+  return ks;
+}
+
+int rec_size_class(int n_rec) { return static_cast<int>(std::lround(2.0 * std::log2(std::max(n_rec, 1)))); }
+}  // namespace
+
+int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err) {
+  *err = hipSuccess;
+  const int n_lig = cx->dev.n_lig;
+  const std::pair<int, int> key{n_lig, rec_size_class(cx->dev.n_rec_atoms)};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) return it->second;
+  }
+  const std::vector<int> ks = lane_candidates(n_lig);
+  int best = ks.front();
+  if (ks.size() > 1) {
+    // Probe poses: fixed rotations, the ligand kept on the receptor.
+    constexpr int kProbe = 128;
+    std::vector<double> genes(kProbe * 6), poses(kProbe * kPoseDoubles);
+    uint64_t s = 0x9e3779b97f4a7c15ull;
+    auto u01 = [&s]() {
+      s = s * 6364136223846793005ull + 1442695040888963407ull;
+      return static_cast<double>(s >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int i = 0; i < kProbe; ++i) {
+      for (int a = 0; a < 3; ++a) genes[6 * i + a] = 6.283185307179586 * u01();
+      for (int a = 3; a < 6; ++a) genes[6 * i + a] = 10.0 * (u01() - 0.5);
+    }
+    gdock_prepare_poses(genes.data(), kProbe, poses.data());
+    double* d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&d, kProbe * (kPoseDoubles + kOutDoubles + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d, poses.data(), poses.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    double* d_out = d + kProbe * kPoseDoubles;
+    int32_t* d_ns = reinterpret_cast<int32_t*>(d_out + kProbe * kOutDoubles);
+    int32_t* d_redo = d_ns + kProbe;
+    const Weights w{0.4, 0.05, 3.4, 1.0};
+    float best_ms = 0.0f;
+    for (size_t c = 0; c < ks.size() && e == hipSuccess; ++c) {
+      e = launch_score(cx->dev, cx->d_dev, d, nullptr, nullptr, kProbe, w, d_out, d_ns, d_redo, ctx->stream, ks[c]);
+      if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+      for (int r = 0; r < 2 && e == hipSuccess; ++r)
+        e = launch_score(cx->dev, cx->d_dev, d, nullptr, nullptr, kProbe, w, d_out, d_ns, d_redo, ctx->stream, ks[c]);
+      if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && (c == 0 || ms < best_ms)) { best_ms = ms; best = ks[c]; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d) (void)hipFree(d);
+    if (e != hipSuccess) { *err = e; return 0; }
+  }
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tuned.emplace(key, best).first->second;
+}
+
+}  // namespace gdock
+
+extern "C" {
+
+gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes, int32_t n,
+                                const int32_t* poses_per_complex, gdock_batch** out) {
+  if (!ctx || !out || n < 0 || (n > 0 && (!complexes || !poses_per_complex))) {
+    if (ctx) set_error(ctx, "batch_create: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  int64_t total = 0;
+  int max_rec = 1;
+  for (int32_t i = 0; i < n; ++i) {
+    if (!complexes[i] || poses_per_complex[i] < 0) {
+      set_error(ctx, "batch_create: complex %d: null complex or negative pose count", i);
+      return GDOCK_ERR_INVALID_ARG;
+    }
+    if (complexes[i]->dev.n_rec > kMaxLdsReceptor) {
+      set_error(ctx, "batch_create: complex %d: %d receptor slots exceed the LDS table (%d)", i,
+                complexes[i]->dev.n_rec, kMaxLdsReceptor);
+      return GDOCK_ERR_INVALID_ARG;
+    }
+    total += poses_per_complex[i];
+    max_rec = std::max(max_rec, complexes[i]->dev.n_rec);
+  }
+  if (total > (1 << 30)) {
+    set_error(ctx, "batch_create: %lld poses is too many", static_cast<long long>(total));
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch_create: hipSetDevice");
+
+  auto* b = new gdock_batch();
+  b->ctx = ctx;
+  b->cx.assign(complexes, complexes + n);
+  b->pose_off.assign(n + 1, 0);
+  for (int32_t i = 0; i < n; ++i) b->pose_off[i + 1] = b->pose_off[i] + poses_per_complex[i];
+  b->n_poses = static_cast<int32_t>(total);
+  b->max_n_rec = max_rec;
+  b->lanes_k.resize(n);
+  for (int32_t i = 0; i < n; ++i) {
+    b->lanes_k[i] = tune_lanes(ctx, b->cx[i], &e);
+    if (e != hipSuccess) { delete b; return hip_fail(ctx, e, "batch_create: autotune"); }
+  }
+
+  // Items longest first: cost = receptor slots x ligand lane-slots walked.
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int32_t i) {
+    const DevComplex& d = b->cx[i]->dev;
+    const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * b->lanes_k[i];
+    const int64_t chunks = (d.n_lig + per_chunk - 1) / per_chunk;
+    return static_cast<int64_t>(d.n_rec) * chunks * per_chunk;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
+  for (int32_t i : order)
+    for (int32_t p = b->pose_off[i]; p < b->pose_off[i + 1]; ++p) b->items.push_back(make_int2(i, p));
+
+  // Device memory.
+  std::vector<DevComplex> table(std::max(n, 1));
+  for (int32_t i = 0; i < n; ++i) {
+    table[i] = b->cx[i]->dev;
+    table[i].lanes_k = b->lanes_k[i];
+  }
+  const size_t np = std::max<int64_t>(total, 1);
+  Layout Ly;
+  const size_t o_tab = Ly.add<DevComplex>(table.size()), o_items = Ly.add<int2>(np), o_n = Ly.add<int32_t>(4),
+               o_poses = Ly.add<double>(np * kPoseDoubles), o_out = Ly.add<double>(np * kOutDoubles),
+               o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np);
+  e = hipMalloc(&b->mem, Ly.size);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_poses), np * kPoseDoubles * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_out), np * kOutDoubles * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_nsat), np * sizeof(int32_t));
+  if (e != hipSuccess) {
+    gdock_batch_destroy(ctx, b);
+    return hip_fail(ctx, e, "batch_create: allocation");
+  }
+  char* m = static_cast<char*>(b->mem);
+  int32_t head[4] = {static_cast<int32_t>(b->items.size()), 0, 0, 0};  // n_items, counter, exit count
+  std::vector<uint8_t> blob(o_poses, 0);
+  std::memcpy(blob.data() + o_tab, table.data(), table.size() * sizeof(DevComplex));
+  if (!b->items.empty()) std::memcpy(blob.data() + o_items, b->items.data(), b->items.size() * sizeof(int2));
+  std::memcpy(blob.data() + o_n, head, sizeof head);
+  e = hipMemcpyAsync(m, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m + o_ns, 0, (o_redo - o_ns) + np * sizeof(int32_t), ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    gdock_batch_destroy(ctx, b);
+    return hip_fail(ctx, e, "batch_create: upload");
+  }
+  BatchLaunch& L = b->L;
+  L.table = reinterpret_cast<const DevComplex*>(m + o_tab);
+  L.items = reinterpret_cast<const int2*>(m + o_items);
+  L.n_items = reinterpret_cast<const int32_t*>(m + o_n);
+  L.counter = reinterpret_cast<int32_t*>(m + o_n) + 1;
+  L.exit_count = reinterpret_cast<int32_t*>(m + o_n) + 2;
+  L.poses = reinterpret_cast<const double*>(m + o_poses);
+  L.out = reinterpret_cast<double*>(m + o_out);
+  L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
+  L.redo = reinterpret_cast<int32_t*>(m + o_redo);
+  L.lds = batch_lds_bytes(max_rec);
+  L.capacity = static_cast<int>(std::max<size_t>(b->items.size(), 1));
+  L.grid = std::min(batch_grid(L.lds, ctx->device), L.capacity);
+  *out = b;
+  return GDOCK_OK;
+}
+
+void gdock_batch_destroy(gdock_ctx* ctx, gdock_batch* b) {
+  if (!b) return;
+  if (ctx) { (void)hipSetDevice(ctx->device); (void)hipStreamSynchronize(ctx->stream); }
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->mem) (void)hipFree(b->mem);
+  if (b->h_poses) (void)hipHostFree(b->h_poses);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_nsat) (void)hipHostFree(b->h_nsat);
+  delete b;
+}
+
+int32_t gdock_batch_n_poses(const gdock_batch* b) { return b ? b->n_poses : -1; }
+
+gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* grid, int32_t* lds_bytes) {
+  if (!b) return GDOCK_ERR_INVALID_ARG;
+  if (lanes_k) std::copy(b->lanes_k.begin(), b->lanes_k.end(), lanes_k);
+  if (grid) *grid = b->L.grid;
+  if (lds_bytes) *lds_bytes = static_cast<int32_t>(b->L.lds);
+  return GDOCK_OK;
+}
+
+gdock_status gdock_batch_device_buffers(gdock_batch* b, double** d_poses, double** d_out, int32_t** d_nsat) {
+  if (!b) return GDOCK_ERR_INVALID_ARG;
+  if (d_poses) *d_poses = const_cast<double*>(b->L.poses);
+  if (d_out) *d_out = b->L.out;
+  if (d_nsat) *d_nsat = b->L.nsat;
+  return GDOCK_OK;
+}
+
+// Captures (once per set of weights) and replays the generation graph.
+gdock_status gdock_batch_launch(gdock_ctx* ctx, gdock_batch* b, const gdock_weights* w, void* stream) {
+  if (!ctx || !b || !w) {
+    if (ctx) set_error(ctx, "batch_launch: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (b->items.empty()) return GDOCK_OK;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch_launch: hipSetDevice");
+  if (!b->have_graph || std::memcmp(&ww, &b->gw, sizeof ww) != 0) {
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->have_graph = false;
+    hipStream_t cap = nullptr;
+    e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t el = launch_batch(b->L, ww, cap);
+      e = hipStreamEndCapture(cap, &b->graph);
+      if (e == hipSuccess) e = el;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return hip_fail(ctx, e, "batch_launch: graph capture");
+    b->gw = ww;
+    b->have_graph = true;
+  }
+  e = hipGraphLaunch(b->exec, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch_launch: hipGraphLaunch");
+  return GDOCK_OK;
+}
+
+gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* genes, const gdock_weights* w,
+                               double* out, int32_t* n_satisfied) {
+  if (!ctx || !b || !w || (b->n_poses > 0 && (!genes || !out))) {
+    if (ctx) set_error(ctx, "batch_score: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  const int32_t n = b->n_poses;
+  if (n == 0) return GDOCK_OK;
+  for (int64_t i = 0; i < static_cast<int64_t>(n) * 6; ++i)
+    if (!std::isfinite(genes[i])) {
+      set_error(ctx, "batch_score: gene %lld is not finite", static_cast<long long>(i));
+      return GDOCK_ERR_INVALID_ARG;
+    }
+  gdock_prepare_poses(genes, n, b->h_poses);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<double*>(b->L.poses), b->h_poses, static_cast<size_t>(n) * kPoseDoubles * 8,
+                       hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch_score: upload");
+  gdock_status st = gdock_batch_launch(ctx, b, w, nullptr);
+  if (st != GDOCK_OK) return st;
+  e = hipMemcpyAsync(b->h_out, b->L.out, static_cast<size_t>(n) * kOutDoubles * 8, hipMemcpyDeviceToHost,
+                     ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->h_nsat, b->L.nsat, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch_score: download");
+  std::memcpy(out, b->h_out, static_cast<size_t>(n) * kOutDoubles * 8);
+  if (n_satisfied) std::memcpy(n_satisfied, b->h_nsat, static_cast<size_t>(n) * 4);
+  return GDOCK_OK;
+}
+
+}  // extern "C"

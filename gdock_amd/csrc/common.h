@@ -40,6 +40,7 @@ struct DevComplex {
   const double* rec_asp;       // atomic solvation parameter (0 = skipped)
   const double* rec_radius;
   const int2* rec_meta;        // {serial, 1 if the serial also occurs in the ligand}
+  const int4* coll_pairs;      // {receptor slot, ligand atom, receptor class, 0}: equal serials
   const int32_t* class_start;  // n_class + 1
   const double* class_rmin2;
   const double* class_eps;
@@ -59,7 +60,8 @@ struct DevComplex {
   int32_t n_rec;        // receptor slots including class padding (multiple of kRecGroup)
   int32_t n_rec_atoms;  // real receptor atoms
   int32_t n_lig, n_class, n_restr;
-  int32_t has_serial_collision;
+  int32_t n_coll;       // pairs in coll_pairs
+  int32_t lanes_k;      // ligand atoms per lane chosen for batched launches (0: ceil(n_lig / 256), at most 8)
   double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
   double origin[3];  // FP32 frame origin
   double rec_extent;   // max |rec - origin| component
@@ -67,5 +69,22 @@ struct DevComplex {
 };
 
 struct Weights { double vdw, elec, desolv, air; };
+
+// Device buffers of one batched (persistent) scoring launch.
+struct BatchLaunch {
+  const DevComplex* table;  // complex slots (device)
+  const int2* items;        // {slot, global pose}, longest first
+  const int32_t* n_items;   // device-resident item count
+  int32_t* counter;         // work counter and exit count: zero between launches
+  int32_t* exit_count;
+  const double* poses;      // pose records, kPoseDoubles each
+  double* out;              // kOutDoubles per pose
+  int32_t* nsat;
+  int32_t* redo;
+  int grid;                 // persistent workgroups
+  int capacity;             // item capacity (grid of the exact pass)
+  size_t lds;
+};
+
 
 }  // namespace gdock

@@ -1,0 +1,390 @@
+// GA driver of a docking job: the reference's generation loop (run.rs), its
+// operators (population.rs, ga.rs, chromosome.rs) and Hall of Fame, on the
+// host, with every population scored on the GPU.
+#include "ga.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+namespace gdock {
+
+namespace {
+constexpr double kTwoPi = 2.0 * kPI;
+constexpr double kHofRotation = 0.2;     // run.rs:139-158
+constexpr double kHofTranslation = 2.0;
+constexpr size_t kHofCapacity = 500;     // run.rs:84-99
+constexpr int kHofPerGeneration = 10;    // run.rs:194-216
+constexpr size_t kOutputModels = 5;      // run.rs:61
+
+double rem_euclid(double a, double b) {  // Rust f64::rem_euclid
+  const double r = std::fmod(a, b);
+  return r < 0.0 ? r + std::fabs(b) : r;
+}
+
+double angular_difference(double a, double b) {  // run.rs:160-169
+  const double d = std::fabs(a - b);
+  return d > kPI ? kTwoPi - d : d;
+}
+
+bool by_fitness(const Individual& a, const Individual& b) { return a.fitness < b.fitness; }
+}  // namespace
+
+// ---- random stream ---------------------------------------------------------
+Xoshiro256::Xoshiro256(uint64_t seed) {
+  uint64_t x = seed;
+  for (uint64_t& v : s_) {  // SplitMix64
+    x += 0x9e3779b97f4a7c15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    v = z ^ (z >> 31);
+  }
+}
+
+uint64_t Xoshiro256::next() {
+  auto rotl = [](uint64_t v, int k) { return (v << k) | (v >> (64 - k)); };
+  const uint64_t out = rotl(s_[1] * 5, 7) * 9;
+  const uint64_t t = s_[1] << 17;
+  s_[2] ^= s_[0];
+  s_[3] ^= s_[1];
+  s_[1] ^= s_[2];
+  s_[0] ^= s_[3];
+  s_[2] ^= t;
+  s_[3] = rotl(s_[3], 45);
+  return out;
+}
+
+double Xoshiro256::unit() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
+
+double Xoshiro256::range_closed(double lo, double hi) {
+  const double v = lo + (hi - lo) * (static_cast<double>(next() >> 11) / 9007199254740991.0);
+  return v > hi ? hi : v;
+}
+
+uint64_t Xoshiro256::below(uint64_t n) {  // Lemire's multiply-shift with rejection
+  unsigned __int128 m = static_cast<unsigned __int128>(next()) * n;
+  uint64_t low = static_cast<uint64_t>(m);
+  if (low < n) {
+    const uint64_t threshold = (0 - n) % n;
+    while (low < threshold) {
+      m = static_cast<unsigned __int128>(next()) * n;
+      low = static_cast<uint64_t>(m);
+    }
+  }
+  return static_cast<uint64_t>(m >> 64);
+}
+
+double Xoshiro256::normal(double mean, double sd) {  // Box-Muller, cosine branch
+  const double u1 = 1.0 - unit(), u2 = unit();
+  return mean + sd * (std::sqrt(-2.0 * std::log(u1)) * std::cos(kTwoPi * u2));
+}
+
+// ---- Hall of Fame -------------------------------------------------------------
+bool HallOfFame::offer(const double* g, double fitness, uint32_t generation) {
+  for (const HofEntry& e : entries_) {  // run.rs:129-158: similar to a kept pose?
+    bool similar = true;
+    for (int i = 0; i < 3 && similar; ++i) similar = angular_difference(g[i], e.genes[i]) <= kHofRotation;
+    for (int i = 3; i < 6 && similar; ++i) similar = std::fabs(g[i] - e.genes[i]) <= kHofTranslation;
+    if (similar) return false;
+  }
+  HofEntry e;
+  std::memcpy(e.genes, g, sizeof e.genes);
+  e.fitness = fitness;
+  e.generation = generation;
+  entries_.push_back(e);
+  if (entries_.size() > kHofCapacity) {  // run.rs:171-177: keep the best
+    std::stable_sort(entries_.begin(), entries_.end(),
+                     [](const HofEntry& a, const HofEntry& b) { return a.fitness < b.fitness; });
+    entries_.resize(kHofCapacity);
+  }
+  return true;
+}
+
+void HallOfFame::offer_population(const std::vector<Individual>& pop, uint32_t generation) {
+  std::vector<size_t> order(pop.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return pop[a].fitness < pop[b].fitness; });
+  int added = 0;
+  for (size_t i : order) {
+    if (added >= kHofPerGeneration) break;
+    added += offer(pop[i].genes, pop[i].fitness, generation) ? 1 : 0;
+  }
+}
+
+// ---- GA job ---------------------------------------------------------------------
+GaJob::GaJob(uint64_t seed, const gdock_ga_params& p) : p_(p), rng_(seed) {
+  pop_.resize(static_cast<size_t>(p.population_size));
+  for (Individual& c : pop_) {  // chromosome.rs:27-54
+    for (int i = 0; i < 3; ++i) c.genes[i] = rng_.range_closed(0.0, kTwoPi);
+    for (int i = 3; i < 6; ++i) c.genes[i] = rng_.range_closed(-p.max_displacement, p.max_displacement);
+  }
+  refresh_genes();
+}
+
+void GaJob::refresh_genes() {
+  genes_.resize(pop_.size() * 6);
+  for (size_t i = 0; i < pop_.size(); ++i) std::memcpy(&genes_[6 * i], pop_[i].genes, 6 * sizeof(double));
+}
+
+void GaJob::absorb(const double* out) {
+  if (done_) return;
+  for (size_t i = 0; i < pop_.size(); ++i) {
+    Individual& c = pop_[i];
+    c.vdw = out[6 * i];
+    c.elec = out[6 * i + 1];
+    c.desolv = out[6 * i + 2];
+    c.air = out[6 * i + 3];
+    c.penalty = out[6 * i + 4];
+    c.fitness = out[6 * i + 5];
+  }
+  if (p_.keep_trace) {
+    trace_genes.insert(trace_genes.end(), genes_.begin(), genes_.end());
+    for (const Individual& c : pop_) trace_fitness.push_back(c.fitness);
+  }
+  hof_.offer_population(pop_, static_cast<uint32_t>(generation_));
+  size_t best = 0;  // the first individual of least fitness
+  for (size_t i = 1; i < pop_.size(); ++i)
+    if (pop_[i].fitness < pop_[best].fitness) best = i;
+  if (p_.keep_trace) trace_best.push_back(static_cast<int32_t>(best));
+  const double f = pop_.empty() ? 0.0 : pop_[best].fitness;
+  if (generation_ > 0) {  // run.rs:480-531: relative improvement of the best
+    const double improvement = (last_best_ - f) / std::fabs(last_best_);
+    stall_ = improvement < p_.convergence_threshold ? stall_ + 1 : 0;
+  }
+  last_best_ = f;
+  generations_ = generation_ + 1;
+  if (p_.early_stopping && stall_ >= p_.convergence_window) {
+    converged_at_ = generation_;
+    done_ = true;
+    return;
+  }
+  ++generation_;
+  if (generation_ >= p_.max_generations) {
+    done_ = true;
+    return;
+  }
+  breed();
+  refresh_genes();
+}
+
+// population.rs:74-158 with ga.rs:7-39 and chromosome.rs:56-77.
+void GaJob::breed() {
+  const size_t n = pop_.size();
+  std::vector<Individual> elite = pop_;
+  std::stable_sort(elite.begin(), elite.end(), by_fitness);
+  elite.resize(std::min<size_t>(static_cast<size_t>(p_.elitism), elite.size()));
+
+  std::vector<Individual> chosen;  // tournament selection, first minimum wins
+  chosen.reserve(n);
+  for (size_t k = 0; k < n; ++k) {
+    const Individual* best = nullptr;
+    for (int t = 0; t < p_.tournament_size; ++t) {
+      const Individual* c = &pop_[rng_.below(n)];
+      if (!best || c->fitness < best->fitness) best = c;
+    }
+    chosen.push_back(*best);
+  }
+  for (size_t i = chosen.size(); i > 1; --i) std::swap(chosen[i - 1], chosen[rng_.below(i)]);  // Fisher-Yates
+
+  std::vector<Individual> next;
+  next.reserve(n);
+  for (size_t i = 0; i + 1 < chosen.size(); i += 2) {  // uniform crossover of neighbours
+    Individual a = chosen[i], b = chosen[i + 1];
+    for (int g = 0; g < 6; ++g)
+      if (rng_.unit() <= p_.crossover_rate) std::swap(a.genes[g], b.genes[g]);
+    next.push_back(a);
+    next.push_back(b);
+  }
+  for (Individual& c : next)
+    for (int g = 0; g < 6; ++g) {
+      if (!(rng_.unit() <= p_.mutation_rate)) continue;
+      if (g < 3)
+        c.genes[g] = rem_euclid(c.genes[g] + rng_.normal(0.0, 0.174), kTwoPi);
+      else
+        c.genes[g] = std::min(std::max(c.genes[g] + rng_.normal(0.0, 1.0), -p_.max_displacement),
+                              p_.max_displacement);
+    }
+  // the elite replace the children that inherited the worst fitness
+  std::stable_sort(next.begin(), next.end(), [](const Individual& a, const Individual& b) { return b.fitness < a.fitness; });
+  for (size_t i = 0; i < elite.size() && i < next.size(); ++i) next[i] = elite[i];
+  pop_ = std::move(next);
+}
+
+void GaJob::select_models(const std::vector<gdock_cluster_result>& clusters) {
+  selected_.clear();
+  const auto& h = hof_.entries();
+  struct Centre { int64_t idx; double fitness; int64_t size; };
+  std::vector<Centre> centres;
+  for (const gdock_cluster_result& c : clusters) centres.push_back({c.center_idx, h[c.center_idx].fitness, c.size});
+  std::stable_sort(centres.begin(), centres.end(), [](const Centre& a, const Centre& b) { return a.fitness < b.fitness; });
+  for (size_t i = 0; i < centres.size() && selected_.size() < kOutputModels; ++i)
+    selected_.push_back({centres[i].idx, centres[i].size});
+  if (selected_.size() < kOutputModels) {
+    std::vector<int64_t> order(h.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return h[a].fitness < h[b].fitness; });
+    for (int64_t idx : order) {
+      if (selected_.size() >= kOutputModels) break;
+      bool taken = false;
+      for (const GaSelected& s : selected_) taken = taken || s.hof_index == idx;
+      if (!taken) selected_.push_back({idx, 1});
+    }
+  }
+}
+
+}  // namespace gdock
+
+using namespace gdock;
+
+struct gdock_ga_result {
+  GaJob job;
+  std::vector<gdock_cluster_result> clusters;
+  explicit gdock_ga_result(uint64_t seed, const gdock_ga_params& p) : job(seed, p) {}
+};
+
+namespace gdock {
+
+// Clusters the Hall of Fame poses on the device (run.rs:665-693) and selects
+// the output models.
+gdock_status finish_job(gdock_ctx* ctx, gdock_complex* cx, GaJob* job, std::vector<gdock_cluster_result>* clusters) {
+  const auto& h = job->hof().entries();
+  const int32_t n = static_cast<int32_t>(h.size());
+  std::vector<double> genes(static_cast<size_t>(n) * 6);
+  for (int32_t i = 0; i < n; ++i) std::memcpy(&genes[6 * i], h[i].genes, 6 * sizeof(double));
+  const gdock_clustering_config cfg{5.0, 0.60, 0.75, 4};  // ClusteringConfig::default()
+  std::vector<gdock_cluster_result> res(std::max(n, 1));
+  std::vector<int64_t> members(std::max(n, 1));
+  int32_t nc = 0;
+  if (n > 0) {
+    const gdock_status st = gdock_cluster_structures(ctx, cx, genes.data(), n, &cfg, res.data(), members.data(), &nc);
+    if (st != GDOCK_OK) return st;
+  }
+  clusters->assign(res.begin(), res.begin() + nc);
+  job->select_models(*clusters);
+  return GDOCK_OK;
+}
+
+}  // namespace gdock
+
+extern "C" {
+
+gdock_status gdock_ga_default_params(gdock_ga_params* p) {
+  if (!p) return GDOCK_ERR_INVALID_ARG;
+  *p = gdock_ga_params{};
+  p->population_size = 150;      // constants.rs:2-16
+  p->max_generations = 250;
+  p->elitism = 5;
+  p->tournament_size = 3;
+  p->convergence_window = 10;
+  p->early_stopping = 1;
+  p->cluster = 1;
+  p->keep_trace = 0;
+  p->mutation_rate = 0.1;
+  p->crossover_rate = 0.6;
+  p->convergence_threshold = 0.001;
+  p->max_displacement = 50.0;
+  return GDOCK_OK;
+}
+
+gdock_status gdock_ga_validate(gdock_ctx* ctx, const gdock_ga_params* p) {
+  if (!p || p->population_size < 2 || p->population_size > (1 << 20) || p->max_generations < 1 || p->elitism < 0 ||
+      p->tournament_size < 1 || p->convergence_window < 1 || !(p->max_displacement >= 0.0) ||
+      !std::isfinite(p->max_displacement) || !std::isfinite(p->mutation_rate) || !std::isfinite(p->crossover_rate)) {
+    if (ctx) set_error(ctx, "ga: invalid parameters");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  return GDOCK_OK;
+}
+
+gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
+                          const gdock_ga_params* params, gdock_ga_result** out) {
+  if (!ctx || !cx || !w || !out) {
+    if (ctx) set_error(ctx, "ga_run: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  gdock_ga_params p;
+  if (params) p = *params; else gdock_ga_default_params(&p);
+  gdock_status st = gdock_ga_validate(ctx, &p);
+  if (st != GDOCK_OK) return st;
+  auto* r = new gdock_ga_result(seed, p);
+  GaJob& job = r->job;
+  std::vector<double> res(static_cast<size_t>(job.population()) * 6);
+  while (!job.done()) {  // run.rs:361-531, one plain scoring launch per generation
+    st = gdock_score_poses(ctx, cx, job.genes().data(), job.population(), w, res.data(), nullptr);
+    if (st != GDOCK_OK) { delete r; return st; }
+    job.absorb(res.data());
+  }
+  if (p.cluster) {
+    st = finish_job(ctx, cx, &job, &r->clusters);
+    if (st != GDOCK_OK) { delete r; return st; }
+  }
+  *out = r;
+  return GDOCK_OK;
+}
+
+void gdock_ga_free(gdock_ga_result* r) { delete r; }
+
+gdock_status gdock_ga_info(const gdock_ga_result* r, int32_t* generations, int32_t* converged_at,
+                           int32_t* hof_size, int32_t* n_clusters, int32_t* n_selected) {
+  if (!r) return GDOCK_ERR_INVALID_ARG;
+  if (generations) *generations = r->job.generations();
+  if (converged_at) *converged_at = r->job.converged_at();
+  if (hof_size) *hof_size = static_cast<int32_t>(r->job.hof().entries().size());
+  if (n_clusters) *n_clusters = static_cast<int32_t>(r->clusters.size());
+  if (n_selected) *n_selected = static_cast<int32_t>(r->job.selected().size());
+  return GDOCK_OK;
+}
+
+gdock_status gdock_ga_trace(const gdock_ga_result* r, double* genes, double* fitness, int32_t* best_idx) {
+  if (!r) return GDOCK_ERR_INVALID_ARG;
+  const GaJob& j = r->job;
+  if (genes) std::copy(j.trace_genes.begin(), j.trace_genes.end(), genes);
+  if (fitness) std::copy(j.trace_fitness.begin(), j.trace_fitness.end(), fitness);
+  if (best_idx) std::copy(j.trace_best.begin(), j.trace_best.end(), best_idx);
+  return GDOCK_OK;
+}
+
+int32_t gdock_ga_hof(const gdock_ga_result* r, double* genes, double* fitness, int32_t* generation) {
+  if (!r) return -1;
+  const auto& h = r->job.hof().entries();
+  for (size_t i = 0; i < h.size(); ++i) {
+    if (genes) std::memcpy(genes + 6 * i, h[i].genes, 6 * sizeof(double));
+    if (fitness) fitness[i] = h[i].fitness;
+    if (generation) generation[i] = static_cast<int32_t>(h[i].generation);
+  }
+  return static_cast<int32_t>(h.size());
+}
+
+int32_t gdock_ga_clusters(const gdock_ga_result* r, int64_t* center, int64_t* size) {
+  if (!r) return -1;
+  for (size_t i = 0; i < r->clusters.size(); ++i) {
+    if (center) center[i] = r->clusters[i].center_idx;
+    if (size) size[i] = r->clusters[i].size;
+  }
+  return static_cast<int32_t>(r->clusters.size());
+}
+
+int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t* cluster_size) {
+  if (!r) return -1;
+  const auto& s = r->job.selected();
+  for (size_t i = 0; i < s.size(); ++i) {
+    if (hof_index) hof_index[i] = s[i].hof_index;
+    if (cluster_size) cluster_size[i] = s[i].cluster_size;
+  }
+  return static_cast<int32_t>(s.size());
+}
+
+void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out) {
+  Xoshiro256 r(seed);
+  for (int32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+}  // extern "C"

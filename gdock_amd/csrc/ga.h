@@ -1,0 +1,90 @@
+// Genetic-algorithm driver (host side) of a docking job.  Population scoring
+// runs on the GPU; selection, crossover, mutation, the Hall of Fame and model
+// selection are cheap sequential host work.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+
+namespace gdock {
+
+// Project-defined random stream: state seeded by SplitMix64, xoshiro256**
+// generator, Lemire's unbiased bounded integers, Box-Muller normals (one per
+// call).  The reference draws from rand's StdRng; the algorithms consuming
+// the stream follow the reference step by step.
+class Xoshiro256 {
+ public:
+  explicit Xoshiro256(uint64_t seed);
+  uint64_t next();
+  double unit();                              // [0, 1), 53 bits
+  double range_closed(double lo, double hi);  // [lo, hi]
+  uint64_t below(uint64_t n);                 // [0, n)
+  double normal(double mean, double sd);
+
+ private:
+  uint64_t s_[4];
+};
+
+struct Individual {
+  double genes[6];
+  double fitness = 0.0, vdw = 0.0, elec = 0.0, desolv = 0.0, air = 0.0, penalty = 0.0;
+};
+
+struct HofEntry {
+  double genes[6];
+  double fitness;
+  uint32_t generation;
+};
+
+// run.rs:77-216: unique poses (angles within 0.2 rad, shifts within 2 A count
+// as the same pose), the 10 best of each generation offered, at most 500 kept.
+class HallOfFame {
+ public:
+  bool offer(const double* genes, double fitness, uint32_t generation);
+  void offer_population(const std::vector<Individual>& pop, uint32_t generation);
+  const std::vector<HofEntry>& entries() const { return entries_; }
+
+ private:
+  std::vector<HofEntry> entries_;
+};
+
+struct GaSelected { int64_t hof_index, cluster_size; };
+
+// One docking job's GA, advanced one generation at a time: genes() are the
+// poses to score; absorb() takes their six outputs per pose (vdw, elec,
+// desolv, air, restraint_penalty, fitness), updates the Hall of Fame and the
+// convergence test, and breeds the next generation unless the job is done.
+class GaJob {
+ public:
+  GaJob(uint64_t seed, const gdock_ga_params& p);
+  const std::vector<double>& genes() const { return genes_; }
+  int population() const { return static_cast<int>(pop_.size()); }
+  void absorb(const double* out);
+  bool done() const { return done_; }
+  int generations() const { return generations_; }
+  int converged_at() const { return converged_at_; }
+  const HallOfFame& hof() const { return hof_; }
+  // run.rs:694-733: cluster centres sorted by fitness, then the best-scored
+  // remaining entries, up to five models.
+  void select_models(const std::vector<gdock_cluster_result>& clusters);
+  const std::vector<GaSelected>& selected() const { return selected_; }
+  // per evaluated generation: genes (pop x 6), fitness (pop), best index
+  std::vector<double> trace_genes, trace_fitness;
+  std::vector<int32_t> trace_best;
+
+ private:
+  void breed();
+  void refresh_genes();
+  gdock_ga_params p_;
+  Xoshiro256 rng_;
+  std::vector<Individual> pop_;
+  std::vector<double> genes_;
+  HallOfFame hof_;
+  std::vector<GaSelected> selected_;
+  int generation_ = 0, generations_ = 0, converged_at_ = -1, stall_ = 0;
+  double last_best_ = 1.7976931348623157e308;
+  bool done_ = false;
+};
+
+}  // namespace gdock

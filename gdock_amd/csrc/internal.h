@@ -71,9 +71,15 @@ void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::v
 
 // Scores n_poses poses: the fast kernel, then the exact kernel for any pose
 // whose settle buffer overflowed.  d_redo: n_poses int32 of device scratch.
+// lanes_k: ligand atoms per lane (1-8); 0 picks ceil(n_lig / 256) capped at 8.
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         const double* d_extent, int n_poses, const Weights& w, double* d_out,
-                        int32_t* d_nsat, int32_t* d_redo, hipStream_t stream);
+                        int32_t* d_nsat, int32_t* d_redo, hipStream_t stream, int lanes_k = 0);
+int score_atoms_per_lane(int n_lig);
+
+size_t batch_lds_bytes(int max_n_rec);
+int batch_grid(size_t lds, int device);
+hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream);
 
 // Host-side bump layout of one device allocation (256-byte aligned pieces).
 struct Layout {

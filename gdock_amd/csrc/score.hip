@@ -143,11 +143,10 @@ __device__ __forceinline__ void class_factors(CxRef cx, int c, int l, float& ir6
   eps = static_cast<float>(sqrt(cx.class_eps[c] * cx.lig_eps[l]));
 }
 
-// The reference's decisions for one pair, in FP64 on the pose's coordinates.
-__device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords, int l,
-                                            int j, int rser, bool& x64, bool& x144, bool& x225) {
-  double px, py, pz;
-  ligand_atom(cx, pc, coords, l, px, py, pz);
+// The reference's decisions for one pair, in FP64 on the pose's coordinates
+// (px, py, pz: ligand atom l after the pose transform).
+__device__ __forceinline__ void exact_flags_at(CxRef cx, double px, double py, double pz, int l, int j, int rser,
+                                               bool& x64, bool& x144, bool& x225) {
   const double ex = cx.rec_x[j] - px, ey = cx.rec_y[j] - py, ez = cx.rec_z[j] - pz;
   const double e2 = ex * ex + ey * ey + ez * ez;
   x64 = e2 <= kD2DesolvIncl;
@@ -155,19 +154,20 @@ __device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const 
   x225 = e2 > kD2ElecMinExcl && e2 <= kD2ElecMaxIncl;
 }
 
+__device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords, int l,
+                                            int j, int rser, bool& x64, bool& x144, bool& x225) {
+  double px, py, pz;
+  ligand_atom(cx, pc, coords, l, px, py, pz);
+  exact_flags_at(cx, px, py, pz, l, j, rser, x64, x144, x225);
+}
+
 struct SettleEvent { int32_t j, pad; uint64_t mask; };  // first receptor slot of a group, lanes
 constexpr int kEventsPerWave = 128;
 
 // Fast path of one receptor atom against the lane's K ligand atoms; returns
 // this wave's burial count of the receptor atom and flags band pairs.
-template <int K, bool COLL>
-__device__ __forceinline__ int32_t receptor_atom(CxRef cx, int base, int j, const RecAtom r, const Bands& B,
-                                                 LaneAtoms<K>& a, int32_t& ambv) {
-  int rser = INT_MIN;  // INT_MIN: no ligand atom shares this receptor atom's serial
-  if (COLL) {
-    const int2 m = cx.rec_meta[j];
-    rser = m.y ? m.x : INT_MIN;
-  }
+template <int K>
+__device__ __forceinline__ int32_t receptor_atom(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int32_t& ambv) {
   int32_t cj = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -176,8 +176,7 @@ __device__ __forceinline__ int32_t receptor_atom(CxRef cx, int base, int j, cons
     const bool amb = (far1 != !(d2 < B.hi1)) | (md != (d2 < B.hi64)) | (mv0 != (d2 < B.hi144)) |
                      (me0 != (d2 < B.hi225));
     ambv = amb ? 1 : ambv;
-    bool mv = mv0;
-    if (COLL) mv = mv && cx.lig_serial[base + threadIdx.x + kScoreThreads * k] != rser;  // fitness.rs:92
+    const bool mv = mv0;  // pairs of equal serial are taken out afterwards (collision_correction)
     const bool me = me0 && far1;
     float tv, id2;
     pair_terms(d2, a.ir6[k], tv, id2);
@@ -195,17 +194,14 @@ __device__ __forceinline__ int32_t receptor_atom(CxRef cx, int base, int j, cons
 
 // Band pairs of one receptor atom: recorded as events (fast kernel) or
 // settled on the spot in FP64 (exact kernel).
-template <int K, bool COLL, bool EXACT>
+template <int K, bool EXACT>
 __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords,
                                            int base, int j, const RecAtom r, const Bands& B,
                                            LaneAtoms<K>& a, SettleEvent* __restrict__ ev, int& nev, int& overflow,
                                            int32_t* __restrict__ s_rcnt) {
   const int lane = threadIdx.x & 63;
-  int rser = INT_MIN;
-  if (COLL) {
-    const int2 m = cx.rec_meta[j];
-    rser = m.y ? m.x : INT_MIN;
-  }
+  const int2 m = cx.rec_meta[j];
+  const int rser = m.y ? m.x : INT_MIN;  // INT_MIN: no ligand atom shares this receptor atom's serial
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
@@ -224,8 +220,7 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
     if ((am >> lane) & 1ull) {
       bool x64, x144, x225;
       exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
-      bool f144 = d2 <= B.lo144;
-      if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
+      const bool f144 = d2 <= B.lo144;
       const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
       float tv, id2;
       pair_terms(d2, a.ir6[k], tv, id2);
@@ -243,8 +238,13 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
 // Settles the wave's recorded events.  An event names a group of kRecGroup
 // receptor atoms and the lanes that saw a band pair in it; each band pair is
 // decided in FP64 and the difference to the FP32 decisions is added to the
-// FP64 sums, the lane's burial count and the receptor atom's count.
-template <int K, bool COLL>
+// FP64 sums, the lane's burial count and the receptor atom's count.  The
+// ligand slot is a loop variable here, so its FP32 coordinates are rebuilt
+// from the FP64 transform (the same operations that filled the lane's
+// registers) and its burial-count changes are carried as 16-bit fields of
+// two packed integers: the lane's register arrays are only ever indexed by
+// constants and stay in registers.
+template <int K>
 __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict__ pose_rec,
                                               const double* __restrict__ coords, int base, const Bands& B,
                                               const SettleEvent* __restrict__ ev, int nev, LaneAtoms<K>& a,
@@ -252,35 +252,38 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
   const int lane = threadIdx.x & 63;
   const int32_t GDOCK_AS4* cstart = (const int32_t GDOCK_AS4*)cx.class_start;
   const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
+  PoseCoef pc{};
+  if (!coords) pc = load_pose(pose_rec);
+  int64_t dpack0 = 0, dpack1 = 0;  // slot k's burial-count change in bits [16 (k % 4), +16)
   for (int e = 0; e < nev; ++e) {
     const SettleEvent E = ev[e];
     if (!((E.mask >> lane) & 1ull)) continue;
     int c = 0;
     while (cstart[c + 1] <= E.j) ++c;  // groups never straddle classes
-    for (int j = E.j; j < E.j + kRecGroup; ++j) {
-      const RecAtom r = rec_atom(rq, j);
-      int rser = INT_MIN;
-      if (COLL) {
-        const int2 m = cx.rec_meta[j];
-        rser = m.y ? m.x : INT_MIN;
-      }
 #pragma unroll 1
-      for (int k = 0; k < K; ++k) {  // one copy of the FP64 path: pick slot k by value
-        float lx = a.x[0], ly = a.y[0], lz = a.z[0];
-#pragma unroll
-        for (int kk = 1; kk < K; ++kk)
-          if (kk == k) { lx = a.x[kk]; ly = a.y[kk]; lz = a.z[kk]; }
+    for (int k = 0; k < K; ++k) {  // one copy of the FP64 path
+      const int l = base + threadIdx.x + kScoreThreads * k;
+      if (l >= cx.n_lig) break;
+      double px, py, pz;
+      ligand_atom(cx, pc, coords, l, px, py, pz);
+      const float lx = static_cast<float>(px - cx.origin[0]);
+      const float ly = static_cast<float>(py - cx.origin[1]);
+      const float lz = static_cast<float>(pz - cx.origin[2]);
+      bool have_factors = false;
+      float ir6 = 0.0f, eps = 0.0f;
+      for (int j = E.j; j < E.j + kRecGroup; ++j) {
+        const RecAtom r = rec_atom(rq, j);
         const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
-        const int l = base + threadIdx.x + kScoreThreads * k;
-        if (!in_band(d2, B) || l >= cx.n_lig) continue;
-        PoseCoef pc{};
-        if (!coords) pc = load_pose(pose_rec);
-        float ir6, eps;
-        class_factors(cx, c, l, ir6, eps);
+        if (!in_band(d2, B)) continue;
+        const int2 m = cx.rec_meta[j];
+        const int rser = m.y ? m.x : INT_MIN;
+        if (!have_factors) {
+          class_factors(cx, c, l, ir6, eps);
+          have_factors = true;
+        }
         bool x64, x144, x225;
-        exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
-        bool f144 = d2 <= B.lo144;
-        if (COLL) f144 = f144 && cx.lig_serial[l] != rser;
+        exact_flags_at(cx, px, py, pz, l, j, rser, x64, x144, x225);
+        const bool f144 = d2 <= B.lo144;
         const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
         float tv, id2;
         pair_terms(d2, ir6, tv, id2);
@@ -289,12 +292,19 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
           sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
         const int dc = static_cast<int>(x64) - static_cast<int>(f64);
         if (dc) {
-#pragma unroll
-          for (int kk = 0; kk < K; ++kk) a.cnt[kk] += kk == k ? dc : 0;
+          const int64_t d = static_cast<int64_t>(dc) << (16 * (k & 3));
+          if (k < 4) dpack0 += d; else dpack1 += d;
           atomicAdd(&s_rcnt[j], dc);
         }
       }
     }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {  // unpack: fields are signed, so peel them from the bottom
+    int64_t& p = k < 4 ? dpack0 : dpack1;
+    const int32_t v = static_cast<int16_t>(p & 0xffff);
+    p = (p - v) >> 16;
+    a.cnt[k] += v;
   }
 }
 
@@ -303,7 +313,7 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
 // scoring coordinates as given, with `extent` = their largest |x - origin|.
 // o: 6 outputs; *redo is set when the fast variant could not settle every
 // band pair (the exact variant then rescores the pose).
-template <int K, bool COLL, bool EXACT>
+template <int K, bool EXACT>
 __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ pose_rec,
                                            const double* __restrict__ coords, double extent, const Weights& w,
                                            double* __restrict__ o, int32_t* __restrict__ nsat_out,
@@ -369,21 +379,21 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
         int32_t ambv = 0;
         const int t = lane - (j - g0);
-        const int32_t c0 = receptor_atom<K, COLL>(cx, base, j, r0, B, a, ambv);
+        const int32_t c0 = receptor_atom<K>(r0, B, a, ambv);
         rc = t == 0 ? c0 : rc;
-        const int32_t c1 = receptor_atom<K, COLL>(cx, base, j + 1, r1, B, a, ambv);
+        const int32_t c1 = receptor_atom<K>(r1, B, a, ambv);
         rc = t == 1 ? c1 : rc;
-        const int32_t c2 = receptor_atom<K, COLL>(cx, base, j + 2, r2, B, a, ambv);
+        const int32_t c2 = receptor_atom<K>(r2, B, a, ambv);
         rc = t == 2 ? c2 : rc;
-        const int32_t c3 = receptor_atom<K, COLL>(cx, base, j + 3, r3, B, a, ambv);
+        const int32_t c3 = receptor_atom<K>(r3, B, a, ambv);
         rc = t == 3 ? c3 : rc;
         const uint64_t amask = __builtin_amdgcn_ballot_w64(ambv != 0);
         if (amask != 0ull) {  // rare: this group holds band pairs
           if (EXACT) {
-            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, COLL, EXACT>(cx, pc, coords, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
           } else if (nev < kEventsPerWave) {  // settled once per ligand chunk
             if (lane == 0) my_ev[nev] = SettleEvent{j, 0, amask};
             ++nev;
@@ -404,7 +414,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       }
     }
     if (!EXACT && nev > 0)
-      settle_events<K, COLL>(cx, pose_rec, coords, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
+      settle_events<K>(cx, pose_rec, coords, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int l = base + tid + kScoreThreads * k;
@@ -424,6 +434,26 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     if (tid == 0) *redo = 1;
     __syncthreads();
     return;
+  }
+  // fitness.rs:92: a receptor and a ligand atom with the same serial number
+  // add no vdw term.  The pair loop includes them; here each such pair that
+  // was not re-decided as a band pair is taken out again with the very FP32
+  // value the pair loop added (same coordinates, factors and operations).
+  for (int i = tid; i < cx.n_coll; i += kScoreThreads) {
+    const int4 P = cx.coll_pairs[i];  // receptor slot, ligand atom, receptor class
+    double px, py, pz;
+    ligand_atom(cx, pc, coords, P.y, px, py, pz);
+    const float lx = static_cast<float>(px - cx.origin[0]);
+    const float ly = static_cast<float>(py - cx.origin[1]);
+    const float lz = static_cast<float>(pz - cx.origin[2]);
+    const float4 r = cx.rec_xyzq[P.x];
+    const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
+    if (d2 <= B.lo144 && !in_band(d2, B)) {
+      float ir6, eps, tv, id2;
+      class_factors(cx, P.z, P.y, ir6, eps);
+      pair_terms(d2, ir6, tv, id2);
+      sum_v -= static_cast<double>(tv) * static_cast<double>(eps);
+    }
   }
   for (int j = tid; j < cx.n_rec; j += kScoreThreads) {  // fitness.rs:141-162
     const double asp = cx.rec_asp[j];
@@ -479,7 +509,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
   __syncthreads();  // LDS is reused by the workgroup's next pose
 }
 
-template <int K, bool COLL, bool EXACT>
+template <int K, bool EXACT>
 __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
     const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
     const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
@@ -489,45 +519,10 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
   extern __shared__ int32_t s_lds[];
   __shared__ int32_t s_overflow;
   CxRef cx = *(const DevComplex GDOCK_AS4*)d_cx;
-  score_pose<K, COLL, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
+  score_pose<K, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
                              coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr,
                              coords ? pose_extent[pose] : 0.0, w, out + static_cast<size_t>(kOutDoubles) * pose,
                              nsat_out ? nsat_out + pose : nullptr, redo + pose, s_lds, &s_overflow);
-}
-
-// Persistent form: each workgroup takes (complex, pose) items from a shared
-// counter until the list is exhausted.  Items come longest first.
-template <int K, bool COLL, bool EXACT>
-__global__ __launch_bounds__(kScoreThreads) void score_items_kernel(
-    const DevComplex* __restrict__ complexes, const int2* __restrict__ items, int n_items,
-    int32_t* __restrict__ counter, const double* __restrict__ poses, const Weights w, double* __restrict__ out,
-    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
-  extern __shared__ int32_t s_lds[];
-  __shared__ int32_t s_overflow, s_item;
-  for (;;) {
-    if (EXACT) {  // one item per workgroup: only items whose settle buffer overflowed
-      if (blockIdx.x >= static_cast<unsigned>(n_items)) return;
-    } else {
-      if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-      __syncthreads();
-    }
-    const int it = EXACT ? static_cast<int>(blockIdx.x) : s_item;
-    __syncthreads();
-    if (it >= n_items) return;
-    const int2 I = items[it];  // {complex, global pose index}
-    if (!EXACT || redo[I.y] != 0) {
-      CxRef cx = ((const DevComplex GDOCK_AS4*)complexes)[I.x];
-      score_pose<K, COLL, EXACT>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, nullptr, 0.0, w,
-                                 out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds,
-                                 &s_overflow);
-    }
-    if (EXACT) return;
-  }
-}
-
-size_t score_lds_bytes(const DevComplex& cx) {
-  return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
-         (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent);
 }
 
 int score_atoms_per_lane(int n_lig) {
@@ -535,13 +530,87 @@ int score_atoms_per_lane(int n_lig) {
   return k < 1 ? 1 : (k > kMaxAtomsPerLane ? kMaxAtomsPerLane : k);
 }
 
-template <int K, bool COLL>
+// Batched form over many complexes.  An item is {complex slot, global pose
+// index}; each complex carries its own ligand atoms per lane (lanes_k), so the
+// workgroup picks the matching instantiation of score_pose per item.
+template <bool EXACT>
+__device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ pose_rec, const Weights& w,
+                                           double* __restrict__ o, int32_t* __restrict__ nsat,
+                                           int32_t* __restrict__ redo, int32_t* __restrict__ s_lds,
+                                           int32_t* s_overflow) {
+  switch (cx.lanes_k) {
+#define GDOCK_ITEM(KK)                                                                                   \
+  case KK:                                                                                               \
+    score_pose<KK, EXACT>(cx, pose_rec, nullptr, 0.0, w, o, nsat, redo, s_lds, s_overflow);             \
+    break;
+    GDOCK_ITEM(1) GDOCK_ITEM(2) GDOCK_ITEM(3) GDOCK_ITEM(4)
+    GDOCK_ITEM(5) GDOCK_ITEM(6) GDOCK_ITEM(7) GDOCK_ITEM(8)
+#undef GDOCK_ITEM
+    default: break;  // lanes_k is validated on the host
+  }
+}
+
+// Persistent scoring of a batch: every workgroup takes items from a shared
+// counter until the list (longest first) is exhausted.  The item count is
+// read from device memory so that one captured launch serves any number of
+// items up to the capacity.  The last workgroup to leave resets the counter
+// and the exit count, so the next launch starts from zero with no memset.
+__global__ __launch_bounds__(kScoreThreads) void score_batch_kernel(
+    const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
+    int32_t* __restrict__ counter, int32_t* __restrict__ exit_count, const double* __restrict__ poses,
+    const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
+  extern __shared__ int32_t s_lds[];
+  __shared__ int32_t s_overflow, s_item;
+  const int n_items = *n_items_p;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= n_items) break;
+    const int2 I = items[it];
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x];
+    score_item<false>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
+                      out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, &s_overflow);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(exit_count, 1) == static_cast<int>(gridDim.x) - 1) {
+      atomicExch(counter, 0);
+      atomicExch(exit_count, 0);
+    }
+  }
+}
+
+// Exact rescoring of the batch items whose settle buffer overflowed (one
+// workgroup per item slot; all others return at once).
+__global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
+    const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
+    const double* __restrict__ poses, const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out,
+    int32_t* __restrict__ redo) {
+  extern __shared__ int32_t s_lds[];
+  __shared__ int32_t s_overflow;
+  const int it = blockIdx.x;
+  if (it >= *n_items_p) return;
+  const int2 I = items[it];
+  if (redo[I.y] == 0) return;
+  CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x];
+  score_item<true>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
+                   out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, &s_overflow);
+}
+
+size_t score_lds_bytes(const DevComplex& cx) {
+  return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
+         (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent);
+}
+
+template <int K>
 static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, const double* d_extent,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, size_t lds,
                         hipStream_t stream) {
-  hipLaunchKernelGGL((score_poses_kernel<K, COLL, false>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
+  hipLaunchKernelGGL((score_poses_kernel<K, false>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
                      d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
-  hipLaunchKernelGGL((score_poses_kernel<K, COLL, true>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
+  hipLaunchKernelGGL((score_poses_kernel<K, true>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
                      d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
 }
 
@@ -549,21 +618,42 @@ static void launch_pair(const DevComplex* d_cx, const double* d_poses, const dou
 // that did not overflow its settle buffer).  d_redo: n_poses int32 scratch.
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         const double* d_extent, int n_poses, const Weights& w, double* d_out, int32_t* d_nsat,
-                        int32_t* d_redo, hipStream_t stream) {
+                        int32_t* d_redo, hipStream_t stream, int lanes_k) {
   if (n_poses <= 0) return hipSuccess;
   const size_t lds = score_lds_bytes(cx);
-  const bool coll = cx.has_serial_collision != 0;
-  switch (score_atoms_per_lane(cx.n_lig)) {
+  switch (lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig)) {
 #define GDOCK_CASE(KK)                                                                                            \
   case KK:                                                                                                        \
-    if (coll) launch_pair<KK, true>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream); \
-    else launch_pair<KK, false>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);     \
+    launch_pair<KK>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);         \
     break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)
 #undef GDOCK_CASE
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+size_t batch_lds_bytes(int max_n_rec) {
+  DevComplex cx{};
+  cx.n_rec = max_n_rec;
+  return score_lds_bytes(cx);
+}
+
+int batch_grid(size_t lds, int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, score_batch_kernel, kScoreThreads, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+  return per_cu * cus;
+}
+
+hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream) {
+  hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
+                     b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo);
+  hipLaunchKernelGGL(score_batch_exact_kernel, dim3(b.capacity), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
+                     b.n_items, b.poses, w, b.out, b.nsat, b.redo);
   return hipGetLastError();
 }
 

@@ -158,3 +158,60 @@ class Complex:
             m = members[r.members_offset:r.members_offset + r.size].tolist()
             out.append(ClusterResult(r.center_idx, m, r.size))
         return out
+
+
+class Batch:
+    """Several complexes scored together: one persistent launch per generation,
+    replayed from a captured HIP graph (gdock_batch_*)."""
+
+    def __init__(self, complexes: list[Complex], poses_per_complex, ctx: Context | None = None):
+        self.ctx = ctx or (complexes[0].ctx if complexes else default_context())
+        self.complexes = list(complexes)  # kept alive: the batch points at their device memory
+        counts = np.ascontiguousarray(np.array(list(poses_per_complex), dtype=np.int32))
+        assert len(counts) == len(self.complexes)
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        arr = (C.c_void_p * max(len(self.complexes), 1))(*[c.handle.value for c in self.complexes])
+        h = C.c_void_p()
+        check(lib().gdock_batch_create(self.ctx.handle, arr, len(self.complexes), ptr(counts, C.c_int32),
+                                       C.byref(h)), self.ctx.handle, "gdock_batch_create")
+        self.handle = h
+        self.n_poses = int(lib().gdock_batch_n_poses(h))
+
+    def plan(self) -> dict:
+        k = np.zeros(max(len(self.complexes), 1), np.int32)
+        grid, lds = C.c_int32(), C.c_int32()
+        check(lib().gdock_batch_plan(self.handle, ptr(k, C.c_int32), C.byref(grid), C.byref(lds)),
+              what="gdock_batch_plan")
+        return {"lanes_k": k[:len(self.complexes)].tolist(), "grid": grid.value, "lds_bytes": lds.value}
+
+    def score(self, genes: np.ndarray, weights: constants.EnergyWeights | None = None):
+        g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
+        assert len(g) == self.n_poses, (len(g), self.n_poses)
+        out = np.zeros((len(g), 6))
+        ns = np.zeros(len(g), np.int32)
+        w = weights_c(weights)
+        check(lib().gdock_batch_score(self.ctx.handle, self.handle, ptr(g), C.byref(w), ptr(out),
+                                      ptr(ns, C.c_int32)), self.ctx.handle, "gdock_batch_score")
+        return out, ns
+
+    def device_buffers(self):
+        p, o, n = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().gdock_batch_device_buffers(self.handle, C.byref(p), C.byref(o), C.byref(n)),
+              what="gdock_batch_device_buffers")
+        return p.value, o.value, n.value
+
+    def launch(self, weights: constants.EnergyWeights | None = None, stream: int | None = None) -> None:
+        w = weights_c(weights)
+        check(lib().gdock_batch_launch(self.ctx.handle, self.handle, C.byref(w), C.c_void_p(stream or 0)),
+              self.ctx.handle, "gdock_batch_launch")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().gdock_batch_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -136,6 +136,73 @@ gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const d
                                       int32_t n, const gdock_weights* w, double* d_out,
                                       int32_t* d_nsat, void* stream);
 
+/* ---- batched scoring (many complexes, one persistent launch) ------------- */
+/* Population::eval_fitness (src/population.rs:48-61) for several complexes at
+ * once: one generation of each complex's population, called where
+ * src/commands/run.rs:382,554 call eval_fitness, for many jobs together.
+ * The batch fixes the complexes and their pose counts; complex i owns pose
+ * rows [off_i, off_i + poses_per_complex[i]) in complex order.  At creation
+ * the library measures, per complex shape, how many ligand atoms each lane
+ * holds (1-8) and orders the work longest first; every launch is one replay
+ * of a captured HIP graph.  Complexes must outlive the batch.  Results equal
+ * gdock_score_poses within the scoring tolerance. */
+typedef struct gdock_batch gdock_batch;
+gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes, int32_t n,
+                                const int32_t* poses_per_complex, gdock_batch** out);
+void gdock_batch_destroy(gdock_ctx* ctx, gdock_batch* b);
+int32_t gdock_batch_n_poses(const gdock_batch* b);
+/* Chosen ligand atoms per lane (n entries), persistent grid and LDS bytes. */
+gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* grid, int32_t* lds_bytes);
+/* Synchronous: genes (n_poses x 6) in, out (n_poses x 6) and optional
+ * n_satisfied out, as gdock_score_poses. */
+gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* genes, const gdock_weights* w,
+                               double* out, int32_t* n_satisfied);
+/* Device-resident form: the batch's own HBM buffers (pose records of 12
+ * doubles as gdock_prepare_poses writes them, outputs n_poses x 6, satisfied
+ * counts) and an asynchronous graph replay on `stream`. */
+gdock_status gdock_batch_device_buffers(gdock_batch* b, double** d_poses, double** d_out, int32_t** d_nsat);
+gdock_status gdock_batch_launch(gdock_ctx* ctx, gdock_batch* b, const gdock_weights* w, void* stream);
+
+/* ---- GA docking job (host driver, device scoring) ------------------------ */
+/* commands::run (src/commands/run.rs:227-733) from the positioned complex on:
+ * initial population (src/chromosome.rs:27-54), per generation
+ * Population::eval_fitness on the device, Hall of Fame (run.rs:77-216),
+ * early stopping (run.rs:480-531), Population::evolve (src/population.rs:
+ * 74-158, src/ga.rs:7-39, src/chromosome.rs:56-77), then FCC clustering of
+ * the Hall of Fame (run.rs:665-693) and model selection (run.rs:694-733).
+ * The random stream is this library's (SplitMix64-seeded xoshiro256**,
+ * Box-Muller normals), not rand's StdRng: runs are reproducible per seed. */
+typedef struct gdock_ga_params {
+  int32_t population_size;   /* 150 (constants.rs:2) */
+  int32_t max_generations;   /* 250 */
+  int32_t elitism;           /* 5 */
+  int32_t tournament_size;   /* 3 */
+  int32_t convergence_window;/* 10 */
+  int32_t early_stopping;    /* 1 */
+  int32_t cluster;           /* 1: FCC-cluster the Hall of Fame and select models */
+  int32_t keep_trace;        /* 1: keep every generation's genes and fitness */
+  double mutation_rate;      /* 0.1 */
+  double crossover_rate;     /* 0.6 */
+  double convergence_threshold; /* 0.001 */
+  double max_displacement;   /* 50.0 */
+} gdock_ga_params;
+typedef struct gdock_ga_result gdock_ga_result;
+gdock_status gdock_ga_default_params(gdock_ga_params* p);
+/* One job, one plain scoring launch per generation (the drop-in path). */
+gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
+                          const gdock_ga_params* params, gdock_ga_result** out);
+void gdock_ga_free(gdock_ga_result* r);
+gdock_status gdock_ga_info(const gdock_ga_result* r, int32_t* generations, int32_t* converged_at,
+                           int32_t* hof_size, int32_t* n_clusters, int32_t* n_selected);
+/* With keep_trace: generations x population genes (x6) and fitness, and the
+ * index of each generation's best. */
+gdock_status gdock_ga_trace(const gdock_ga_result* r, double* genes, double* fitness, int32_t* best_idx);
+int32_t gdock_ga_hof(const gdock_ga_result* r, double* genes, double* fitness, int32_t* generation);
+int32_t gdock_ga_clusters(const gdock_ga_result* r, int64_t* center, int64_t* size);
+int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t* cluster_size);
+/* The first n outputs of the GA's random stream for a seed. */
+void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
+
 /* ---- FCC clustering ---------------------------------------------------------- */
 /* Contact bitsets (clustering::calculate_contacts, src/clustering.rs:62-115) of
  * the complexes receptor + (ligand moved by genes i).  Bit b of a pose's set

@@ -200,3 +200,25 @@ def test_lattice_cutoff_storm_uses_exact_path():
     ref = [orc.vdw_energy(rec.atoms, lig.atoms), orc.elec_energy(rec.atoms, lig.atoms),
            orc.desolv_energy(rec.atoms, lig.atoms)]
     assert_energies_close(got[0, :3], np.array(ref), "lattice")
+
+
+def test_serial_collisions_at_cutoffs():
+    """fitness.rs:92 skips vdw for a receptor and a ligand atom with the same
+    serial.  Colliding pairs placed inside, on and around the 1, 8, 12 and
+    15 A cutoffs, next to non-colliding pairs at the same distances."""
+    dists = [0.9, 1.0, 3.0, 7.99, 8.0, 11.5, 11.999999, 12.0, 12.000001, 14.0, 15.0, 20.0]
+    rec = structure.Molecule(mol(*[A(0.0, 40.0 * i, 0.0, charge=0.3, element="O", serial=100 + i, resseq=i + 1)
+                                   for i in range(len(dists))]))
+    lig_atoms = []
+    for i, d in enumerate(dists):  # colliding partner along +x, a distinct-serial twin along -x
+        lig_atoms.append(A(d, 40.0 * i, 0.0, charge=-0.4, element="N", serial=100 + i, chainid="B", resseq=i + 1))
+        lig_atoms.append(A(-d, 40.0 * i, 0.0, charge=-0.4, element="C", serial=900 + i, chainid="B", resseq=i + 1))
+    lig = structure.Molecule(mol(*lig_atoms))
+    cx = Complex(rec, lig)
+    got, _ = cx.score_coords(lig.xyz[None])
+    ref = [orc.vdw_energy(rec.atoms, lig.atoms), orc.elec_energy(rec.atoms, lig.atoms),
+           orc.desolv_energy(rec.atoms, lig.atoms)]
+    assert_energies_close(got[0, :3], np.array(ref), "serial collisions")
+    # and moved by a pose, through the transform path
+    genes = np.array([[0.3, 0.2, 0.1, 0.05, -0.02, 0.01], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]])
+    _check_poses(rec, lig, cx, [], genes, "serial collisions, posed")
