@@ -47,6 +47,30 @@ class ClusterResultC(C.Structure):
     _fields_ = [("center_idx", C.c_int64), ("size", C.c_int64), ("members_offset", C.c_int64)]
 
 
+class GaParams(C.Structure):
+    _fields_ = [("population_size", C.c_int32), ("max_generations", C.c_int32), ("elitism", C.c_int32),
+                ("tournament_size", C.c_int32), ("convergence_window", C.c_int32), ("early_stopping", C.c_int32),
+                ("cluster", C.c_int32), ("keep_trace", C.c_int32), ("mutation_rate", C.c_double),
+                ("crossover_rate", C.c_double), ("convergence_threshold", C.c_double),
+                ("max_displacement", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_jobs", C.c_int32), ("max_population", C.c_int32), ("max_receptor_atoms", C.c_int32),
+                ("reserved", C.c_int32), ("memory_budget", C.c_int64), ("weights", Weights)]
+
+
+class JobStats(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("generations", C.c_int32), ("slot", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("poses_scored", C.c_int64), ("jobs_finished", C.c_int64),
+                ("jobs_queued", C.c_int64), ("peak_active", C.c_int32), ("reserved", C.c_int32),
+                ("busy_s", C.c_double), ("bytes_in_use", C.c_int64), ("memory_budget", C.c_int64)]
+
+
 class GdockError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -108,33 +132,29 @@ def lib() -> C.CDLL:
         "gdock_batch_score": (C.c_int, [P, P, D, C.POINTER(Weights), D, I32]),
         "gdock_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "gdock_batch_launch": (C.c_int, [P, P, C.POINTER(Weights), P]),
-    }
-    for name, (res, args) in sig.items():
-        f = getattr(L, name)
-        f.restype = res
-        f.argtypes = args
-    _register_optional(L)
-    _lib = L
-    return L
-
-
-def _register_optional(L: C.CDLL) -> None:
-    """Entry points of the GA and serving layers."""
-    opt = {
-        "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.c_int32, C.c_int32, C.c_int, P, C.POINTER(P)]),
+        "gdock_ga_default_params": (C.c_int, [C.POINTER(GaParams)]),
+        "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_free": (None, [P]),
-        "gdock_ga_info": (C.c_int, [P, I32]),
+        "gdock_ga_info": (C.c_int, [P, I32, I32, I32, I32, I32]),
         "gdock_ga_trace": (C.c_int, [P, D, D, I32]),
         "gdock_ga_hof": (C.c_int32, [P, D, D, I32]),
         "gdock_ga_clusters": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_ga_selected": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_rng_stream": (None, [C.c_uint64, C.c_int32, C.POINTER(C.c_uint64)]),
+        "gdock_server_default_config": (C.c_int, [C.POINTER(ServerConfig)]),
+        "gdock_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(P)]),
+        "gdock_server_destroy": (None, [P]),
+        "gdock_server_submit": (C.c_int, [P, P, C.c_int32, P, C.c_int32, I32, C.c_int32, C.c_uint64,
+                                          C.POINTER(GaParams), C.POINTER(C.c_int64)]),
+        "gdock_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(P), C.POINTER(JobStats)]),
+        "gdock_server_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
     }
-    for name, (res, args) in opt.items():
-        f = getattr(L, name, None)
-        if f is not None:
-            f.restype = res
-            f.argtypes = args
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
 
 
 def ptr(a: np.ndarray, ctype=C.c_double):

@@ -13,8 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libgdock_hip.so"
-SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip", "batch.cpp", "ga.cpp"]
-EXTRA_SOURCES = ["serve.cpp"]
+SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip", "batch.cpp", "ga.cpp", "serve.cpp"]
 
 # -ffp-contract=off: FP64 sums and products round as in the reference (no
 # implicit FMA); the FP32 pair loop spells its FMAs out.  Packed FP32
@@ -35,9 +34,7 @@ def hipcc() -> str:
 
 
 def sources() -> list[str]:
-    src = [str(PKG / "csrc" / s) for s in SOURCES]
-    src += [str(PKG / "csrc" / s) for s in EXTRA_SOURCES if (PKG / "csrc" / s).exists()]
-    return src
+    return [str(PKG / "csrc" / s) for s in SOURCES]
 
 
 def build(verbose: bool = True) -> Path:

@@ -244,16 +244,18 @@ void GaJob::select_models(const std::vector<gdock_cluster_result>& clusters) {
 
 using namespace gdock;
 
-struct gdock_ga_result {
-  GaJob job;
-  std::vector<gdock_cluster_result> clusters;
-  explicit gdock_ga_result(uint64_t seed, const gdock_ga_params& p) : job(seed, p) {}
-};
-
 namespace gdock {
 
-// Clusters the Hall of Fame poses on the device (run.rs:665-693) and selects
-// the output models.
+gdock_status validate_ga_params(gdock_ctx* ctx, const gdock_ga_params* p) {
+  if (!p || p->population_size < 2 || p->population_size > (1 << 20) || p->max_generations < 1 || p->elitism < 0 ||
+      p->tournament_size < 1 || p->convergence_window < 1 || !(p->max_displacement >= 0.0) ||
+      !std::isfinite(p->max_displacement) || !std::isfinite(p->mutation_rate) || !std::isfinite(p->crossover_rate)) {
+    if (ctx) set_error(ctx, "ga: invalid parameters");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  return GDOCK_OK;
+}
+
 gdock_status finish_job(gdock_ctx* ctx, gdock_complex* cx, GaJob* job, std::vector<gdock_cluster_result>* clusters) {
   const auto& h = job->hof().entries();
   const int32_t n = static_cast<int32_t>(h.size());
@@ -294,16 +296,6 @@ gdock_status gdock_ga_default_params(gdock_ga_params* p) {
   return GDOCK_OK;
 }
 
-gdock_status gdock_ga_validate(gdock_ctx* ctx, const gdock_ga_params* p) {
-  if (!p || p->population_size < 2 || p->population_size > (1 << 20) || p->max_generations < 1 || p->elitism < 0 ||
-      p->tournament_size < 1 || p->convergence_window < 1 || !(p->max_displacement >= 0.0) ||
-      !std::isfinite(p->max_displacement) || !std::isfinite(p->mutation_rate) || !std::isfinite(p->crossover_rate)) {
-    if (ctx) set_error(ctx, "ga: invalid parameters");
-    return GDOCK_ERR_INVALID_ARG;
-  }
-  return GDOCK_OK;
-}
-
 gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
                           const gdock_ga_params* params, gdock_ga_result** out) {
   if (!ctx || !cx || !w || !out) {
@@ -312,7 +304,7 @@ gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights
   }
   gdock_ga_params p;
   if (params) p = *params; else gdock_ga_default_params(&p);
-  gdock_status st = gdock_ga_validate(ctx, &p);
+  gdock_status st = validate_ga_params(ctx, &p);
   if (st != GDOCK_OK) return st;
   auto* r = new gdock_ga_result(seed, p);
   GaJob& job = r->job;

@@ -87,4 +87,17 @@ class GaJob {
   bool done_ = false;
 };
 
+gdock_status validate_ga_params(gdock_ctx* ctx, const gdock_ga_params* p);
+
+// Clusters the job's Hall of Fame on the device (run.rs:665-693) and selects
+// its output models.
+gdock_status finish_job(gdock_ctx* ctx, gdock_complex* cx, GaJob* job, std::vector<gdock_cluster_result>* clusters);
+
 }  // namespace gdock
+
+// A finished job: its GA state and the FCC clusters of its Hall of Fame.
+struct gdock_ga_result {
+  gdock::GaJob job;
+  std::vector<gdock_cluster_result> clusters;
+  gdock_ga_result(uint64_t seed, const gdock_ga_params& p) : job(seed, p) {}
+};

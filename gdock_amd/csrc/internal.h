@@ -91,6 +91,9 @@ struct Layout {
     return off;
   }
 };
+// Ligand atoms per lane for a complex, measured once per complex shape.
+int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err);
+
 void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host);
 void bind_contact_tables(gdock_complex* cx);
 void release_fcc_scratch(gdock_ctx* ctx);

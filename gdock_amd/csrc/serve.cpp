@@ -1,0 +1,427 @@
+// Docking server: continuous batching of GA jobs on one GPU.
+//
+// Jobs are queued by gdock_server_submit and admitted by a worker thread as
+// slots and device memory free up.  Each admitted job keeps its complex
+// (receptor, positioned ligand, restraints, contact tables) resident in HBM
+// until it finishes.  Every scheduling step scores the current generation of
+// every active job in one persistent launch (replayed from a graph captured
+// once at fixed capacity), advances each job's GA on the host, and retires
+// finished jobs after clustering their Hall of Fame.  A job's results do not
+// depend on which other jobs share its launches: every pose is scored by the
+// same kernel code with the same per-shape choice of lanes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+#include "ga.h"
+#include "internal.h"
+
+using namespace gdock;
+
+namespace {
+
+enum JobState : int32_t { kQueued = 0, kActive = 1, kFinished = 2, kFailed = 3 };
+
+struct Job {
+  int64_t id = 0;
+  std::vector<gdock_atom> rec, lig;
+  std::vector<int32_t> pairs;
+  uint64_t seed = 0;
+  gdock_ga_params params{};
+  JobState state = kQueued;
+  gdock_status error = GDOCK_OK;
+  std::string message;
+  double t_submit = 0, t_admit = 0, t_finish = 0;
+  gdock_complex* cx = nullptr;
+  size_t bytes = 0;   // device bytes held while active
+  int slot = -1;
+  int lanes = 0;
+  std::unique_ptr<gdock_ga_result> result;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct gdock_server {
+  gdock_server_config cfg{};
+  gdock_ctx* ctx = nullptr;
+  std::chrono::steady_clock::time_point t0;
+  // device state of the launches
+  BatchLaunch L{};
+  void* mem = nullptr;
+  double* h_poses = nullptr;  // pinned
+  double* h_out = nullptr;
+  int2* h_items = nullptr;
+  DevComplex* h_table = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int max_pop = 0;
+  // jobs
+  std::mutex mu;
+  std::condition_variable cv;      // worker: new work or stop
+  std::condition_variable done_cv; // waiters: a job finished
+  std::deque<Job*> queue;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Job*> slots;
+  size_t bytes_in_use = 0;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  // statistics
+  int64_t n_steps = 0, n_poses = 0, n_finished = 0;
+  int32_t peak_active = 0;
+  double busy_s = 0;
+
+  void run();
+  void admit_locked(std::vector<Job*>* fresh);
+  bool step();
+};
+
+// Moves queued jobs into free slots while their device memory fits the
+// budget (first come, first served; the head of the queue waits for memory).
+void gdock_server::admit_locked(std::vector<Job*>* fresh) {
+  while (!queue.empty()) {
+    Job* j = queue.front();
+    const auto free_slot = std::find(slots.begin(), slots.end(), nullptr);
+    if (free_slot == slots.end()) return;
+    // Device bytes of a job: admission uses an estimate from the atom counts
+    // (256 B per atom and 1 MiB for contact tables and launch state); once
+    // the complex exists the larger of the estimate and its arena is charged.
+    const size_t est = 256 * (j->rec.size() + j->lig.size()) + (size_t{1} << 20);
+    const bool nothing_active = std::all_of(slots.begin(), slots.end(), [](Job* s) { return s == nullptr; });
+    if (bytes_in_use + est > static_cast<size_t>(cfg.memory_budget) && !nothing_active) return;
+    queue.pop_front();
+    j->slot = static_cast<int>(free_slot - slots.begin());
+    *free_slot = j;
+    j->state = kActive;
+    j->bytes = est;
+    bytes_in_use += est;
+    fresh->push_back(j);
+  }
+}
+
+// One scheduling step; returns false when there was nothing to run.
+bool gdock_server::step() {
+  std::vector<Job*> fresh, active;
+  {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return stop || !queue.empty() || std::any_of(slots.begin(), slots.end(), [](Job* s) { return s; }); });
+    if (stop) return false;
+    admit_locked(&fresh);
+  }
+  // Materialise newly admitted jobs on the device (outside the lock).
+  for (Job* j : fresh) {
+    j->t_admit = now_s(t0);
+    gdock_status st = GDOCK_OK;
+    std::vector<gdock_atom> lig = j->lig;
+    if (!lig.empty() && !j->rec.empty())
+      st = gdock_position_ligand(j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
+                                 static_cast<int32_t>(lig.size()));
+    if (st == GDOCK_OK)
+      st = gdock_complex_create(ctx, j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
+                                static_cast<int32_t>(lig.size()), j->pairs.data(),
+                                static_cast<int32_t>(j->pairs.size() / 2), &j->cx);
+    if (st == GDOCK_OK && j->cx->dev.n_rec > cfg.max_receptor_atoms + 64) {
+      set_error(ctx, "receptor of %d slots exceeds the server's %d", j->cx->dev.n_rec, cfg.max_receptor_atoms);
+      st = GDOCK_ERR_INVALID_ARG;
+    }
+    hipError_t he = hipSuccess;
+    if (st == GDOCK_OK) {
+      j->lanes = tune_lanes(ctx, j->cx, &he);
+      if (he != hipSuccess) st = hip_fail(ctx, he, "server: autotune");
+    }
+    if (st == GDOCK_OK) {
+      j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
+      DevComplex d = j->cx->dev;
+      d.lanes_k = j->lanes;
+      h_table[j->slot] = d;
+      he = hipMemcpyAsync(const_cast<DevComplex*>(L.table) + j->slot, &h_table[j->slot], sizeof(DevComplex),
+                          hipMemcpyHostToDevice, ctx->stream);
+      if (he != hipSuccess) st = hip_fail(ctx, he, "server: table upload");
+    }
+    std::lock_guard<std::mutex> g(mu);
+    if (st == GDOCK_OK) {  // charge the larger of the estimate and the arena
+      const size_t charged = std::max(j->bytes, j->cx->mem_bytes);
+      bytes_in_use += charged - j->bytes;
+      j->bytes = charged;
+    } else {
+      j->state = kFailed;
+      j->error = st;
+      j->message = ctx->err;
+      j->t_finish = now_s(t0);
+      if (j->cx) { gdock_complex_destroy(ctx, j->cx); j->cx = nullptr; }
+      bytes_in_use -= j->bytes;
+      slots[j->slot] = nullptr;
+      ++n_finished;
+      done_cv.notify_all();
+    }
+  }
+  {
+    std::lock_guard<std::mutex> g(mu);
+    for (Job* s : slots)
+      if (s && s->state == kActive) active.push_back(s);
+    peak_active = std::max<int32_t>(peak_active, static_cast<int32_t>(active.size()));
+  }
+  if (active.empty()) return true;
+  const auto t_step = std::chrono::steady_clock::now();
+
+  // Items longest first; pose rows slot * max_pop + p.
+  std::stable_sort(active.begin(), active.end(), [](Job* a, Job* b) {
+    auto cost = [](Job* j) {
+      const int64_t lanes = static_cast<int64_t>(kScoreThreads) * j->lanes;
+      return static_cast<int64_t>(j->cx->dev.n_rec) * ((j->cx->dev.n_lig + lanes - 1) / lanes) * lanes;
+    };
+    return cost(a) > cost(b);
+  });
+  int n_items = 0, max_row = 0;
+  for (Job* j : active) {
+    const GaJob& ga = j->result->job;
+    const int pop = ga.population();
+    const int row0 = j->slot * max_pop;
+    gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
+    for (int p = 0; p < pop; ++p) h_items[n_items++] = make_int2(j->slot, row0 + p);
+    max_row = std::max(max_row, row0 + pop);
+  }
+  int32_t head = n_items;
+  hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
+                                static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<int2*>(L.items), h_items, static_cast<size_t>(n_items) * sizeof(int2),
+                       hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<int32_t*>(L.n_items), &head, sizeof head, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_out, L.out, static_cast<size_t>(max_row) * kOutDoubles * sizeof(double),
+                       hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
+  ++n_steps;
+  n_poses += n_items;
+
+  std::vector<Job*> finished;
+  for (Job* j : active) {
+    if (e != hipSuccess) {
+      j->state = kFailed;
+      j->error = hip_fail(ctx, e, "server: generation launch");
+      j->message = ctx->err;
+      finished.push_back(j);
+      continue;
+    }
+    GaJob& ga = j->result->job;
+    ga.absorb(h_out + static_cast<size_t>(j->slot) * max_pop * kOutDoubles);
+    if (!ga.done()) continue;
+    if (j->params.cluster) {
+      const gdock_status st = finish_job(ctx, j->cx, &ga, &j->result->clusters);
+      if (st != GDOCK_OK) { j->state = kFailed; j->error = st; j->message = ctx->err; }
+    }
+    finished.push_back(j);
+  }
+  for (Job* j : finished) {
+    gdock_complex_destroy(ctx, j->cx);
+    j->cx = nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (j->state == kActive) j->state = kFinished;
+    j->t_finish = now_s(t0);
+    bytes_in_use -= j->bytes;
+    slots[j->slot] = nullptr;
+    ++n_finished;
+  }
+  if (!finished.empty()) done_cv.notify_all();
+  return true;
+}
+
+void gdock_server::run() {
+  (void)hipSetDevice(ctx->device);
+  while (step()) {
+  }
+}
+
+extern "C" {
+
+gdock_status gdock_server_default_config(gdock_server_config* c) {
+  if (!c) return GDOCK_ERR_INVALID_ARG;
+  *c = gdock_server_config{};
+  c->max_jobs = 64;
+  c->max_population = 150;
+  c->max_receptor_atoms = 8192;
+  c->memory_budget = 0;
+  c->weights = gdock_weights{0.4, 0.05, 3.4, 1.0};
+  return GDOCK_OK;
+}
+
+gdock_status gdock_server_create(int device, const gdock_server_config* config, gdock_server** out) {
+  if (!out) return GDOCK_ERR_INVALID_ARG;
+  *out = nullptr;
+  gdock_server_config c;
+  if (config) c = *config; else gdock_server_default_config(&c);
+  if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_receptor_atoms < 1 ||
+      c.max_receptor_atoms > kMaxLdsReceptor - 64 || c.memory_budget < 0)
+    return GDOCK_ERR_INVALID_ARG;
+  auto* s = new gdock_server();
+  s->cfg = c;
+  s->t0 = std::chrono::steady_clock::now();
+  gdock_status st = gdock_ctx_create(device, &s->ctx);
+  if (st != GDOCK_OK) { delete s; return st; }
+  if (s->cfg.memory_budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) s->cfg.memory_budget = static_cast<int64_t>(fr / 2);
+    else s->cfg.memory_budget = int64_t{8} << 30;
+  }
+  s->max_pop = c.max_population;
+  s->slots.assign(c.max_jobs, nullptr);
+  const size_t cap = static_cast<size_t>(c.max_jobs) * c.max_population;
+  Layout Ly;
+  const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_items = Ly.add<int2>(cap), o_n = Ly.add<int32_t>(4),
+               o_poses = Ly.add<double>(cap * kPoseDoubles), o_out = Ly.add<double>(cap * kOutDoubles),
+               o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap);
+  hipError_t e = hipMalloc(&s->mem, Ly.size);
+  if (e == hipSuccess) e = hipMemsetAsync(s->mem, 0, Ly.size, s->ctx->stream);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), cap * kOutDoubles * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items), cap * sizeof(int2));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_table), c.max_jobs * sizeof(DevComplex));
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  char* m = static_cast<char*>(s->mem);
+  BatchLaunch& L = s->L;
+  if (e == hipSuccess) {
+    L.table = reinterpret_cast<const DevComplex*>(m + o_tab);
+    L.items = reinterpret_cast<const int2*>(m + o_items);
+    L.n_items = reinterpret_cast<const int32_t*>(m + o_n);
+    L.counter = reinterpret_cast<int32_t*>(m + o_n) + 1;
+    L.exit_count = reinterpret_cast<int32_t*>(m + o_n) + 2;
+    L.poses = reinterpret_cast<const double*>(m + o_poses);
+    L.out = reinterpret_cast<double*>(m + o_out);
+    L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
+    L.redo = reinterpret_cast<int32_t*>(m + o_redo);
+    L.lds = batch_lds_bytes(c.max_receptor_atoms + 64);
+    L.capacity = static_cast<int>(cap);
+    L.grid = std::min(batch_grid(L.lds, device), L.capacity);
+    // One graph for every step: the kernels read the item count from device memory.
+    hipStream_t cs = nullptr;
+    const Weights w{c.weights.vdw, c.weights.elec, c.weights.desolv, c.weights.air};
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t el = launch_batch(L, w, cs);
+      e = hipStreamEndCapture(cs, &s->graph);
+      if (e == hipSuccess) e = el;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+  }
+  if (e != hipSuccess) {
+    gdock_server_destroy(s);
+    return GDOCK_ERR_DEVICE;
+  }
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return GDOCK_OK;
+}
+
+void gdock_server_destroy(gdock_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& kv : s->jobs)
+    if (kv.second->cx) gdock_complex_destroy(s->ctx, kv.second->cx);
+  if (s->ctx) (void)hipSetDevice(s->ctx->device);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->mem) (void)hipFree(s->mem);
+  if (s->h_poses) (void)hipHostFree(s->h_poses);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+  if (s->h_items) (void)hipHostFree(s->h_items);
+  if (s->h_table) (void)hipHostFree(s->h_table);
+  if (s->ctx) gdock_ctx_destroy(s->ctx);
+  delete s;
+}
+
+gdock_status gdock_server_submit(gdock_server* s, const gdock_atom* rec, int32_t n_rec, const gdock_atom* lig,
+                                 int32_t n_lig, const int32_t* restraint_pairs, int32_t n_pairs, uint64_t seed,
+                                 const gdock_ga_params* params, int64_t* job_id) {
+  if (!s || !job_id || n_rec < 0 || n_lig < 0 || n_pairs < 0 || (n_rec && !rec) || (n_lig && !lig) ||
+      (n_pairs && !restraint_pairs))
+    return GDOCK_ERR_INVALID_ARG;
+  gdock_ga_params p;
+  if (params) p = *params; else gdock_ga_default_params(&p);
+  if (validate_ga_params(nullptr, &p) != GDOCK_OK || p.population_size > s->max_pop) return GDOCK_ERR_INVALID_ARG;
+  auto j = std::make_unique<Job>();
+  j->rec.assign(rec, rec + n_rec);
+  j->lig.assign(lig, lig + n_lig);
+  j->pairs.assign(restraint_pairs, restraint_pairs + 2 * n_pairs);
+  j->seed = seed;
+  j->params = p;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    j->id = s->next_id++;
+    j->t_submit = now_s(s->t0);
+    *job_id = j->id;
+    s->queue.push_back(j.get());
+    s->jobs.emplace(j->id, std::move(j));
+  }
+  s->cv.notify_all();
+  return GDOCK_OK;
+}
+
+gdock_status gdock_server_wait(gdock_server* s, int64_t job_id, double timeout_s, gdock_ga_result** result,
+                               gdock_job_stats* stats) {
+  if (!s) return GDOCK_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return GDOCK_ERR_NOT_FOUND;
+  Job* j = it->second.get();
+  auto finished = [j] { return j->state == kFinished || j->state == kFailed; };
+  if (timeout_s < 0) {
+    s->done_cv.wait(lk, finished);
+  } else if (!s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished)) {
+    return GDOCK_ERR_BUSY;
+  }
+  if (stats) {
+    stats->submit_s = j->t_submit;
+    stats->admit_s = j->t_admit;
+    stats->finish_s = j->t_finish;
+    stats->generations = j->result ? j->result->job.generations() : 0;
+    stats->slot = j->slot;
+  }
+  const gdock_status err = j->state == kFailed ? j->error : GDOCK_OK;
+  if (result) *result = err == GDOCK_OK ? j->result.release() : nullptr;
+  s->jobs.erase(it);
+  return err;
+}
+
+gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out) {
+  if (!s || !out) return GDOCK_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  out->steps = s->n_steps;
+  out->poses_scored = s->n_poses;
+  out->jobs_finished = s->n_finished;
+  out->jobs_queued = static_cast<int64_t>(s->queue.size());
+  out->peak_active = s->peak_active;
+  out->busy_s = s->busy_s;
+  out->bytes_in_use = static_cast<int64_t>(s->bytes_in_use);
+  out->memory_budget = s->cfg.memory_budget;
+  return GDOCK_OK;
+}
+
+}  // extern "C"

@@ -203,6 +203,51 @@ int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t*
 /* The first n outputs of the GA's random stream for a seed. */
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
 
+/* ---- docking server (continuous batching) -------------------------------- */
+/* Many commands::run jobs (src/commands/run.rs:227-733) served concurrently
+ * on one GPU.  gdock_server_submit queues a job (receptor and raw ligand
+ * atoms as structure::read_pdb gives them, restraint residue pairs, seed, GA
+ * parameters); a worker thread admits queued jobs as slots and device memory
+ * free up, keeps each admitted job's complex resident in HBM, scores the
+ * current generation of every active job in one persistent launch per step
+ * (one graph, captured at creation), advances each job's GA, and clusters a
+ * finished job's Hall of Fame.  A job's result is the same whichever jobs
+ * share its launches.  The server owns its own context on `device`. */
+typedef struct gdock_server_config {
+  int32_t max_jobs;            /* active job slots (64) */
+  int32_t max_population;      /* largest population a job may use (150) */
+  int32_t max_receptor_atoms;  /* sizes the kernel's LDS receptor table (8192) */
+  int32_t reserved;
+  int64_t memory_budget;       /* device bytes for resident jobs; 0: half the free memory */
+  gdock_weights weights;       /* one energy weighting for every job */
+} gdock_server_config;
+typedef struct gdock_job_stats {
+  double submit_s, admit_s, finish_s;  /* seconds since the server started */
+  int32_t generations, slot;
+} gdock_job_stats;
+typedef struct gdock_server_stats_t {
+  int64_t steps, poses_scored, jobs_finished, jobs_queued;
+  int32_t peak_active, reserved;
+  double busy_s;               /* host time spent in scoring steps */
+  int64_t bytes_in_use, memory_budget;
+} gdock_server_stats_t;
+typedef struct gdock_server gdock_server;
+gdock_status gdock_server_default_config(gdock_server_config* c);
+gdock_status gdock_server_create(int device, const gdock_server_config* config, gdock_server** out);
+/* Stops the worker; jobs still queued or running are dropped. */
+void gdock_server_destroy(gdock_server* s);
+/* Thread safe.  The ligand is positioned on the receptor (utils::
+ * position_ligand) at admission.  params: NULL for the defaults. */
+gdock_status gdock_server_submit(gdock_server* s, const gdock_atom* rec, int32_t n_rec, const gdock_atom* lig,
+                                 int32_t n_lig, const int32_t* restraint_pairs, int32_t n_pairs, uint64_t seed,
+                                 const gdock_ga_params* params, int64_t* job_id);
+/* Blocks until the job ends (timeout_s < 0: no limit; GDOCK_ERR_BUSY on
+ * timeout).  On success *result (free with gdock_ga_free) holds the job's GA
+ * and clusters; the job id is released either way once it has ended. */
+gdock_status gdock_server_wait(gdock_server* s, int64_t job_id, double timeout_s, gdock_ga_result** result,
+                               gdock_job_stats* stats);
+gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out);
+
 /* ---- FCC clustering ---------------------------------------------------------- */
 /* Contact bitsets (clustering::calculate_contacts, src/clustering.rs:62-115) of
  * the complexes receptor + (ligand moved by genes i).  Bit b of a pose's set

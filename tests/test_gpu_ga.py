@@ -1,0 +1,99 @@
+"""GA docking job on the device against the oracle's GA (run.rs restated),
+same seed, same random stream.
+
+The device scores every generation within the energy tolerance of the FP64
+oracle, so the two runs take the same decisions (tournaments, elite, Hall of
+Fame, early stop) unless two fitness values in a generation are ordered
+differently by the two scorers, which can only happen when they lie within
+the tolerance of each other (a near-tie).  The test walks the generations:
+populations must be bit-identical while every decision is identical; at the
+first generation whose fitness order differs it checks that the pair is a
+near-tie and stops comparing.  With no such generation the whole run
+(generations, Hall of Fame, clusters, selected models) must match."""
+import numpy as np
+import pytest
+
+import oracle.oracle as orc
+from gdock_amd import ga, structure, synth
+from gdock_amd.device import Complex
+from helpers import RESTRAINTS_2OOB, read_text
+
+pytestmark = pytest.mark.gpu
+W = (0.4, 0.05, 3.4, 1.0)
+
+
+def _fitness_tol(f):
+    # components within max(1e-5 |E|, 1e-3): the weighted sum within the weighted tolerances
+    return 2e-5 * np.abs(f) + (0.4 + 0.05 + 3.4 + 1.0) * 1e-3
+
+
+def _order_flip(fg, fo):
+    """A pair (i, j) ordered differently by the two fitness vectors, or None."""
+    sg = np.sign(fg[:, None] - fg[None, :])
+    so = np.sign(fo[:, None] - fo[None, :])
+    bad = np.argwhere(sg != so)
+    return None if len(bad) == 0 else tuple(bad[0])
+
+
+def _compare(res, ora, pop, threshold=0.001):
+    """Walks both traces; returns how the comparison ended."""
+    G = min(res.generations, ora.generations)
+    for g in range(G):
+        assert np.array_equal(res.genes[g], ora.genes[g]), f"generation {g}: populations differ with equal decisions"
+        fg, fo = res.fitness[g], ora.fitness[g]
+        tol = _fitness_tol(fo)
+        assert (np.abs(fg - fo) <= tol).all(), f"generation {g}: fitness off by {np.abs(fg - fo).max()}"
+        flip = _order_flip(fg, fo)
+        if flip is not None:  # the scorers order two individuals differently: must be a near-tie
+            i, j = flip
+            assert abs(fo[i] - fo[j]) <= tol[i] + tol[j], f"generation {g}: order flip of a non-tie {fo[i]} {fo[j]}"
+            return f"near-tie at generation {g}"
+        if g > 0:  # run.rs early stop: relative improvement of the best against the threshold
+            bg0, bg1 = res.fitness[g - 1].min(), fg.min()
+            bo0, bo1 = ora.fitness[g - 1].min(), fo.min()
+            ig, io = (bg0 - bg1) / abs(bg0), (bo0 - bo1) / abs(bo0)
+            if (ig < threshold) != (io < threshold):
+                assert abs(io - threshold) <= abs(ig - io) + 1e-12, f"generation {g}: early-stop decision differs"
+                return f"early-stop near-tie at generation {g}"
+    assert res.generations == ora.generations and res.converged_at == ora.converged_at
+    assert np.array_equal(res.best_idx, ora.best_idx)
+    assert np.array_equal(res.hof_genes, ora.hof_genes)
+    assert np.array_equal(res.hof_generation, ora.hof_generation)
+    assert (np.abs(res.hof_fitness - ora.hof_fitness) <= _fitness_tol(ora.hof_fitness)).all()
+    assert res.cluster_center.tolist() == ora.cluster_center.tolist()
+    assert res.cluster_size.tolist() == ora.cluster_size.tolist()
+    assert res.selected == ora.selected
+    return "identical"
+
+
+@pytest.mark.parametrize("seed", [42, 7])
+def test_ga_synthetic_60x76_matches_oracle(seed):
+    c = synth.make_complex(0, rec_res=60, lig_res=76, n_restraints=6)
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+    cx = Complex(rec, lig, c.restraints)
+    res = ga.run(cx, seed=seed, keep_trace=1)
+    ora = orc.GaRun(rec.atoms, lig.atoms, c.restraints, W, seed=seed, pop_size=150, max_gen=250, nthreads=16)
+    outcome = _compare(res, ora, 150)
+    print(f"seed {seed}: {res.generations} generations, {len(res.hof_fitness)} HoF, outcome: {outcome}")
+
+
+def test_ga_2oob_short_run_matches_oracle():
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(read_text("2oob_B.pdb"))[0])
+    cx = Complex(rec, lig, RESTRAINTS_2OOB)
+    res = ga.run(cx, seed=3, keep_trace=1, max_generations=25)
+    ora = orc.GaRun(rec.atoms, lig.atoms, RESTRAINTS_2OOB, W, seed=3, pop_size=150, max_gen=25, nthreads=16)
+    outcome = _compare(res, ora, 150)
+    print(f"2oob: {res.generations} generations, outcome: {outcome}")
+
+
+def test_ga_small_population_parameters():
+    c = synth.make_complex(5, rec_res=30, lig_res=25, n_restraints=4)
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+    cx = Complex(rec, lig, c.restraints)
+    res = ga.run(cx, seed=11, keep_trace=1, population_size=40, max_generations=30)
+    ora = orc.GaRun(rec.atoms, lig.atoms, c.restraints, W, seed=11, pop_size=40, max_gen=30, nthreads=16)
+    _compare(res, ora, 40)
+    assert res.generations <= 30 and len(res.selected) <= 5

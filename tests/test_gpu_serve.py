@@ -1,0 +1,90 @@
+"""Docking server: continuous batching of GA jobs (gdock_server_*).
+
+A job's results must be the same alone and in a mix of other jobs: every
+pose is scored by the same kernel code with the same per-shape lane choice,
+so the whole GA (populations, fitness, Hall of Fame, clusters, models) is
+bit-identical.  A served job is also checked against the oracle's GA."""
+import numpy as np
+import pytest
+
+import oracle.oracle as orc
+from gdock_amd import _lib, structure, synth
+from gdock_amd.serve import Server
+from test_gpu_ga import _compare
+
+pytestmark = pytest.mark.gpu
+W = (0.4, 0.05, 3.4, 1.0)
+SMALL = dict(population_size=60, max_generations=40, keep_trace=1)
+
+
+def _job(seed, rec_res, lig_res, n_restraints=5):
+    c = synth.make_complex(seed, rec_res=rec_res, lig_res=lig_res, n_restraints=n_restraints)
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.read_pdb_text(c.ligand_pdb)[0]  # raw: the server positions it
+    return rec, lig, c.restraints
+
+
+def _same(a, b):
+    assert a.generations == b.generations and a.converged_at == b.converged_at
+    assert np.array_equal(a.genes, b.genes) and np.array_equal(a.fitness, b.fitness)
+    assert np.array_equal(a.hof_genes, b.hof_genes) and np.array_equal(a.hof_fitness, b.hof_fitness)
+    assert a.cluster_center.tolist() == b.cluster_center.tolist()
+    assert a.cluster_size.tolist() == b.cluster_size.tolist()
+    assert a.selected == b.selected
+
+
+def test_alone_equals_in_mix():
+    jobs = [_job(60 + i, 20 + 7 * i, 18 + 5 * i) for i in range(5)]
+    srv = Server(max_jobs=8)
+    try:
+        alone, _ = srv.wait(srv.submit(*jobs[0], seed=3, **SMALL), 180.0)
+        order = [1, 2, 0, 3, 4]
+        ids = [srv.submit(*jobs[i], seed=3 if i == 0 else 10 + i, **SMALL) for i in order]
+        got = {i: srv.wait(j, 180.0) for i, j in zip(order, ids)}
+        _same(alone, got[0][0])
+        assert srv.stats()["peak_active"] >= 2
+        for i in order:
+            assert got[i][1].latency_s > 0 and got[i][1].generations == got[i][0].generations
+    finally:
+        srv.close()
+
+
+def test_served_job_matches_oracle_ga():
+    rec, lig, pairs = _job(0, 60, 76, 6)
+    srv = Server(max_jobs=4)
+    try:
+        jid = srv.submit(rec, lig, pairs, seed=42, keep_trace=1)
+        res, t = srv.wait(jid, 180.0)
+    finally:
+        srv.close()
+    lig_pos = structure.position_ligand(rec, lig)
+    ora = orc.GaRun(rec.atoms, lig_pos.atoms, pairs, W, seed=42, pop_size=150, max_gen=250, nthreads=16)
+    print("served vs oracle:", _compare(res, ora, 150), f"latency {t.latency_s:.2f} s")
+
+
+def test_memory_admission_limits_active_jobs():
+    jobs = [_job(80 + i, 25, 20) for i in range(5)]
+    est = max(256 * (len(r) + len(l)) + (1 << 20) for r, l, _ in jobs)
+    srv = Server(max_jobs=8, memory_budget=2 * est + 1000)
+    try:
+        ids = [srv.submit(*j, seed=5 + k, population_size=40, max_generations=12) for k, j in enumerate(jobs)]
+        for j in ids:
+            res, t = srv.wait(j, 180.0)
+            assert res.generations >= 1 and t.admit_s >= t.submit_s
+        st = srv.stats()
+        assert st["jobs_finished"] == 5 and st["peak_active"] <= 2 and st["bytes_in_use"] == 0
+    finally:
+        srv.close()
+
+
+def test_submit_rejects_bad_parameters():
+    rec, lig, pairs = _job(90, 20, 15)
+    srv = Server(max_jobs=2, max_population=50)
+    try:
+        with pytest.raises(_lib.GdockError) as e:
+            srv.submit(rec, lig, pairs, population_size=80)
+        assert e.value.status == _lib.INVALID_ARG
+        with pytest.raises(_lib.GdockError):
+            srv.wait(12345)
+    finally:
+        srv.close()

@@ -113,3 +113,28 @@ def test_random_genes_are_reference_euler_angles():
     g = synth.random_genes(1, 256)
     assert np.all((g[:, :3] >= 0) & (g[:, :3] < 2 * np.pi))
     assert np.all(np.linalg.norm(g[:, 3:], axis=1) <= 20.0)
+
+
+def test_ga_random_stream_matches_oracle():
+    """The product GA's random stream (gdock_rng_stream) equals the oracle's
+    restatement for several seeds; the stream's first outputs for seed 0 are
+    pinned to the published SplitMix64 / xoshiro256** definitions."""
+    from gdock_amd import ga
+    for seed in (0, 1, 42, 2**63 + 5):
+        assert np.array_equal(ga.rng_stream(seed, 1000), orc.rng_stream(seed, 1000))
+    # independent restatement in Python integers of the two published algorithms
+    M = (1 << 64) - 1
+    x, s = 0, []
+    for _ in range(4):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        s.append(z ^ (z >> 31))
+    rotl = lambda v, k: ((v << k) | (v >> (64 - k))) & M
+    exp = []
+    for _ in range(8):
+        exp.append((rotl((s[1] * 5) & M, 7) * 9) & M)
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45)
+    assert ga.rng_stream(0, 8).tolist() == exp
