@@ -90,6 +90,28 @@ bool finite_atoms(const gdock_atom* a, int32_t n) {
   return true;
 }
 
+// Spatial order by recursive bisection: idx[lo, hi) is split across its widest
+// extent at a multiple of `block`, so every aligned run of `block` entries of
+// the result is a compact cluster of atoms.  Deterministic (stable sorts).
+void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo, size_t hi, size_t block) {
+  if (hi - lo <= block) return;
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (size_t i = lo; i < hi; ++i) {
+    const gdock_atom& a = atoms[idx[i]];
+    const double p[3] = {a.x, a.y, a.z};
+    for (int d = 0; d < 3; ++d) { mn[d] = std::min(mn[d], p[d]); mx[d] = std::max(mx[d], p[d]); }
+  }
+  int ax = 0;
+  for (int d = 1; d < 3; ++d)
+    if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+  auto coord = [&](int32_t i) { return ax == 0 ? atoms[i].x : (ax == 1 ? atoms[i].y : atoms[i].z); };
+  std::stable_sort(idx.begin() + lo, idx.begin() + hi, [&](int32_t a, int32_t b) { return coord(a) < coord(b); });
+  const size_t n = hi - lo;
+  const size_t k = std::max(block, (n / block / 2) * block);
+  bisect_order(atoms, idx, lo, lo + k, block);
+  bisect_order(atoms, idx, lo + k, hi, block);
+}
+
 }  // namespace gdock
 
 extern "C" {
@@ -184,6 +206,9 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   for (auto& kv : cls_of) kv.second = nc++;
   std::vector<std::vector<int32_t>> members(nc);
   for (int32_t i = 0; i < n_rec; ++i) members[cls_of[{cx->rec[i].rmin2, cx->rec[i].epsilon}]].push_back(i);
+  // within a class, groups of 4 consecutive atoms are spatially compact (their
+  // bounding spheres let a wave skip groups beyond every cutoff)
+  for (auto& mem : members) bisect_order(cx->rec.data(), mem, 0, mem.size(), kRecGroup);
   std::vector<int32_t> class_start(nc + 1, 0), perm;  // perm[j] = receptor atom or -1 (padding)
   std::vector<double> class_rmin(nc), class_eps(nc);
   for (auto& kv : cls_of) { class_rmin[kv.second] = kv.first.first; class_eps[kv.second] = kv.first.second; }
@@ -197,8 +222,14 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   double origin[3] = {0, 0, 0}, lcom[3] = {0, 0, 0};
   if (n_rec) com_of(rec, n_rec, origin);
   if (n_lig) com_of(lig, n_lig, lcom);
-  std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;
-  for (int32_t i = 0; i < n_lig; ++i) lig_by_serial[cx->lig[i].serial].push_back(i);
+  // Ligand device order: blocks of kLigBlock compact atoms (a wave's slots).
+  std::vector<int32_t> lig_perm(n_lig);  // device index -> input index
+  for (int32_t i = 0; i < n_lig; ++i) lig_perm[i] = i;
+  bisect_order(cx->lig.data(), lig_perm, 0, lig_perm.size(), kLigBlock);
+  cx->lig_slot.assign(n_lig, 0);
+  for (int32_t q = 0; q < n_lig; ++q) cx->lig_slot[lig_perm[q]] = q;
+  std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;  // serial -> device indices
+  for (int32_t q = 0; q < n_lig; ++q) lig_by_serial[cx->lig[lig_perm[q]].serial].push_back(q);
 
   std::vector<float4> xyzq(n_pad);
   std::vector<double> rx(n_pad), ry(n_pad), rz(n_pad), rasp(n_pad, 0.0), rrad(n_pad, 0.0);
@@ -226,11 +257,29 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     if (hit != lig_by_serial.end())
       for (int32_t l : hit->second) coll.push_back(make_int4(j, l, cls, 0));
   }
+  // bounding spheres of the receptor groups, from the FP32 coordinates the kernel uses
+  std::vector<float4> rgroup(n_pad / kRecGroup);
+  for (int32_t g = 0; g < n_pad / kRecGroup; ++g) {
+    double c[3] = {0, 0, 0};
+    int nreal = 0;
+    for (int32_t j = g * kRecGroup; j < (g + 1) * kRecGroup; ++j)
+      if (perm[j] >= 0) { c[0] += xyzq[j].x; c[1] += xyzq[j].y; c[2] += xyzq[j].z; ++nreal; }
+    const float cf[3] = {static_cast<float>(c[0] / nreal), static_cast<float>(c[1] / nreal),
+                         static_cast<float>(c[2] / nreal)};
+    double r = 0.0;
+    for (int32_t j = g * kRecGroup; j < (g + 1) * kRecGroup; ++j)
+      if (perm[j] >= 0) {
+        const double dx = static_cast<double>(xyzq[j].x) - cf[0], dy = static_cast<double>(xyzq[j].y) - cf[1],
+                     dz = static_cast<double>(xyzq[j].z) - cf[2];
+        r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+      }
+    rgroup[g] = make_float4(cf[0], cf[1], cf[2], std::nextafter(static_cast<float>(r * (1.0 + 1e-6) + 1e-4), 1e30f));
+  }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);
   double lig_r = 0.0;
-  for (int32_t i = 0; i < n_lig; ++i) {
-    const gdock_atom& a = cx->lig[i];
+  for (int32_t i = 0; i < n_lig; ++i) {  // device index i holds input atom lig_perm[i]
+    const gdock_atom& a = cx->lig[lig_perm[i]];
     lx[i] = a.x; ly[i] = a.y; lz[i] = a.z;
     lrm[i] = a.rmin2; leps[i] = a.epsilon; lq[i] = a.charge;
     lasp[i] = asp_of(a); lrad[i] = a.vdw_radius; lser[i] = a.serial;
@@ -251,14 +300,15 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
       if (field(lig[i].name, sizeof lig[i].name) == "CA" && lig[i].resseq == pairs[2 * p + 1]) li = i;
     if (ri < 0 || li < 0) continue;  // dropped with a warning in the reference
     rr_xyz.insert(rr_xyz.end(), {rec[ri].x, rec[ri].y, rec[ri].z});
-    rl_idx.push_back(li);
+    rl_idx.push_back(cx->lig_slot[li]);
   }
   cx->n_restr = static_cast<int32_t>(rl_idx.size());
 
   Layout L;
   const size_t o_xyzq = L.add<float4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
                o_rz = L.add<double>(n_pad), o_rasp = L.add<double>(n_pad), o_rrad = L.add<double>(n_pad),
-               o_meta = L.add<int2>(n_pad), o_coll = L.add<int4>(coll.size()), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
+               o_meta = L.add<int2>(n_pad), o_coll = L.add<int4>(coll.size()),
+               o_rgrp = L.add<float4>(rgroup.size()), o_lorig = L.add<int32_t>(n_lig), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
                o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
@@ -274,6 +324,8 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_rasp, rasp.data(), n_pad * 8); put(o_rrad, rrad.data(), n_pad * 8);
   put(o_meta, meta.data(), n_pad * sizeof(int2));
   put(o_coll, coll.data(), coll.size() * sizeof(int4));
+  put(o_rgrp, rgroup.data(), rgroup.size() * sizeof(float4));
+  put(o_lorig, lig_perm.data(), lig_perm.size() * 4);
   put(o_cs, class_start.data(), (nc + 1) * 4); put(o_crm, class_rmin.data(), nc * 8); put(o_ceps, class_eps.data(), nc * 8);
   put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
   put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
@@ -295,6 +347,8 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.rec_radius = reinterpret_cast<const double*>(m + o_rrad);
   d.rec_meta = reinterpret_cast<const int2*>(m + o_meta);
   d.coll_pairs = reinterpret_cast<const int4*>(m + o_coll);
+  d.rec_group = reinterpret_cast<const float4*>(m + o_rgrp);
+  d.lig_orig = reinterpret_cast<const int32_t*>(m + o_lorig);
   d.class_start = reinterpret_cast<const int32_t*>(m + o_cs);
   d.class_rmin2 = reinterpret_cast<const double*>(m + o_crm);
   d.class_eps = reinterpret_cast<const double*>(m + o_ceps);

@@ -29,6 +29,10 @@ constexpr int kMaxLdsReceptor = 16000;  // receptor burial counters kept in LDS
 constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away ...
 constexpr float kPadRecCoord = -1.0e4f;  // ... on the other side from padding receptor atoms
 constexpr int kRecGroup = 4;             // receptor atoms per scalar load
+constexpr int kLigBlock = 64;            // ligand atoms per spatial block (one wave's slot)
+// A receptor group and a wave's ligand atoms farther apart than this have no
+// pair within any cutoff (15 A) or its guard band.
+constexpr float kCullReach = 15.01f;
 
 // Device view of one complex (kernel argument, plain pointers).
 struct DevComplex {
@@ -41,10 +45,11 @@ struct DevComplex {
   const double* rec_radius;
   const int2* rec_meta;        // {serial, 1 if the serial also occurs in the ligand}
   const int4* coll_pairs;      // {receptor slot, ligand atom, receptor class, 0}: equal serials
+  const float4* rec_group;     // per kRecGroup slots: bounding sphere centre (FP32, origin frame), radius
   const int32_t* class_start;  // n_class + 1
   const double* class_rmin2;
   const double* class_eps;
-  // ligand (positioned), original order
+  // ligand (positioned), in spatial block order (lig_orig: index in the input)
   const double* lig_x;
   const double* lig_y;
   const double* lig_z;
@@ -54,6 +59,7 @@ struct DevComplex {
   const double* lig_asp;
   const double* lig_radius;
   const int32_t* lig_serial;
+  const int32_t* lig_orig;
   // restraints: receptor CA coordinates (FP64) and ligand CA index
   const double* restr_rec_xyz;  // n_restr x 3
   const int32_t* restr_lig;     // n_restr

@@ -67,8 +67,10 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
     const double n = static_cast<double>(cx->rec.size());
     origin[0] = sx / n; origin[1] = sy / n; origin[2] = sz / n;
   }
+  std::vector<float4> rsph;
   for (auto& kv : rres) {
     cx->rec_res_key.push_back(kv.first);
+    const size_t first = rxyz.size();
     for (int32_t i : kv.second) {
       const gdock_atom& a = cx->rec[i];
       rxyz.push_back(make_float4(static_cast<float>(a.x - origin[0]), static_cast<float>(a.y - origin[1]),
@@ -76,13 +78,28 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
       rx.push_back(a.x); ry.push_back(a.y); rz.push_back(a.z);
     }
     rstart.push_back(static_cast<int32_t>(rxyz.size()));
+    double c[3] = {0, 0, 0};
+    for (size_t j = first; j < rxyz.size(); ++j) { c[0] += rxyz[j].x; c[1] += rxyz[j].y; c[2] += rxyz[j].z; }
+    const double nres = static_cast<double>(rxyz.size() - first);
+    const float cf[3] = {static_cast<float>(c[0] / nres), static_cast<float>(c[1] / nres), static_cast<float>(c[2] / nres)};
+    double r = 0.0;
+    for (size_t j = first; j < rxyz.size(); ++j) {
+      const double dx = static_cast<double>(rxyz[j].x) - cf[0], dy = static_cast<double>(rxyz[j].y) - cf[1],
+                   dz = static_cast<double>(rxyz[j].z) - cf[2];
+      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    rsph.push_back(make_float4(cf[0], cf[1], cf[2], std::nextafter(static_cast<float>(r * (1.0 + 1e-6) + 1e-4), 1e30f)));
   }
+  // ligand heavy atoms in blocks of kLigBlock spatially compact atoms (a wave's slots)
+  std::vector<int32_t> lheavy, res_of(cx->lig.size(), -1);
   int32_t li = 0;
   for (auto& kv : lres) {
     cx->lig_res_key.push_back(kv.first);
-    for (int32_t i : kv.second) { latom.push_back(i); lresi.push_back(li); }
+    for (int32_t i : kv.second) { lheavy.push_back(i); res_of[i] = li; }
     ++li;
   }
+  bisect_order(cx->lig.data(), lheavy, 0, lheavy.size(), kLigBlock);
+  for (int32_t i : lheavy) { latom.push_back(cx->lig_slot[i]); lresi.push_back(res_of[i]); }
   DevContacts& c = cx->con;
   c.n_rec_heavy = static_cast<int32_t>(rxyz.size());
   c.n_lig_heavy = static_cast<int32_t>(latom.size());
@@ -99,7 +116,8 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
   const size_t begin = cx->contacts_layout_begin;
   const size_t o1 = L->add<float4>(rxyz.size()), o2 = L->add<double>(rx.size()), o3 = L->add<double>(ry.size()),
                o4 = L->add<double>(rz.size()), o5 = L->add<int32_t>(rstart.size()),
-               o6 = L->add<int32_t>(latom.size()), o7 = L->add<int32_t>(lresi.size());
+               o6 = L->add<int32_t>(latom.size()), o7 = L->add<int32_t>(lresi.size()),
+               o8 = L->add<float4>(rsph.size());
   host->assign(L->size - begin, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) {
     if (bytes) std::memcpy(host->data() + (off - begin), src, bytes);
@@ -107,7 +125,8 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
   put(o1, rxyz.data(), rxyz.size() * sizeof(float4));
   put(o2, rx.data(), rx.size() * 8); put(o3, ry.data(), ry.size() * 8); put(o4, rz.data(), rz.size() * 8);
   put(o5, rstart.data(), rstart.size() * 4); put(o6, latom.data(), latom.size() * 4); put(o7, lresi.data(), lresi.size() * 4);
-  cx->con_offsets = {o1, o2, o3, o4, o5, o6, o7};
+  put(o8, rsph.data(), rsph.size() * sizeof(float4));
+  cx->con_offsets = {o1, o2, o3, o4, o5, o6, o7, o8};
 }
 
 void bind_contact_tables(gdock_complex* cx) {
@@ -122,6 +141,7 @@ void bind_contact_tables(gdock_complex* cx) {
   c.rec_res_start = reinterpret_cast<const int32_t*>(m + o[4]);
   c.lig_atom = reinterpret_cast<const int32_t*>(m + o[5]);
   c.lig_res = reinterpret_cast<const int32_t*>(m + o[6]);
+  c.rec_res_sphere = reinterpret_cast<const float4*>(m + o[7]);
 }
 
 void release_fcc_scratch(gdock_ctx* ctx) { ctx->fcc.release(); }
@@ -172,9 +192,13 @@ __device__ __forceinline__ void band(double T, double X, float& lo, float& hi) {
 }
 
 // ---------------------------------------------------------------- K6 contacts
+// Wave w holds ligand heavy atoms base + w * 64 K + k * 64 + lane (K blocks
+// of kLigBlock spatially compact atoms).  A receptor residue whose bounding
+// sphere lies farther than `reach` (the cutoff plus a margin well above the
+// FP32 error) from the box of the wave's atoms has no contact with them.
 template <int K>
 __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, const DevContacts ct,
-                                                       const double* __restrict__ poses, double d2_incl,
+                                                       const double* __restrict__ poses, double d2_incl, float reach,
                                                        unsigned long long* __restrict__ bits_out,
                                                        int32_t* __restrict__ sizes) {
   extern __shared__ uint32_t s_bits[];
@@ -193,7 +217,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
     int32_t lres[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int h = base + tid + 256 * k;
+      const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
       if (h < ct.n_lig_heavy) {
         double px, py, pz;
         pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
@@ -206,7 +230,26 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
         lres[k] = 0;
       }
     }
+    float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (base + wave * (kLigBlock * K) + k * kLigBlock + lane < ct.n_lig_heavy) {
+        x0 = fminf(x0, lx[k]); y0 = fminf(y0, ly[k]); z0 = fminf(z0, lz[k]);
+        x1 = fmaxf(x1, lx[k]); y1 = fmaxf(y1, ly[k]); z1 = fmaxf(z1, lz[k]);
+      }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      x0 = fminf(x0, __shfl_xor(x0, off)); y0 = fminf(y0, __shfl_xor(y0, off)); z0 = fminf(z0, __shfl_xor(z0, off));
+      x1 = fmaxf(x1, __shfl_xor(x1, off)); y1 = fmaxf(y1, __shfl_xor(y1, off)); z1 = fmaxf(z1, __shfl_xor(z1, off));
+    }
+    x0 = uniform_f(x0); y0 = uniform_f(y0); z0 = uniform_f(z0);
+    x1 = uniform_f(x1); y1 = uniform_f(y1); z1 = uniform_f(z1);
     for (int ri = 0; ri < ct.n_rec_res; ++ri) {
+      const float4 S = ct.rec_res_sphere[ri];
+      const float ex = fmaxf(fmaxf(x0 - S.x, S.x - x1), 0.0f), ey = fmaxf(fmaxf(y0 - S.y, S.y - y1), 0.0f),
+                  ez = fmaxf(fmaxf(z0 - S.z, S.z - z1), 0.0f);
+      const float rr = S.w + reach;
+      if (ex * ex + ey * ey + ez * ez > rr * rr) continue;
       bool hit[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) hit[k] = false;
@@ -220,7 +263,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
           const bool a = d2 < hi, b = d2 <= lo;
           bool m = a;
           if (__any(a != b)) {
-            const int h = base + tid + 256 * k;
+            const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
             if (a != b && h < ct.n_lig_heavy) {  // clustering.rs:98-99 in FP64
               double px, py, pz;
               pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
@@ -256,13 +299,16 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
 }
 
 // ---------------------------------------------------------------- K7 overlap
-// OR of all poses' words (32-bit view): union[w].
+// OR of all poses' words (32-bit view) into uni[w] (zeroed beforehand); each
+// workgroup row folds a slice of kUnionPoses poses, coalesced across words.
+constexpr int kUnionPoses = 64;
 __global__ void union_kernel(const uint32_t* __restrict__ bits, int n, int words32, uint32_t* __restrict__ uni) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= words32) return;
+  const int p0 = blockIdx.y * kUnionPoses, p1 = min(n, p0 + kUnionPoses);
   uint32_t u = 0;
-  for (int p = 0; p < n; ++p) u |= bits[static_cast<size_t>(p) * words32 + w];
-  uni[w] = u;
+  for (int p = p0; p < p1; ++p) u |= bits[static_cast<size_t>(p) * words32 + w];
+  if (u) atomicOr(&uni[w], u);
 }
 
 // Word-major compacted copy: T[k * n_pad + p] = bits[p][col[k]].
@@ -436,8 +482,9 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
 template <int K>
 hipError_t launch_contacts_k(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
                              unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
+  const float reach = static_cast<float>(std::sqrt(std::max(d2_incl, 0.0))) + 0.01f;
   hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), static_cast<size_t>(cx->con.words64) * 8, s, cx->dev,
-                     cx->con, d_poses, d2_incl, d_bits, d_sizes);
+                     cx->con, d_poses, d2_incl, reach, d_bits, d_sizes);
   return hipGetLastError();
 }
 
@@ -571,8 +618,10 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   e = launch_contacts(cx, static_cast<double*>(P(o_p)), n, contact_d2_threshold(cfg->contact_distance),
                       static_cast<unsigned long long*>(P(o_b)), static_cast<int32_t*>(P(o_s)), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: contacts");
-  hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256), dim3(256), 0, ctx->stream,
-                     static_cast<const uint32_t*>(P(o_b)), n, W32, static_cast<uint32_t*>(P(o_u)));
+  e = hipMemsetAsync(P(o_u), 0, W32 * 4, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
+  hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256, (n + kUnionPoses - 1) / kUnionPoses), dim3(256), 0,
+                     ctx->stream, static_cast<const uint32_t*>(P(o_b)), n, W32, static_cast<uint32_t*>(P(o_u)));
   std::vector<uint32_t> uni(W32);
   e = hipMemcpyAsync(uni.data(), P(o_u), W32 * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

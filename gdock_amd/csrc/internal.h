@@ -28,6 +28,7 @@ struct DevContacts {
   const int32_t* rec_res_start;  // n_rec_res + 1
   const int32_t* lig_atom;    // ligand heavy atom -> ligand atom index
   const int32_t* lig_res;     // ligand heavy atom -> ligand residue index
+  const float4* rec_res_sphere;  // per receptor residue: bounding sphere of its heavy atoms (FP32, origin frame)
   int32_t n_rec_heavy, n_lig_heavy, n_rec_res, n_lig_res;
   int32_t words64;            // uint64 words per pose bitset (n_rec_res * n_lig_res bits)
   int32_t supported;          // 0: topology outside the device contact path
@@ -53,6 +54,7 @@ struct gdock_complex {
   void* mem = nullptr;
   size_t mem_bytes = 0;
   std::vector<gdock_atom> rec, lig;
+  std::vector<int32_t> lig_slot;  // original ligand atom -> its index in the device arrays
   int32_t n_restr = 0;
   // contact tables (host)
   size_t contacts_layout_begin = 0;
@@ -91,6 +93,10 @@ struct Layout {
     return off;
   }
 };
+// Orders idx[lo, hi) (atom indices) so that aligned runs of `block` entries are
+// spatially compact (recursive bisection across the widest extent).
+void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo, size_t hi, size_t block);
+
 // Ligand atoms per lane for a complex, measured once per complex shape.
 int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err);
 

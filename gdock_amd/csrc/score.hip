@@ -63,8 +63,8 @@ __device__ __forceinline__ PoseCoef load_pose(const double* __restrict__ p) {
 // -ffp-contract=off, so every product and sum rounds as in the reference.
 __device__ __forceinline__ void ligand_atom(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords, int l,
                                             double& X, double& Y, double& Z) {
-  if (coords) {  // this pose's coordinates, n_lig x 3
-    const double* p = coords + static_cast<size_t>(l) * 3;
+  if (coords) {  // this pose's coordinates, n_lig x 3 in input order
+    const double* p = coords + static_cast<size_t>(cx.lig_orig[l]) * 3;
     X = p[0]; Y = p[1]; Z = p[2];
     return;
   }
@@ -106,6 +106,50 @@ __device__ __forceinline__ void guard_band(double T, double X, float& lo, float&
 // then re-decides those pairs in FP64 exactly as the reference does and adds
 // the difference to what the fast path accumulated.
 struct Bands { float lo1, hi1, lo64, hi64, lo144, hi144, lo225, hi225; };
+
+// Ligand atom held in slot k of this lane for the chunk starting at `base`:
+// wave w holds the K consecutive blocks of kLigBlock atoms (each block is
+// spatially compact) at base + w * kLigBlock * K.
+template <int K>
+__device__ __forceinline__ int lig_index(int base, int k) {
+  return base + static_cast<int>(threadIdx.x >> 6) * (kLigBlock * K) + k * kLigBlock +
+         static_cast<int>(threadIdx.x & 63);
+}
+
+// Axis-aligned box of a wave's ligand atoms (FP32, origin frame; empty when
+// the wave holds none) and the test that a receptor group's bounding sphere
+// is farther from it than every cutoff.
+struct WaveBox { float x0, y0, z0, x1, y1, z1; };
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+  return uniform(v);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return uniform(v);
+}
+
+__device__ __forceinline__ bool beyond_reach(const float4 S, const WaveBox& b) {
+  const float dx = fmaxf(fmaxf(b.x0 - S.x, S.x - b.x1), 0.0f);
+  const float dy = fmaxf(fmaxf(b.y0 - S.y, S.y - b.y1), 0.0f);
+  const float dz = fmaxf(fmaxf(b.z0 - S.z, S.z - b.z1), 0.0f);
+  const float reach = S.w + kCullReach;
+  return dx * dx + dy * dy + dz * dz > reach * reach;
+}
+
+// Bit i set: receptor group g0 / kRecGroup + i (i < 16) has atoms within
+// reach of the wave's ligand atoms.  Lanes 0-15 test one group each.
+__device__ __forceinline__ uint32_t groups_in_reach(CxRef cx, int g0, const WaveBox& b) {
+  const int lane = threadIdx.x & 63;
+  const int g = g0 / kRecGroup + lane;
+  bool near = false;
+  if (lane < 64 / kRecGroup && g < cx.n_rec / kRecGroup) near = !beyond_reach(cx.rec_group[g], b);
+  return static_cast<uint32_t>(__builtin_amdgcn_ballot_w64(near));
+}
 
 template <int K>
 struct LaneAtoms {
@@ -205,7 +249,7 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
-    const int l = base + threadIdx.x + kScoreThreads * k;
+    const int l = lig_index<K>(base, k);
     const uint64_t am = __builtin_amdgcn_ballot_w64(in_band(d2, B) && l < cx.n_lig);
     if (am == 0ull) continue;
     if (!EXACT) {  // settled once per ligand chunk
@@ -262,7 +306,7 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
     while (cstart[c + 1] <= E.j) ++c;  // groups never straddle classes
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {  // one copy of the FP64 path
-      const int l = base + threadIdx.x + kScoreThreads * k;
+      const int l = lig_index<K>(base, k);
       if (l >= cx.n_lig) break;
       double px, py, pz;
       ligand_atom(cx, pc, coords, l, px, py, pz);
@@ -350,7 +394,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     LaneAtoms<K> a;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int l = base + tid + kScoreThreads * k;
+      const int l = lig_index<K>(base, k);
       if (l < cx.n_lig) {
         double px, py, pz;
         ligand_atom(cx, pc, coords, l, px, py, pz);
@@ -362,17 +406,33 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       }
       a.eps[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
+    WaveBox box;
+    {
+      float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (lig_index<K>(base, k) < cx.n_lig) {
+          x0 = fminf(x0, a.x[k]); y0 = fminf(y0, a.y[k]); z0 = fminf(z0, a.z[k]);
+          x1 = fmaxf(x1, a.x[k]); y1 = fmaxf(y1, a.y[k]); z1 = fmaxf(z1, a.z[k]);
+        }
+      box = WaveBox{wave_min(x0), wave_min(y0), wave_min(z0), wave_max(x1), wave_max(y1), wave_max(z1)};
+    }
     int nev = 0;
     int cls = -1, cend = 0;
     for (int g0 = 0; g0 < cx.n_rec; g0 += 64) {
       const int g1 = min(g0 + 64, cx.n_rec);
+      // Groups beyond every cutoff from all of this wave's atoms add nothing
+      // and hold no band pair: skipped (the whole tile when none is near).
+      const uint32_t keep = groups_in_reach(cx, g0, box);
+      if (keep == 0u) continue;
       int32_t rc = 0;  // lane t: burial count of receptor slot g0 + t from this wave
       for (int j = g0; j < g1; j += kRecGroup) {
-        if (j == cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
-          do { ++cls; cend = cstart[cls + 1]; } while (cend == j);
+        if (!((keep >> ((j - g0) / kRecGroup)) & 1u)) continue;
+        if (j >= cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
+          do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const int l = base + tid + kScoreThreads * k;
+            const int l = lig_index<K>(base, k);
             if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
           }
         }
@@ -405,7 +465,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane], rc);
 #pragma unroll
       for (int k = 0; k < K; ++k) {  // fold the tile's FP32 partial sums into FP64
-        const int l = base + tid + kScoreThreads * k;
+        const int l = lig_index<K>(base, k);
         if (l < cx.n_lig) {
           sum_v += static_cast<double>(a.acv[k]);
           sum_e += static_cast<double>(a.ace[k]) * (kCoulomb * cx.lig_q[l]);
@@ -417,7 +477,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       settle_events<K>(cx, pose_rec, coords, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int l = base + tid + kScoreThreads * k;
+      const int l = lig_index<K>(base, k);
       if (l < cx.n_lig) {
         const double asp = cx.lig_asp[l];
         if (fabs(asp) >= 0.001) {  // fitness.rs:165-181
