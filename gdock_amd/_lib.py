@@ -141,6 +141,10 @@ def lib() -> C.CDLL:
         "gdock_ga_clusters": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_ga_selected": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_rng_stream": (None, [C.c_uint64, C.c_int32, C.POINTER(C.c_uint64)]),
+        "gdock_evaluator_create": (C.c_int, [P, P, P, C.c_int32, C.POINTER(P)]),
+        "gdock_evaluator_destroy": (None, [P, P]),
+        "gdock_evaluator_info": (C.c_int, [P, I32, I32, I32]),
+        "gdock_calc_metrics": (C.c_int, [P, P, D, C.c_int32, D]),
         "gdock_server_default_config": (C.c_int, [C.POINTER(ServerConfig)]),
         "gdock_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(P)]),
         "gdock_server_destroy": (None, [P]),

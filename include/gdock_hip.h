@@ -203,6 +203,23 @@ int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t*
 /* The first n outputs of the GA's random stream for a seed. */
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
 
+/* ---- DockQ evaluator --------------------------------------------------------- */
+/* Evaluator::new (src/evaluator.rs:50-66) for the complex's receptor and a
+ * reference (native) ligand as read: interface residues within 5 A, the
+ * interface atoms and the native contacts.  Models are the complex's
+ * positioned ligand moved by genes, as Population::eval_metrics
+ * (src/population.rs:63-71) builds them; the reference must list the ligand's
+ * atoms in the same order (the reference zips them). */
+typedef struct gdock_evaluator gdock_evaluator;
+gdock_status gdock_evaluator_create(gdock_ctx* ctx, gdock_complex* cx, const gdock_atom* reference, int32_t n_ref,
+                                    gdock_evaluator** out);
+void gdock_evaluator_destroy(gdock_ctx* ctx, gdock_evaluator* ev);
+gdock_status gdock_evaluator_info(const gdock_evaluator* ev, int32_t* n_receptor_interface_res,
+                                  int32_t* n_reference_interface_res, int32_t* n_native_contacts);
+/* Evaluator::calc_metrics (src/evaluator.rs:68-145) for n poses: metrics is
+ * n x 4 = {rmsd (L-RMSD), irmsd, fnat, dockq}. */
+gdock_status gdock_calc_metrics(gdock_ctx* ctx, gdock_evaluator* ev, const double* genes, int32_t n, double* metrics);
+
 /* ---- docking server (continuous batching) -------------------------------- */
 /* Many commands::run jobs (src/commands/run.rs:227-733) served concurrently
  * on one GPU.  gdock_server_submit queues a job (receptor and raw ligand

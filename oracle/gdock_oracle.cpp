@@ -818,6 +818,77 @@ void to_records(const Molecule& m, RecordAtom* r) {
 }  // namespace
 
 // =================================================================== C ABI
+// ---------------------------------------------------------------- evaluator
+// evaluator.rs: DockQ against a reference ligand.  Contacts are atom pairs
+// keyed by (chain, resseq, atom name) on both sides, kept as a Vec in
+// receptor-major order; RMSDs sum distances (not squared distances), as the
+// reference does (evaluator.rs:88-132).
+struct EvContact {
+  char ci; int16_t ri; std::string ai; char cj; int16_t rj; std::string aj;
+  bool operator==(const EvContact& o) const {
+    return ci == o.ci && ri == o.ri && ai == o.ai && cj == o.cj && rj == o.rj && aj == o.aj;
+  }
+};
+
+// evaluator.rs:147-168
+std::vector<EvContact> ev_contacts(const Molecule& r, const Molecule& l) {
+  std::vector<EvContact> out;
+  for (const Atom& a : r)
+    for (const Atom& b : l)
+      if (distance(a, b) <= 5.0) out.push_back({a.chainid, a.resseq, a.name, b.chainid, b.resseq, b.name});
+  return out;
+}
+
+// evaluator.rs:170-187
+void ev_interface(const Molecule& r, const Molecule& ref, std::set<int16_t>* ir, std::set<int16_t>* il) {
+  for (const Atom& a : r)
+    for (const Atom& b : ref)
+      if (distance(a, b) <= 5.0) { ir->insert(a.resseq); il->insert(b.resseq); }
+}
+
+// structure.rs:319-327
+Molecule filter_by_resseq(const Molecule& m, const std::set<int16_t>& keep) {
+  Molecule out;
+  for (const Atom& a : m) if (keep.count(a.resseq)) out.push_back(a);
+  return out;
+}
+
+struct OracleEvaluator {  // evaluator.rs:50-66
+  Molecule reference, receptor_interface, reference_interface;
+  std::set<int16_t> irec, iref;
+  std::vector<EvContact> native;
+  OracleEvaluator(const Molecule& receptor, const Molecule& ref) : reference(ref) {
+    ev_interface(receptor, ref, &irec, &iref);
+    receptor_interface = filter_by_resseq(receptor, irec);
+    reference_interface = filter_by_resseq(ref, iref);
+    native = ev_contacts(receptor_interface, reference_interface);
+  }
+  static double zip_rmsd(const Molecule& a, const Molecule& b, size_t n) {  // evaluator.rs:88-132
+    if (n == 0) return 0.0;
+    double sum = 0.0;
+    for (size_t i = 0; i < a.size() && i < b.size(); ++i) sum += distance(a[i], b[i]);
+    return std::sqrt(sum / static_cast<double>(n));
+  }
+  double rmsd(const Molecule& lig) const { return zip_rmsd(lig, reference, reference.size()); }
+  double irmsd(const Molecule& lig) const {
+    return zip_rmsd(filter_by_resseq(lig, iref), reference_interface, reference_interface.size());
+  }
+  double fnat(const Molecule& lig) const {  // evaluator.rs:134-145
+    const auto docked = ev_contacts(receptor_interface, filter_by_resseq(lig, iref));
+    size_t common = 0;
+    for (const auto& c : docked)
+      if (std::find(native.begin(), native.end(), c) != native.end()) ++common;
+    if (native.empty()) return 0.0;
+    return static_cast<double>(common) / static_cast<double>(native.size());
+  }
+  void metrics(const Molecule& lig, double* out) const {  // evaluator.rs:68-87
+    const double f = fnat(lig), ir = irmsd(lig), r = rmsd(lig);
+    const double irs = 1.0 / (1.0 + powi2(ir / 1.5));
+    const double lrs = 1.0 / (1.0 + powi2(r / 8.5));
+    out[0] = r; out[1] = ir; out[2] = f; out[3] = (f + irs + lrs) / 3.0;
+  }
+};
+
 extern "C" {
 
 // -- molecules ------------------------------------------------------------
@@ -1047,6 +1118,29 @@ void or_evolve(const double* genes, const double* fitness, int n, uint64_t seed,
   Rng rng(seed);
   auto np = evolve(pop, rng);
   for (size_t i = 0; i < np.size(); ++i) { std::memcpy(out_genes + 6 * i, np[i].genes, 6 * sizeof(double)); out_fitness[i] = np[i].fitness; }
+}
+
+// Evaluator (evaluator.rs): receptor and reference ligand as given.
+void* or_evaluator_new(const RecordAtom* r, int nr, const RecordAtom* ref, int nref) {
+  return new OracleEvaluator(from_records(r, nr), from_records(ref, nref));
+}
+void or_evaluator_free(void* h) { delete static_cast<OracleEvaluator*>(h); }
+void or_evaluator_info(void* h, int* n_irec, int* n_iref, int* n_native, int16_t* irec, int16_t* iref) {
+  auto* e = static_cast<OracleEvaluator*>(h);
+  *n_irec = static_cast<int>(e->irec.size());
+  *n_iref = static_cast<int>(e->iref.size());
+  *n_native = static_cast<int>(e->native.size());
+  if (irec) { int k = 0; for (int16_t v : e->irec) irec[k++] = v; }
+  if (iref) { int k = 0; for (int16_t v : e->iref) iref[k++] = v; }
+}
+// out: rmsd, irmsd, fnat, dockq for each of n models (model i: atoms [i*nl, (i+1)*nl))
+void or_calc_metrics(void* h, const RecordAtom* models, int nl, int n, double* out) {
+  auto* e = static_cast<OracleEvaluator*>(h);
+  for (int i = 0; i < n; ++i) e->metrics(from_records(models + static_cast<size_t>(i) * nl, nl), out + 4 * i);
+}
+// evaluator.rs:147-168 contact count between two molecules
+int or_eval_contacts(const RecordAtom* a, int na, const RecordAtom* b, int nb) {
+  return static_cast<int>(ev_contacts(from_records(a, na), from_records(b, nb)).size());
 }
 
 }  // extern "C"

@@ -68,6 +68,11 @@ def lib() -> C.CDLL:
             "or_hof_free": (None, [P]),
             "or_rng_stream": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
             "or_evolve": (None, [D, D, C.c_int, C.c_uint64, D, D]),
+            "or_evaluator_new": (P, [P, C.c_int, P, C.c_int]),
+            "or_evaluator_free": (None, [P]),
+            "or_evaluator_info": (None, [P, I32, I32, I32, C.POINTER(C.c_int16), C.POINTER(C.c_int16)]),
+            "or_calc_metrics": (None, [P, P, C.c_int, C.c_int, D]),
+            "or_eval_contacts": (C.c_int, [P, C.c_int, P, C.c_int]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -284,3 +289,37 @@ class Hof:
 
 def angular_difference(a, b):
     return lib().or_hof_angular_difference(a, b)
+
+
+class Evaluator:
+    """evaluator.rs Evaluator restated: DockQ of models against a reference ligand."""
+
+    def __init__(self, receptor, reference):
+        self.h = lib().or_evaluator_new(_v(receptor), len(receptor), _v(reference), len(reference))
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        lib().or_evaluator_info(self.h, C.byref(a), C.byref(b), C.byref(c), None, None)
+        ir, il = np.zeros(max(a.value, 1), np.int16), np.zeros(max(b.value, 1), np.int16)
+        lib().or_evaluator_info(self.h, C.byref(a), C.byref(b), C.byref(c), _p(ir, C.c_int16), _p(il, C.c_int16))
+        self.receptor_interface = set(ir[:a.value].tolist())
+        self.reference_interface = set(il[:b.value].tolist())
+        self.n_native = c.value
+
+    def calc_metrics(self, models: list) -> np.ndarray:
+        """models: ligand atom arrays of one topology -> (n, 4) rmsd, irmsd, fnat, dockq."""
+        if not models:
+            return np.zeros((0, 4))
+        m = np.ascontiguousarray(np.concatenate(models))
+        out = np.zeros((len(models), 4))
+        lib().or_calc_metrics(self.h, _v(m), len(models[0]), len(models), _p(out))
+        return out
+
+    def __del__(self):
+        try:
+            lib().or_evaluator_free(self.h)
+        except Exception:
+            pass
+
+
+def eval_contacts(a, b) -> int:
+    """evaluator.rs calculate_contacts: number of atom pairs within 5 A."""
+    return lib().or_eval_contacts(_v(a), len(a), _v(b), len(b))

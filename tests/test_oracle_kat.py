@@ -308,3 +308,74 @@ def test_rigid_motion_invariance():
           orc.air_energy(RESTRAINTS_2OOB, r2, l2)]
     assert np.allclose(e0, e1, rtol=1e-9, atol=1e-9)
     assert e0[0] < 0 and e0[3] > 0.0  # 2oob_A/B are given apart: restraints violated
+
+
+# ---- evaluator.rs:319-490 (oracle restatement of the DockQ evaluator) ----------
+def _ev_atom(x, resseq, chain):
+    return A(x, 0.0, 0.0, resseq=resseq, chainid=chain, name="CA", element="C")
+
+
+def test_evaluator_contacts_within_and_beyond():
+    assert orc.eval_contacts(mol(_ev_atom(0, 1, "A")), mol(_ev_atom(3, 10, "B"))) == 1
+    assert orc.eval_contacts(mol(_ev_atom(0, 1, "A")), mol(_ev_atom(10, 10, "B"))) == 0
+    assert orc.eval_contacts(mol(_ev_atom(0, 1, "A")), mol(_ev_atom(5.0, 10, "B"))) == 1  # <= 5.0 inclusive
+
+
+def test_evaluator_interface_and_creation():
+    rec = mol(A(0, 0, 0, resseq=1), A(0, 0, 0, resseq=2, serial=2), A(20, 0, 0, resseq=3, serial=3))
+    ref = mol(_ev_atom(3, 10, "B"))
+    ev = orc.Evaluator(rec, ref)
+    assert ev.receptor_interface == {1, 2} and ev.reference_interface == {10}
+    ev1 = orc.Evaluator(mol(_ev_atom(0, 1, "A")), ref)
+    assert len(ev1.receptor_interface) == 1 and len(ev1.reference_interface) == 1
+
+
+def test_evaluator_rmsd_fnat_dockq_known_answers():
+    rec, ref = mol(_ev_atom(0, 1, "A")), mol(_ev_atom(3, 10, "B"))
+    ev = orc.Evaluator(rec, ref)
+    same = ev.calc_metrics([ref])[0]
+    assert same[0] == 0.0 and same[1] == 0.0 and same[2] == 1.0
+    assert abs(same[3] - 1.0) < 1e-10
+    # one atom 3 A from the reference: sqrt(sum of distances / n) = sqrt(3), evaluator.rs:393-410
+    moved = ev.calc_metrics([mol(_ev_atom(6, 10, "B"))])[0]
+    assert abs(moved[0] - math.sqrt(3.0)) < 1e-10
+    far = ev.calc_metrics([mol(_ev_atom(20, 10, "B"))])[0]
+    assert far[2] == 0.0
+    # DockQ from its closed form (evaluator.rs:73-78)
+    r, ir, f = far[0], far[1], far[2]
+    assert far[3] == (f + 1.0 / (1.0 + (ir / 1.5) ** 2) + 1.0 / (1.0 + (r / 8.5) ** 2)) / 3.0
+
+
+def test_evaluator_fnat_counts_atom_contacts_against_a_set_model():
+    """fnat on a small multi-atom case checked against an independent set model
+    of the reference's Vec semantics (each docked atom pair counts when its
+    (chain, res, name) pair occurs among the native pairs)."""
+    rng = np.random.default_rng(5)
+    names = ["N", "CA", "C", "O", "CB"]
+    rec = mol(*[A(*rng.uniform(-6, 6, 3), resseq=1 + i // 5, name=names[i % 5], serial=i + 1) for i in range(30)])
+    ref = mol(*[A(*(rng.uniform(-6, 6, 3) + [4, 0, 0]), resseq=20 + i // 5, chainid="B", name=names[i % 5],
+                  serial=100 + i) for i in range(25)])
+    ev = orc.Evaluator(rec, ref)
+
+    def keyed_pairs(r, l, iface_r, iface_l):
+        out = []
+        for a in r:
+            if int(a["resseq"]) not in iface_r:
+                continue
+            for b in l:
+                if int(b["resseq"]) not in iface_l:
+                    continue
+                d = math.sqrt((a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2)
+                if d <= 5.0:
+                    out.append((a["chainid"], int(a["resseq"]), a["name"], b["chainid"], int(b["resseq"]), b["name"]))
+        return out
+
+    native = keyed_pairs(rec, ref, ev.receptor_interface, ev.reference_interface)
+    assert len(native) == ev.n_native > 0
+    for k in range(6):
+        model = ref.copy()
+        model["x"] += rng.normal(0, 1.5, len(model))
+        model["y"] += rng.normal(0, 1.5, len(model))
+        docked = keyed_pairs(rec, model, ev.receptor_interface, ev.reference_interface)
+        exp = sum(1 for c in docked if c in set(native)) / len(native)
+        assert ev.calc_metrics([model])[0][2] == exp, k
