@@ -90,6 +90,11 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
     }
     rsph.push_back(make_float4(cf[0], cf[1], cf[2], std::nextafter(static_cast<float>(r * (1.0 + 1e-6) + 1e-4), 1e30f)));
   }
+  // the kernel's form of each receptor atom: -2 r and |r|^2 (FP64 sum of the FP32 squares, rounded once)
+  for (float4& v : rxyz) {
+    const double x = v.x, y = v.y, z = v.z;
+    v = make_float4(-2.0f * v.x, -2.0f * v.y, -2.0f * v.z, static_cast<float>(x * x + y * y + z * z));
+  }
   // ligand heavy atoms in blocks of kLigBlock spatially compact atoms (a wave's slots)
   std::vector<int32_t> lheavy, res_of(cx->lig.size(), -1);
   int32_t li = 0;
@@ -134,7 +139,7 @@ void bind_contact_tables(gdock_complex* cx) {
   char* m = static_cast<char*>(cx->mem);
   const auto& o = cx->con_offsets;
   DevContacts& c = cx->con;
-  c.rec_xyz = reinterpret_cast<const float4*>(m + o[0]);
+  c.rec_c4 = reinterpret_cast<const float4*>(m + o[0]);
   c.rec_x = reinterpret_cast<const double*>(m + o[1]);
   c.rec_y = reinterpret_cast<const double*>(m + o[2]);
   c.rec_z = reinterpret_cast<const double*>(m + o[3]);
@@ -191,6 +196,22 @@ __device__ __forceinline__ void band(double T, double X, float& lo, float& hi) {
   hi = uniform_f(h);
 }
 
+// Guard band for d2 evaluated as |r|^2 + |l|^2 - 2 r.l in FP32 (one add, three
+// FMAs) from coordinates rounded to FP32 in a frame where every component is
+// at most X: |r|^2, |l|^2 <= 3 X^2 and every intermediate is at most 12 X^2, so
+// the roundings of |r|^2 (3 u X^2), |l|^2 (9 u X^2), the add (6 u X^2) and the
+// three FMAs (36 u X^2) stay below 60 u X^2; the coordinate rounding adds at
+// most 4 sqrt(3) u X d.  The band is twice that, rounded outward.
+__device__ __forceinline__ void band_expanded(double T, double X, float& lo, float& hi) {
+  const double u = 5.9604644775390625e-08;
+  const double e = 60.0 * u * X * X + 6.9282032302755092 * u * X * (sqrt(T) + 0.01) + 1e-7;
+  float l = static_cast<float>(T - 2.0 * e), h = static_cast<float>(T + 2.0 * e);
+  if (static_cast<double>(l) > T - 2.0 * e) l = nextafterf(l, -INFINITY);
+  if (static_cast<double>(h) < T + 2.0 * e) h = nextafterf(h, INFINITY);
+  lo = uniform_f(l);
+  hi = uniform_f(h);
+}
+
 // ---------------------------------------------------------------- K6 contacts
 // Wave w holds ligand heavy atoms base + w * 64 K + k * 64 + lane (K blocks
 // of kLigBlock spatially compact atoms).  A receptor residue whose bounding
@@ -209,11 +230,11 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
   const PoseT pc = pose_of(poses + static_cast<size_t>(kPoseDoubles) * pose);
   const double X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
   float lo, hi;
-  band(d2_incl, X, lo, hi);
+  band_expanded(d2_incl, X, lo, hi);
   __syncthreads();
 
   for (int base = 0; base < ct.n_lig_heavy; base += 256 * K) {
-    float lx[K], ly[K], lz[K];
+    float lx[K], ly[K], lz[K], ll[K];
     int32_t lres[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -229,6 +250,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
         lx[k] = ly[k] = lz[k] = kPadCoord;
         lres[k] = 0;
       }
+      ll[k] = __builtin_fmaf(lz[k], lz[k], __builtin_fmaf(ly[k], ly[k], lx[k] * lx[k]));
     }
     float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
 #pragma unroll
@@ -250,28 +272,32 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
                   ez = fmaxf(fmaxf(z0 - S.z, S.z - z1), 0.0f);
       const float rr = S.w + reach;
       if (ex * ex + ey * ey + ez * ez > rr * rr) continue;
+      // closest receptor atom of the residue to each of the lane's atoms
+      float mn[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) mn[k] = INFINITY;
+      const int j0 = ct.rec_res_start[ri], j1 = ct.rec_res_start[ri + 1];
+      for (int j = j0; j < j1; ++j) {
+        const float4 r = ct.rec_c4[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          mn[k] = fminf(mn[k], __builtin_fmaf(r.x, lx[k], __builtin_fmaf(r.y, ly[k], __builtin_fmaf(r.z, lz[k], r.w + ll[k]))));
+      }
       bool hit[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) hit[k] = false;
-      const int j1 = ct.rec_res_start[ri + 1];
-      for (int j = ct.rec_res_start[ri]; j < j1; ++j) {
-        const float4 r = ct.rec_xyz[j];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float dx = r.x - lx[k], dy = r.y - ly[k], dz = r.z - lz[k];
-          const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-          const bool a = d2 < hi, b = d2 <= lo;
-          bool m = a;
-          if (__any(a != b)) {
-            const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
-            if (a != b && h < ct.n_lig_heavy) {  // clustering.rs:98-99 in FP64
-              double px, py, pz;
-              pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
+      for (int k = 0; k < K; ++k) {
+        hit[k] = mn[k] <= lo;
+        const bool amb = !hit[k] && mn[k] < hi;
+        if (__any(amb)) {  // rare: decide this residue in FP64 (clustering.rs:98-99)
+          const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
+          if (amb && h < ct.n_lig_heavy) {
+            double px, py, pz;
+            pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
+            for (int j = j0; j < j1 && !hit[k]; ++j) {
               const double ex = ct.rec_x[j] - px, ey = ct.rec_y[j] - py, ez = ct.rec_z[j] - pz;
-              m = ex * ex + ey * ey + ez * ez <= d2_incl;
+              hit[k] = ex * ex + ey * ey + ez * ez <= d2_incl;
             }
           }
-          hit[k] = hit[k] || m;
         }
       }
 #pragma unroll
@@ -381,12 +407,25 @@ __global__ __launch_bounds__(256) void overlap_kernel(const uint32_t* __restrict
 // element that has a new member as neighbour (rows of AT).  Cluster sizes are
 // non-increasing along the sequence, so the loop stops as soon as no element
 // can reach min_size (count + 1 < min_size).
+// Neighbour count of every row of A (self included), one wave per row.
+__global__ __launch_bounds__(256) void row_count_kernel(const unsigned long long* __restrict__ A, int n, int nwords,
+                                                        int32_t* __restrict__ counts) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  int c = 0;
+  for (int w = lane; w < nwords; w += 64) c += __popcll(A[static_cast<size_t>(row) * nwords + w]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (lane == 0) counts[row] = c;
+}
+
 constexpr int kGreedyThreads = 1024;
 constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count + 1 bit used flag per element
 
 __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned long long* __restrict__ A,
                                                                 const unsigned long long* __restrict__ AT,
-                                                                int n, int nwords, long long min_size,
+                                                                const int32_t* __restrict__ counts, int n, int nwords,
+                                                                long long min_size,
                                                                 int64_t* __restrict__ out_center,
                                                                 int64_t* __restrict__ out_size,
                                                                 int64_t* __restrict__ out_offset,
@@ -399,11 +438,7 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
   __shared__ unsigned long long s_best;
   __shared__ int64_t s_total;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int a = tid; a < n; a += kGreedyThreads) {
-    int c = 0;
-    for (int w = 0; w < nwords; ++w) c += __popcll(A[static_cast<size_t>(a) * nwords + w]);
-    s_count[a] = c;
-  }
+  for (int a = tid; a < n; a += kGreedyThreads) s_count[a] = counts[a];
   for (int w = tid; w < nwords; w += kGreedyThreads) s_used[w] = 0ull;
   __syncthreads();
   int32_t nclu = 0;
@@ -609,7 +644,8 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
                o_t = L.add<uint32_t>(static_cast<size_t>(W32) * n_pad), o_ca = L.add<int32_t>(max_set + 1),
                o_cb = L.add<int32_t>(max_set + 1), o_A = L.add<uint64_t>(static_cast<size_t>(n) * nwords),
                o_AT = L.add<uint64_t>(static_cast<size_t>(n) * nwords), o_cc = L.add<int64_t>(n), o_cs = L.add<int64_t>(n),
-               o_co = L.add<int64_t>(n), o_cm = L.add<int64_t>(n), o_cn = L.add<int32_t>(1);
+               o_co = L.add<int64_t>(n), o_cm = L.add<int64_t>(n), o_cn = L.add<int32_t>(1),
+               o_rc = L.add<int32_t>(n);
   if (!ctx->fcc.reserve(L.size, &e)) return hip_fail(ctx, e, "cluster: scratch");
   char* m = static_cast<char*>(ctx->fcc.ptr);
   auto P = [&](size_t o) { return static_cast<void*>(m + o); };
@@ -645,9 +681,11 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
                      static_cast<const uint32_t*>(P(o_t)), n, n_pad, wc, static_cast<const int32_t*>(P(o_s)),
                      static_cast<const int32_t*>(P(o_ca)), static_cast<const int32_t*>(P(o_cb)), zero_ok,
                      static_cast<unsigned long long*>(P(o_A)), static_cast<unsigned long long*>(P(o_AT)), nwords);
+  hipLaunchKernelGGL(row_count_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream,
+                     static_cast<const unsigned long long*>(P(o_A)), n, nwords, static_cast<int32_t*>(P(o_rc)));
   hipLaunchKernelGGL(greedy_kernel, dim3(1), dim3(kGreedyThreads), static_cast<size_t>(nwords) * 8 + static_cast<size_t>(n) * 4,
                      ctx->stream, static_cast<const unsigned long long*>(P(o_A)),
-                     static_cast<const unsigned long long*>(P(o_AT)), n, nwords,
+                     static_cast<const unsigned long long*>(P(o_AT)), static_cast<const int32_t*>(P(o_rc)), n, nwords,
                      static_cast<long long>(cfg->min_cluster_size), static_cast<int64_t*>(P(o_cc)),
                      static_cast<int64_t*>(P(o_cs)), static_cast<int64_t*>(P(o_co)), static_cast<int64_t*>(P(o_cm)),
                      static_cast<int32_t*>(P(o_cn)));

@@ -21,7 +21,7 @@ struct DeviceArena {
 
 // Device tables of the inter-chain residue contact candidates of a complex.
 struct DevContacts {
-  const float4* rec_xyz;      // receptor heavy atoms grouped by residue (FP32, origin frame)
+  const float4* rec_c4;       // receptor heavy atoms grouped by residue: {-2x, -2y, -2z, x^2+y^2+z^2} of the FP32 origin-frame coordinates
   const double* rec_x;        // FP64 coordinates of the same atoms
   const double* rec_y;
   const double* rec_z;
