@@ -153,7 +153,7 @@ __device__ __forceinline__ uint32_t groups_in_reach(CxRef cx, int g0, const Wave
 
 template <int K>
 struct LaneAtoms {
-  float x[K], y[K], z[K], eps[K], ir6[K], acv[K], ace[K];
+  float x[K], y[K], z[K], eps[K], n2e[K], ir6[K], acv[K], ace[K];  // n2e = -2 eps
   int32_t cnt[K];
 };
 
@@ -162,15 +162,17 @@ __device__ __forceinline__ float pair_d2(float rx, float ry, float rz, float lx,
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// Soft-core LJ shape (rho^2 - 2 rho, rho = R^6 / (d^6 + R^6/2)) and 1/d^2,
-// both from one reciprocal: 1 / (d^2 (d^6/R^6 + 1/2)).
-__device__ __forceinline__ void pair_terms(float d2, float ir6, float& tv, float& id2) {
+// Soft-core LJ (fitness.rs:57-70): the pair's vdw term is eps (rho^2 - 2 rho)
+// = t rho with t = eps rho - 2 eps, rho = R^6 / (d^6 + R^6/2); rho and 1/d^2
+// come from one reciprocal, 1 / (d^2 (d^6/R^6 + 1/2)).
+__device__ __forceinline__ void pair_terms(float d2, float ir6, float eps, float n2e, float& rho, float& t,
+                                           float& id2) {
   const float d6 = (d2 * d2) * d2;
   const float den = __builtin_fmaf(d6, ir6, 0.5f);
   const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, d2, 1.0e-30f));
-  const float rho = rp * d2;
+  rho = rp * d2;
   id2 = rp * den;
-  tv = (rho - 2.0f) * rho;
+  t = __builtin_fmaf(eps, rho, n2e);
 }
 
 __device__ __forceinline__ bool in_band(float d2, const Bands& B) {
@@ -222,9 +224,9 @@ __device__ __forceinline__ int32_t receptor_atom(const RecAtom r, const Bands& B
     ambv = amb ? 1 : ambv;
     const bool mv = mv0;  // pairs of equal serial are taken out afterwards (collision_correction)
     const bool me = me0 && far1;
-    float tv, id2;
-    pair_terms(d2, a.ir6[k], tv, id2);
-    a.acv[k] = __builtin_fmaf(mv ? tv : 0.0f, a.eps[k], a.acv[k]);
+    float rho, t, id2;
+    pair_terms(d2, a.ir6[k], a.eps[k], a.n2e[k], rho, t, id2);
+    a.acv[k] = __builtin_fmaf(mv ? t : 0.0f, rho, a.acv[k]);
     a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
     a.cnt[k] += md ? 1 : 0;
     cj += __popcll(__builtin_amdgcn_ballot_w64(d2 <= B.lo64));
@@ -266,9 +268,9 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
       exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
       const bool f144 = d2 <= B.lo144;
       const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
-      float tv, id2;
-      pair_terms(d2, a.ir6[k], tv, id2);
-      if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? tv : -tv, a.eps[k], a.acv[k]);
+      float rho, t, id2;
+      pair_terms(d2, a.ir6[k], a.eps[k], a.n2e[k], rho, t, id2);
+      if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? t : -t, rho, a.acv[k]);
       if (x225 != f225) a.ace[k] = __builtin_fmaf(x225 ? id2 : -id2, r.w, a.ace[k]);
       const int dc = static_cast<int>(x64) - static_cast<int>(f64);
       if (dc) {
@@ -329,9 +331,9 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
         exact_flags_at(cx, px, py, pz, l, j, rser, x64, x144, x225);
         const bool f144 = d2 <= B.lo144;
         const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
-        float tv, id2;
-        pair_terms(d2, ir6, tv, id2);
-        if (x144 != f144) sum_v += static_cast<double>(x144 ? tv : -tv) * static_cast<double>(eps);
+        float rho, t, id2;
+        pair_terms(d2, ir6, eps, -2.0f * eps, rho, t, id2);
+        if (x144 != f144) sum_v += static_cast<double>(x144 ? t : -t) * static_cast<double>(rho);
         if (x225 != f225)
           sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
         const int dc = static_cast<int>(x64) - static_cast<int>(f64);
@@ -404,7 +406,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       } else {
         a.x[k] = a.y[k] = a.z[k] = kPadCoord;
       }
-      a.eps[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
+      a.eps[k] = 0.0f; a.n2e[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
     WaveBox box;
     {
@@ -433,7 +435,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             const int l = lig_index<K>(base, k);
-            if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
+            if (l < cx.n_lig) {
+              class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
+              a.n2e[k] = -2.0f * a.eps[k];
+            }
           }
         }
         const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
@@ -509,10 +514,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     const float4 r = cx.rec_xyzq[P.x];
     const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
     if (d2 <= B.lo144 && !in_band(d2, B)) {
-      float ir6, eps, tv, id2;
+      float ir6, eps, rho, t, id2;
       class_factors(cx, P.z, P.y, ir6, eps);
-      pair_terms(d2, ir6, tv, id2);
-      sum_v -= static_cast<double>(tv) * static_cast<double>(eps);
+      pair_terms(d2, ir6, eps, -2.0f * eps, rho, t, id2);
+      sum_v -= static_cast<double>(t) * static_cast<double>(rho);
     }
   }
   for (int j = tid; j < cx.n_rec; j += kScoreThreads) {  // fitness.rs:141-162
