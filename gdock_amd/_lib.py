@@ -135,6 +135,7 @@ def lib() -> C.CDLL:
         "gdock_ga_default_params": (C.c_int, [C.POINTER(GaParams)]),
         "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_free": (None, [P]),
+        "gdock_ga_run_dockq": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_info": (C.c_int, [P, I32, I32, I32, I32, I32]),
         "gdock_ga_trace": (C.c_int, [P, D, D, I32]),
         "gdock_ga_hof": (C.c_int32, [P, D, D, I32]),

@@ -322,6 +322,34 @@ gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights
   return GDOCK_OK;
 }
 
+gdock_status gdock_ga_run_dockq(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, uint64_t seed,
+                                const gdock_ga_params* params, gdock_ga_result** out) {
+  if (!ctx || !cx || !ev || !out) {
+    if (ctx) set_error(ctx, "ga_run_dockq: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  gdock_ga_params p;
+  if (params) p = *params; else gdock_ga_default_params(&p);
+  gdock_status st = validate_ga_params(ctx, &p);
+  if (st != GDOCK_OK) return st;
+  auto* r = new gdock_ga_result(seed, p);
+  GaJob& job = r->job;
+  const size_t pop = static_cast<size_t>(job.population());
+  std::vector<double> m(pop * 4), res(pop * 6, 0.0);
+  while (!job.done()) {  // chromosome.rs:97-108: fitness = -DockQ, energy terms zero
+    st = gdock_calc_metrics(ctx, ev, job.genes().data(), job.population(), m.data());
+    if (st != GDOCK_OK) { delete r; return st; }
+    for (size_t i = 0; i < pop; ++i) res[6 * i + 5] = -m[4 * i + 3];
+    job.absorb(res.data());
+  }
+  if (p.cluster) {
+    st = finish_job(ctx, cx, &job, &r->clusters);
+    if (st != GDOCK_OK) { delete r; return st; }
+  }
+  *out = r;
+  return GDOCK_OK;
+}
+
 void gdock_ga_free(gdock_ga_result* r) { delete r; }
 
 gdock_status gdock_ga_info(const gdock_ga_result* r, int32_t* generations, int32_t* converged_at,

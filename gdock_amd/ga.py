@@ -45,6 +45,15 @@ def run(cx: Complex, seed: int = 42, weights: constants.EnergyWeights | None = N
     return take_result(h, p)
 
 
+def run_dockq(cx: Complex, evaluator, seed: int = 42, **params) -> GaResult:
+    """Debug mode (chromosome.rs:97-108): fitness = -DockQ against the evaluator's reference."""
+    p = default_params(**params)
+    h = C.c_void_p()
+    check(lib().gdock_ga_run_dockq(cx.ctx.handle, cx.handle, evaluator.handle, seed, C.byref(p), C.byref(h)),
+          cx.ctx.handle, "gdock_ga_run_dockq")
+    return take_result(h, p)
+
+
 def take_result(h, p: GaParams) -> GaResult:
     """Reads a gdock_ga_result and frees it."""
     L = lib()

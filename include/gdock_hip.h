@@ -191,6 +191,11 @@ gdock_status gdock_ga_default_params(gdock_ga_params* p);
 /* One job, one plain scoring launch per generation (the drop-in path). */
 gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
                           const gdock_ga_params* params, gdock_ga_result** out);
+/* Debug mode (src/chromosome.rs:97-108, `--debug` with a reference): the
+ * fitness is -DockQ from gdock_calc_metrics, the energy terms are zero. */
+typedef struct gdock_evaluator gdock_evaluator; /* see the DockQ evaluator section */
+gdock_status gdock_ga_run_dockq(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, uint64_t seed,
+                                const gdock_ga_params* params, gdock_ga_result** out);
 void gdock_ga_free(gdock_ga_result* r);
 gdock_status gdock_ga_info(const gdock_ga_result* r, int32_t* generations, int32_t* converged_at,
                            int32_t* hof_size, int32_t* n_clusters, int32_t* n_selected);
@@ -210,7 +215,6 @@ void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
  * positioned ligand moved by genes, as Population::eval_metrics
  * (src/population.rs:63-71) builds them; the reference must list the ligand's
  * atoms in the same order (the reference zips them). */
-typedef struct gdock_evaluator gdock_evaluator;
 gdock_status gdock_evaluator_create(gdock_ctx* ctx, gdock_complex* cx, const gdock_atom* reference, int32_t n_ref,
                                     gdock_evaluator** out);
 void gdock_evaluator_destroy(gdock_ctx* ctx, gdock_evaluator* ev);

@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <functional>
 #include <set>
 #include <sstream>
 #include <string>
@@ -730,9 +731,11 @@ void score_population(const Molecule& rec, const Molecule& lig, const std::vecto
 }
 
 // run.rs:361-531 (generation loop) and 665-733 (HoF -> FCC -> model selection)
+// `score` fills every chromosome's fitness (and components); null scores the
+// energies (Population::eval_fitness, normal mode).
 void run_ga(const Molecule& rec, const Molecule& lig_pos, const std::vector<Restraint>& rs,
             const double* w, uint64_t seed, int pop_size, int max_gen, bool cluster, int nthreads,
-            GaTrace* tr) {
+            GaTrace* tr, const std::function<void(std::vector<Chromo>&)>& score = nullptr) {
   Rng rng(seed);
   std::vector<Chromo> pop;
   for (int i = 0; i < pop_size; ++i) pop.push_back(new_chromosome(rng));
@@ -740,7 +743,7 @@ void run_ga(const Molecule& rec, const Molecule& lig_pos, const std::vector<Rest
   double last = 1.7976931348623157e308;  // f64::MAX
   while (gen < max_gen) {
     for (auto& c : pop) tr->genes.insert(tr->genes.end(), c.genes, c.genes + 6);
-    score_population(rec, lig_pos, rs, w, pop, nthreads);
+    if (score) score(pop); else score_population(rec, lig_pos, rs, w, pop, nthreads);
     for (auto& c : pop) tr->fitness.push_back(c.fitness);
     tr->hof.add_from_population(pop, static_cast<uint32_t>(gen));
     size_t bi = 0;
@@ -1063,6 +1066,25 @@ void* or_ga_run(const RecordAtom* r, int nr, const RecordAtom* l_positioned, int
   auto* tr = new GaTrace();
   run_ga(R, L, rs, w, seed, pop_size > 0 ? pop_size : kPopulationSize,
          max_gen > 0 ? max_gen : kMaxGenerations, cluster != 0, nthreads, tr);
+  return tr;
+}
+// Debug mode (chromosome.rs:97-108): fitness = -DockQ against a reference
+// ligand, energy components zero.
+void* or_ga_run_dockq(const RecordAtom* r, int nr, const RecordAtom* l_positioned, int nl, const RecordAtom* ref,
+                      int nref, uint64_t seed, int pop_size, int max_gen, int cluster) {
+  const Molecule R = from_records(r, nr), L = from_records(l_positioned, nl);
+  const OracleEvaluator ev(R, from_records(ref, nref));
+  auto* tr = new GaTrace();
+  const double w[4] = {0.4, 0.05, 3.4, 1.0};
+  run_ga(R, L, {}, w, seed, pop_size > 0 ? pop_size : kPopulationSize, max_gen > 0 ? max_gen : kMaxGenerations,
+         cluster != 0, 1, tr, [&](std::vector<Chromo>& pop) {
+           for (auto& c : pop) {
+             double m[4];
+             ev.metrics(apply_genes(c.genes, L), m);
+             c.vdw = c.elec = c.desolv = c.air = c.penalty = 0.0;
+             c.fitness = -m[3];
+           }
+         });
   return tr;
 }
 void or_ga_free(void* h) { delete static_cast<GaTrace*>(h); }

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "or_cluster_poses": (C.c_int, [P, C.c_int, P, C.c_int, D, C.c_int, D, C.c_int, I64, I64, I64, I32]),
             "or_ga_run": (P, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_ga_free": (None, [P]),
+            "or_ga_run_dockq": (P, [P, C.c_int, P, C.c_int, P, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int]),
             "or_ga_generations": (C.c_int, [P]),
             "or_ga_converged_at": (C.c_int, [P]),
             "or_ga_trace": (None, [P, D, D, I32]),
@@ -218,15 +219,20 @@ def cluster_poses(rec, lig, genes, config=(5.0, 0.60, 0.75, 4), nthreads=8):
 
 
 class GaRun:
-    """Full GA run (run.rs:361-733 restated) with its per-generation trace."""
+    """Full GA run (run.rs:361-733 restated) with its per-generation trace.
+    With `reference`, the debug mode: fitness = -DockQ (chromosome.rs:97-108)."""
 
     def __init__(self, rec, lig_positioned, pairs, weights=(0.4, 0.05, 3.4, 1.0), seed=42, pop_size=150,
-                 max_gen=250, cluster=True, nthreads=8):
+                 max_gen=250, cluster=True, nthreads=8, reference=None):
         p = _pairs(pairs)
         w = np.ascontiguousarray(weights, np.float64)
         L = lib()
-        self.h = L.or_ga_run(_v(rec), len(rec), _v(lig_positioned), len(lig_positioned), _p(p, C.c_int32), len(p),
-                             _p(w), seed, pop_size, max_gen, 1 if cluster else 0, nthreads)
+        if reference is None:
+            self.h = L.or_ga_run(_v(rec), len(rec), _v(lig_positioned), len(lig_positioned), _p(p, C.c_int32),
+                                 len(p), _p(w), seed, pop_size, max_gen, 1 if cluster else 0, nthreads)
+        else:
+            self.h = L.or_ga_run_dockq(_v(rec), len(rec), _v(lig_positioned), len(lig_positioned), _v(reference),
+                                       len(reference), seed, pop_size, max_gen, 1 if cluster else 0)
         self.generations = L.or_ga_generations(self.h)
         self.converged_at = L.or_ga_converged_at(self.h)
         g = self.generations

@@ -64,3 +64,22 @@ def test_synthetic_reference_and_edges():
     far = structure.Molecule(orc.apply_genes(lig.atoms, [0.0, 0.0, 0.0, 80.0, 0.0, 0.0]))
     got, _ = _check(rec, lig, far, genes[:5], "no interface")
     assert (got[:, 2] == 0.0).all() and (got[:, 1] == 0.0).all()
+
+
+def test_ga_debug_mode_dockq_matches_oracle():
+    """run.rs debug mode: the GA minimises -DockQ against the reference.  The
+    device and the oracle GA take the same decisions (DockQ differs only in
+    the last bits from the summation order), so the runs are identical."""
+    from gdock_amd import ga
+    from test_gpu_ga import _compare
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    native = structure.read_pdb_text(read_text("2oob_B.pdb"))[0]
+    lig = structure.position_ligand(rec, native)
+    cx = Complex(rec, lig)
+    ev = Evaluator(cx, native)
+    res = ga.run_dockq(cx, ev, seed=5, keep_trace=1, population_size=60, max_generations=30)
+    ora = orc.GaRun(rec.atoms, lig.atoms, [], seed=5, pop_size=60, max_gen=30, reference=native.atoms)
+    assert np.allclose(res.fitness, ora.fitness[:len(res.fitness)], rtol=1e-12, atol=1e-14)
+    outcome = _compare(res, ora, 60)
+    assert (res.fitness <= 0).all() and res.fitness.min() >= -1.0 - 1e-12
+    print("debug-mode GA vs oracle:", outcome, res.generations, "generations, best DockQ", -res.fitness.min())

@@ -91,3 +91,63 @@ def test_cluster_medium_synthetic():
     assert [(r.center_idx, r.members) for r in got] == ref
     bits, gsz = cx.contacts(genes)
     assert np.array_equal(gsz, sizes)
+
+
+def test_fcc_10k_full_size_properties():
+    """10,000 poses of the 300 x 200-residue complex (the FCC benchmark input).
+    The oracle cannot cluster 10^4 poses in test time, so the full size is
+    checked through properties fixed by the reference's definitions, using
+    the device's own contact bitsets and FP64 divisions on the host:
+    - two runs give identical clusters;
+    - clusters are disjoint, members ascending, the centre a member, sizes
+      non-increasing and at least min_cluster_size (clustering.rs:183-293);
+    - every member j of a cluster with centre i satisfies j in nbr(i):
+      c / |X_i| >= 0.60 and c / |X_j| >= 0.60 * 0.75 (clustering.rs:151-180);
+    - a sample of 24 poses' contact sets equals the oracle's exactly."""
+    cfg = synth.CONFIGS["fcc_10k"]
+    c = synth.make_complex(cfg["seed"], rec_res=cfg["rec_res"], lig_res=cfg["lig_res"], n_restraints=6)
+    rec, lig, cx = _complex(c.receptor_pdb, c.ligand_pdb)
+    genes = synth.random_genes(cfg["seed"], cfg["n_poses"])
+    a = cx.cluster(genes)
+    b = cx.cluster(genes)
+    assert [(r.center_idx, r.members) for r in a] == [(r.center_idx, r.members) for r in b]
+    bits, sizes = cx.contacts(genes)
+    seen = set()
+    last = None
+    fcc_cut, fcc_v_cut = 0.60, 0.60 * 0.75
+    for r in a:
+        assert r.members == sorted(r.members) and r.center_idx in r.members and r.size == len(r.members) >= 4
+        assert not seen.intersection(r.members)
+        seen.update(r.members)
+        assert last is None or r.size <= last
+        last = r.size
+        xi = bits[r.center_idx]
+        for j in r.members:
+            common = int(np.unpackbits((xi & bits[j]).view(np.uint8)).sum())
+            assert common / int(sizes[r.center_idx]) >= fcc_cut and common / int(sizes[j]) >= fcc_v_cut
+    sample = np.linspace(0, len(genes) - 1, 24).astype(int)
+    exp = _oracle_sets(rec, lig, genes[sample])
+    for k, i in enumerate(sample):
+        assert cx.contact_set(bits[i]) == exp[k], i
+
+
+def test_fcc_subset_of_benchmark_input_exact():
+    """300 poses of the FCC benchmark input plus 10 tight families of 10 poses
+    around some of them (so that clusters form): clusters equal the oracle's."""
+    cfg = synth.CONFIGS["fcc_10k"]
+    c = synth.make_complex(cfg["seed"], rec_res=cfg["rec_res"], lig_res=cfg["lig_res"], n_restraints=6)
+    rec, lig, cx = _complex(c.receptor_pdb, c.ligand_pdb)
+    base = synth.random_genes(cfg["seed"], cfg["n_poses"])[:300]
+    rng = np.random.default_rng(9)
+    fam = []
+    for b in base[:10]:
+        for _ in range(10):
+            g = b.copy()
+            g[:3] = (g[:3] + rng.normal(0, 0.05, 3)) % (2 * np.pi)
+            g[3:] += rng.normal(0, 0.5, 3)
+            fam.append(g)
+    genes = np.concatenate([base, np.array(fam)])
+    got = cx.cluster(genes)
+    exp, _ = orc.cluster_poses(rec.atoms, lig.atoms, genes, nthreads=16)
+    assert [(r.center_idx, r.members) for r in got] == exp
+    assert len(exp) >= 3
