@@ -348,54 +348,74 @@ __global__ void compact_kernel(const uint32_t* __restrict__ bits, int n, int n_p
 
 constexpr int kRowsPerWave = 16;
 
-// Tile: 64 rows (4 waves x 16 rows) x 64 columns (one per lane).  Writes the
-// words A[i][jblk] (bits j with j in nbr(i)) and AT[i][jblk] (bits j with i in
-// nbr(j)) for the tile's rows.
+// Tile: 64 rows (4 waves x 16 rows) x 64 columns (one per lane).  The common
+// count c is symmetric, so only tiles with jblk >= iblk are computed.  A tile
+// writes the words A[i][jblk] (bits j with j in nbr(i)) and AT[i][jblk] (bits j
+// with i in nbr(j)) of its rows and, off the diagonal, the mirrored words
+// A[j][iblk] (bits i with i in nbr(j)) and AT[j][iblk] (bits i with j in
+// nbr(i)), each lane collecting its column's bits over the tile's rows.
 __global__ __launch_bounds__(256) void overlap_kernel(const uint32_t* __restrict__ T, int n, int n_pad, int wc,
                                                       const int32_t* __restrict__ sizes,
                                                       const int32_t* __restrict__ cmin_a,
                                                       const int32_t* __restrict__ cmin_b, int zero_ok,
                                                       unsigned long long* __restrict__ A,
                                                       unsigned long long* __restrict__ AT, int nwords) {
+  __shared__ unsigned long long s_ta[4][64], s_tat[4][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int jblk = blockIdx.x, iblk = blockIdx.y;
+  if (jblk < iblk) return;  // the mirror of tile (jblk, iblk)
   const int j = jblk * 64 + lane;
   const int i0 = iblk * 64 + wave * kRowsPerWave;
-  if (i0 >= n) return;  // whole wave past the last row
-  uint32_t c[kRowsPerWave];
+  unsigned long long ta = 0ull, tat = 0ull;  // mirrored words of column j over this wave's rows
+  if (i0 < n) {
+    uint32_t c[kRowsPerWave];
 #pragma unroll
-  for (int r = 0; r < kRowsPerWave; ++r) c[r] = 0u;
-  const uint32_t* colp = T + j;
-  const uint32_t* rowp = T + i0;
-  for (int k = 0; k < wc; ++k) {
-    const uint32_t xj = colp[static_cast<size_t>(k) * n_pad];
-    const uint32_t* rk = rowp + static_cast<size_t>(k) * n_pad;
+    for (int r = 0; r < kRowsPerWave; ++r) c[r] = 0u;
+    const uint32_t* colp = T + j;
+    const uint32_t* rowp = T + i0;
+    for (int k = 0; k < wc; ++k) {
+      const uint32_t xj = colp[static_cast<size_t>(k) * n_pad];
+      const uint32_t* rk = rowp + static_cast<size_t>(k) * n_pad;
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) c[r] += __popc(rk[r] & xj);
+      for (int r = 0; r < kRowsPerWave; ++r) c[r] += __popc(rk[r] & xj);
+    }
+    const int sj = j < n ? sizes[j] : 0;
+    const int cbj = j < n && sj > 0 ? cmin_b[sj] : INT_MAX;
+    const int caj = j < n && sj > 0 ? cmin_a[sj] : INT_MAX;
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      const int i = i0 + r;
+      if (i >= n) break;
+      const int si = sizes[i];
+      bool fwd, bwd;  // j in nbr(i); i in nbr(j)   (clustering.rs:169-177)
+      if (si == 0 || sj == 0) {
+        fwd = bwd = zero_ok && j < n;
+      } else {
+        const int c_ = static_cast<int>(c[r]);
+        fwd = c_ >= cmin_a[si] && c_ >= cbj;
+        bwd = c_ >= caj && c_ >= cmin_b[si];
+      }
+      fwd = fwd && j < n;
+      bwd = bwd && j < n;
+      const unsigned long long wf = __ballot(fwd);
+      const unsigned long long wb = __ballot(bwd);
+      if (lane == 0) {
+        A[static_cast<size_t>(i) * nwords + jblk] = wf;
+        AT[static_cast<size_t>(i) * nwords + jblk] = wb;
+      }
+      const unsigned long long bit = 1ull << (wave * kRowsPerWave + r);
+      ta |= bwd ? bit : 0ull;
+      tat |= fwd ? bit : 0ull;
+    }
   }
-  const int sj = j < n ? sizes[j] : 0;
-  const int cbj = j < n && sj > 0 ? cmin_b[sj] : INT_MAX;
-  const int caj = j < n && sj > 0 ? cmin_a[sj] : INT_MAX;
-#pragma unroll
-  for (int r = 0; r < kRowsPerWave; ++r) {
-    const int i = i0 + r;
-    if (i >= n) break;
-    const int si = sizes[i];
-    bool fwd, bwd;  // j in nbr(i); i in nbr(j)   (clustering.rs:169-177)
-    if (si == 0 || sj == 0) {
-      fwd = bwd = zero_ok && j < n;
-    } else {
-      const int c_ = static_cast<int>(c[r]);
-      fwd = c_ >= cmin_a[si] && c_ >= cbj;
-      bwd = c_ >= caj && c_ >= cmin_b[si];
-    }
-    const unsigned long long wf = __ballot(fwd && j < n);
-    const unsigned long long wb = __ballot(bwd && j < n);
-    if (lane == 0) {
-      A[static_cast<size_t>(i) * nwords + jblk] = wf;
-      AT[static_cast<size_t>(i) * nwords + jblk] = wb;
-    }
+  if (jblk == iblk) return;  // the diagonal tile is its own mirror
+  s_ta[wave][lane] = ta;
+  s_tat[wave][lane] = tat;
+  __syncthreads();
+  if (wave == 0 && j < n) {
+    A[static_cast<size_t>(j) * nwords + iblk] = s_ta[0][lane] | s_ta[1][lane] | s_ta[2][lane] | s_ta[3][lane];
+    AT[static_cast<size_t>(j) * nwords + iblk] = s_tat[0][lane] | s_tat[1][lane] | s_tat[2][lane] | s_tat[3][lane];
   }
 }
 
@@ -420,7 +440,7 @@ __global__ __launch_bounds__(256) void row_count_kernel(const unsigned long long
 }
 
 constexpr int kGreedyThreads = 1024;
-constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count + 1 bit used flag per element
+constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count, 4 B member slot and 1 bit used flag per element
 
 __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned long long* __restrict__ A,
                                                                 const unsigned long long* __restrict__ AT,
@@ -431,8 +451,9 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
                                                                 int64_t* __restrict__ out_offset,
                                                                 int64_t* __restrict__ out_members,
                                                                 int32_t* __restrict__ out_n) {
-  extern __shared__ unsigned long long s_used[];  // nwords, then count[n]
+  extern __shared__ unsigned long long s_used[];  // nwords, then count[n], then the current members[n]
   int32_t* s_count = reinterpret_cast<int32_t*>(s_used + nwords);
+  int32_t* s_members = s_count + n;
   __shared__ unsigned long long s_key[kGreedyThreads / 64];
   __shared__ int32_t s_scan[kGreedyThreads / 64];
   __shared__ unsigned long long s_best;
@@ -490,14 +511,17 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
     for (int i = 0; i < wave; ++i) wbase += s_scan[i];
     if (tid == kGreedyThreads - 1) s_total = wbase + incl;
     int pos = wbase + incl - local;
-    for (unsigned long long m = mine; m; m &= m - 1)
-      out_members[written + pos++] = static_cast<int64_t>(tid) * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+    for (unsigned long long m = mine; m; m &= m - 1) {
+      const int e = tid * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+      s_members[pos] = e;
+      out_members[written + pos++] = e;
+    }
     if (tid < nwords) s_used[tid] |= mine;
     __syncthreads();
     const int64_t size = s_total;
     // decrement the counts of elements that have a new member as neighbour
     for (int64_t q = tid; q < size * nwords; q += kGreedyThreads) {
-      const int64_t u = out_members[written + q / nwords];
+      const int64_t u = s_members[q / nwords];
       const int w = static_cast<int>(q % nwords);
       for (unsigned long long m = AT[static_cast<size_t>(u) * nwords + w]; m; m &= m - 1)
         atomicSub(&s_count[w * 64 + (__ffsll(static_cast<long long>(m)) - 1)], 1);
@@ -683,7 +707,7 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
                      static_cast<unsigned long long*>(P(o_A)), static_cast<unsigned long long*>(P(o_AT)), nwords);
   hipLaunchKernelGGL(row_count_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream,
                      static_cast<const unsigned long long*>(P(o_A)), n, nwords, static_cast<int32_t*>(P(o_rc)));
-  hipLaunchKernelGGL(greedy_kernel, dim3(1), dim3(kGreedyThreads), static_cast<size_t>(nwords) * 8 + static_cast<size_t>(n) * 4,
+  hipLaunchKernelGGL(greedy_kernel, dim3(1), dim3(kGreedyThreads), static_cast<size_t>(nwords) * 8 + static_cast<size_t>(n) * 8,
                      ctx->stream, static_cast<const unsigned long long*>(P(o_A)),
                      static_cast<const unsigned long long*>(P(o_AT)), static_cast<const int32_t*>(P(o_rc)), n, nwords,
                      static_cast<long long>(cfg->min_cluster_size), static_cast<int64_t*>(P(o_cc)),
