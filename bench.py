@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched_scoring", choices=["batched_scoring", "uneven_batch", "serve", "fcc"])
+    ap.add_argument("--workload", default="batched_scoring", choices=["batched_scoring", "uneven_batch", "serve", "fcc", "ga"])
     ap.add_argument("--jobs", type=int, default=160, help="serve: number of docking jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate, jobs/s (0: all at once)")
     ap.add_argument("--poses", type=int, default=4096)
@@ -89,6 +89,9 @@ def main():
         return
     if args.workload == "fcc":
         bench_fcc(args)
+        return
+    if args.workload == "ga":
+        bench_ga(args)
         return
     import torch
     from gdock_amd import _lib, structure, synth
@@ -342,6 +345,7 @@ def bench_serve(args):
         "pose_evals_per_s": st["poses_scored"] / span,
         "generations": gens, "peak_active_jobs": st["peak_active"], "steps_run": st["steps"],
         "server_busy_s": st["busy_s"], "mean_step_ms": st["busy_s"] / max(st["steps"], 1) * 1e3,
+        "mean_step_kernel_ms": st["kernel_s"] / max(st["steps"], 1) * 1e3,
         "plain": {"jobs_per_min": len(jobs) / pspan * 60.0, "latency_p50_s": float(np.percentile(plat, 50)),
                   "latency_p99_s": float(np.percentile(plat, 99)), "mean_service_s": float(np.mean(service)),
                   "generations": pgens, "pose_evals_per_s": pgens * 150 / float(np.sum(service))},
@@ -392,6 +396,55 @@ def bench_fcc(args):
         "n_clusters": len(clusters),
         "largest_clusters": [cl.size for cl in clusters[:5]],
     }
+    print(json.dumps(result), flush=True)
+
+
+def bench_ga(args):
+    """Plain GA dock of the 60 x 76-residue complex (seed 0, 6 restraints): the
+    full run (population 150, early stop after 10 flat generations, seed 42,
+    FCC of the Hall of Fame and model selection) through gdock_ga_run, one
+    scoring launch per generation.  The CPU leg runs the oracle's GA (same
+    random stream, so the same decisions) on the host's cores."""
+    import torch  # noqa: F401
+    from gdock_amd import ga, structure, synth
+    from gdock_amd.device import Complex, default_context
+
+    cfg = synth.CONFIGS["ga_dock"]
+    c = synth.make_complex(cfg["seed"], rec_res=cfg["rec_res"], lig_res=cfg["lig_res"],
+                           n_restraints=cfg["n_restraints"])
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+    cx = Complex(rec, lig, c.restraints, default_context(0))
+    for _ in range(args.warmup):
+        ga.run(cx, seed=42)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ga.run(cx, seed=42)
+    wall = (time.perf_counter() - t0) / args.steps
+    evals = res.generations * 150
+    result = {
+        "metric": BASELINE_METRIC,
+        "value": evals / wall, "unit": "pose-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded complex)",
+        "config": {"workload": f"ga_dock: receptor {len(rec)} / ligand {len(lig)} atoms (60 x 76 residues, seed 0), "
+                               f"population 150, early stop 10, seed 42, FCC of the Hall of Fame",
+                   "parallelism": "single GPU"},
+        "job_wall_s": wall, "generations": res.generations, "hof": len(res.hof_fitness),
+        "clusters": len(res.cluster_center), "selected": res.selected,
+    }
+    if not args.no_cpu:
+        import oracle.oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        ora = orc.GaRun(rec.atoms, lig.atoms, c.restraints, seed=42, nthreads=threads)
+        cpu = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": ora.generations * 150 / cpu, "unit": "pose-evals/s", "cores": threads,
+                                  "kind": "port", "sample": f"the whole GA run ({ora.generations} generations), "
+                                  f"{cpu:.1f} s wall, FP64 oracle"}
+        result["cpu_job_wall_s"] = cpu
+        result["same_run_as_oracle"] = bool(ora.generations == res.generations and ora.selected == res.selected)
     print(json.dumps(result), flush=True)
 
 

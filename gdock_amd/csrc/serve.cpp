@@ -85,7 +85,8 @@ struct gdock_server {
   // statistics
   int64_t n_steps = 0, n_poses = 0, n_finished = 0;
   int32_t peak_active = 0;
-  double busy_s = 0;
+  double busy_s = 0, kernel_s = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   void run();
   void admit_locked(std::vector<Job*>* fresh);
@@ -206,12 +207,16 @@ bool gdock_server::step() {
                        hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(const_cast<int32_t*>(L.n_items), &head, sizeof head, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
   if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h_out, L.out, static_cast<size_t>(max_row) * kOutDoubles * sizeof(double),
                        hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
+  float kms = 0.0f;
+  if (e == hipSuccess && hipEventElapsedTime(&kms, ev0, ev1) == hipSuccess) kernel_s += kms * 1e-3;
   ++n_steps;
   n_poses += n_items;
 
@@ -298,6 +303,8 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items), cap * sizeof(int2));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_table), c.max_jobs * sizeof(DevComplex));
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev1);
   char* m = static_cast<char*>(s->mem);
   BatchLaunch& L = s->L;
   if (e == hipSuccess) {
@@ -346,6 +353,8 @@ void gdock_server_destroy(gdock_server* s) {
   for (auto& kv : s->jobs)
     if (kv.second->cx) gdock_complex_destroy(s->ctx, kv.second->cx);
   if (s->ctx) (void)hipSetDevice(s->ctx->device);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->mem) (void)hipFree(s->mem);
@@ -419,6 +428,7 @@ gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out) {
   out->jobs_queued = static_cast<int64_t>(s->queue.size());
   out->peak_active = s->peak_active;
   out->busy_s = s->busy_s;
+  out->kernel_s = s->kernel_s;
   out->bytes_in_use = static_cast<int64_t>(s->bytes_in_use);
   out->memory_budget = s->cfg.memory_budget;
   return GDOCK_OK;

@@ -251,6 +251,7 @@ typedef struct gdock_server_stats_t {
   int32_t peak_active, reserved;
   double busy_s;               /* host time spent in scoring steps */
   int64_t bytes_in_use, memory_budget;
+  double kernel_s;             /* device time of the scoring graph replays (HIP events) */
 } gdock_server_stats_t;
 typedef struct gdock_server gdock_server;
 gdock_status gdock_server_default_config(gdock_server_config* c);
