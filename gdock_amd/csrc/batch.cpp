@@ -27,6 +27,7 @@ struct gdock_batch {
   std::vector<gdock_complex*> cx;
   std::vector<int32_t> pose_off;  // n + 1
   std::vector<int32_t> lanes_k;
+  std::vector<int32_t> parts;
   std::vector<int2> items;
   int32_t n_poses = 0;
   int32_t max_n_rec = 0;
@@ -69,6 +70,42 @@ std::vector<int> lane_candidates(int n_lig) {
 
 int rec_size_class(int n_rec) { return static_cast<int>(std::lround(2.0 * std::log2(std::max(n_rec, 1)))); }
 }  // namespace
+
+// Workgroups per pose in batched launches: a pose of 8e6 padded pairs or more
+// is split by receptor range into parts of about 4e6 pairs (at least 512
+// receptor slots each, at most kMaxPoseParts), so that a large complex's poses spread over more CUs when a
+// launch holds few of them.  A function of the complex's shape only, so a
+// complex is scored the same way in any batch.
+#ifndef GDOCK_SPLIT_PAIRS
+#define GDOCK_SPLIT_PAIRS 4000000
+#endif
+int pose_parts(const DevComplex& d, int lanes_k) {
+  const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * std::max(lanes_k, 1);
+  const int64_t chunks = (d.n_lig + per_chunk - 1) / per_chunk;
+  const int64_t pairs = static_cast<int64_t>(d.n_rec) * chunks * per_chunk;
+  int64_t p = std::min<int64_t>(pairs / GDOCK_SPLIT_PAIRS, d.n_rec / 512);
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p, kMaxPoseParts)));
+}
+
+size_t split_state_bytes(const DevComplex& d, int parts, int poses) {
+  if (parts <= 1 || poses <= 0) return 0;
+  return (static_cast<size_t>(poses) * d.n_lig * 4 + 255) / 256 * 256 +
+         (static_cast<size_t>(poses) * parts * 3 * 8 + 255) / 256 * 256 + (static_cast<size_t>(poses) * 2 * 4 + 255) / 256 * 256;
+}
+
+void bind_split_state(DevComplex* d, int parts, int poses, int pose_base, char* mem) {
+  d->parts = parts;
+  d->split_pose_base = pose_base;
+  d->split_lig = nullptr;
+  d->split_sums = nullptr;
+  d->split_sync = nullptr;
+  if (parts <= 1 || !mem) { d->parts = 1; return; }
+  const size_t a = (static_cast<size_t>(poses) * d->n_lig * 4 + 255) / 256 * 256;
+  const size_t b = (static_cast<size_t>(poses) * parts * 3 * 8 + 255) / 256 * 256;
+  d->split_lig = reinterpret_cast<int32_t*>(mem);
+  d->split_sums = reinterpret_cast<double*>(mem + a);
+  d->split_sync = reinterpret_cast<int32_t*>(mem + a + b);
+}
 
 int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err) {
   *err = hipSuccess;
@@ -133,7 +170,7 @@ extern "C" {
 
 gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes, int32_t n,
                                 const int32_t* poses_per_complex, gdock_batch** out) {
-  if (!ctx || !out || n < 0 || (n > 0 && (!complexes || !poses_per_complex))) {
+  if (!ctx || !out || n < 0 || n >= (1 << 24) || (n > 0 && (!complexes || !poses_per_complex))) {
     if (ctx) set_error(ctx, "batch_create: invalid arguments");
     return GDOCK_ERR_INVALID_ARG;
   }
@@ -173,18 +210,31 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
     if (e != hipSuccess) { delete b; return hip_fail(ctx, e, "batch_create: autotune"); }
   }
 
-  // Items longest first: cost = receptor slots x ligand lane-slots walked.
+  b->parts.resize(n);
+  int64_t n_items = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    b->parts[i] = pose_parts(b->cx[i]->dev, b->lanes_k[i]);
+    n_items += static_cast<int64_t>(poses_per_complex[i]) * b->parts[i];
+  }
+  if (n_items > (1 << 30)) {
+    delete b;
+    set_error(ctx, "batch_create: %lld work items is too many", static_cast<long long>(n_items));
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  // Items longest first: cost = receptor slots x ligand lane-slots walked,
+  // per part; the parts of a pose are consecutive items.
   std::vector<int32_t> order(n);
   std::iota(order.begin(), order.end(), 0);
   auto cost = [&](int32_t i) {
     const DevComplex& d = b->cx[i]->dev;
     const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * b->lanes_k[i];
     const int64_t chunks = (d.n_lig + per_chunk - 1) / per_chunk;
-    return static_cast<int64_t>(d.n_rec) * chunks * per_chunk;
+    return static_cast<int64_t>(d.n_rec) * chunks * per_chunk / b->parts[i];
   };
   std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
   for (int32_t i : order)
-    for (int32_t p = b->pose_off[i]; p < b->pose_off[i + 1]; ++p) b->items.push_back(make_int2(i, p));
+    for (int32_t p = b->pose_off[i]; p < b->pose_off[i + 1]; ++p)
+      for (int32_t q = 0; q < b->parts[i]; ++q) b->items.push_back(make_int2(i | (q << 24), p));
 
   // Device memory.
   std::vector<DevComplex> table(std::max(n, 1));
@@ -193,10 +243,18 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
     table[i].lanes_k = b->lanes_k[i];
   }
   const size_t np = std::max<int64_t>(total, 1);
+  const size_t ni = std::max<size_t>(b->items.size(), 1);
   Layout Ly;
-  const size_t o_tab = Ly.add<DevComplex>(table.size()), o_items = Ly.add<int2>(np), o_n = Ly.add<int32_t>(4),
+  const size_t o_tab = Ly.add<DevComplex>(table.size()), o_items = Ly.add<int2>(ni), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(np * kPoseDoubles), o_out = Ly.add<double>(np * kOutDoubles),
                o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np);
+  std::vector<size_t> o_split(n, 0);
+  const size_t split_begin = Ly.size;
+  for (int32_t i = 0; i < n; ++i) {
+    const size_t bytes = split_state_bytes(b->cx[i]->dev, b->parts[i], poses_per_complex[i]);
+    if (bytes) o_split[i] = Ly.add<uint8_t>(bytes);
+  }
+  const size_t split_end = Ly.size;
   e = hipMalloc(&b->mem, Ly.size);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_poses), np * kPoseDoubles * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_out), np * kOutDoubles * sizeof(double));
@@ -206,6 +264,9 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
     return hip_fail(ctx, e, "batch_create: allocation");
   }
   char* m = static_cast<char*>(b->mem);
+  for (int32_t i = 0; i < n; ++i)
+    bind_split_state(&table[i], b->parts[i], poses_per_complex[i], b->pose_off[i],
+                     b->parts[i] > 1 ? m + o_split[i] : nullptr);
   int32_t head[4] = {static_cast<int32_t>(b->items.size()), 0, 0, 0};  // n_items, counter, exit count
   std::vector<uint8_t> blob(o_poses, 0);
   std::memcpy(blob.data() + o_tab, table.data(), table.size() * sizeof(DevComplex));
@@ -213,6 +274,8 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   std::memcpy(blob.data() + o_n, head, sizeof head);
   e = hipMemcpyAsync(m, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemsetAsync(m + o_ns, 0, (o_redo - o_ns) + np * sizeof(int32_t), ctx->stream);
+  if (e == hipSuccess && split_end > split_begin)  // split state starts cleared; the kernel clears it after use
+    e = hipMemsetAsync(m + split_begin, 0, split_end - split_begin, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     gdock_batch_destroy(ctx, b);
@@ -249,9 +312,11 @@ void gdock_batch_destroy(gdock_ctx* ctx, gdock_batch* b) {
 
 int32_t gdock_batch_n_poses(const gdock_batch* b) { return b ? b->n_poses : -1; }
 
-gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* grid, int32_t* lds_bytes) {
+gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* parts, int32_t* grid,
+                              int32_t* lds_bytes) {
   if (!b) return GDOCK_ERR_INVALID_ARG;
   if (lanes_k) std::copy(b->lanes_k.begin(), b->lanes_k.end(), lanes_k);
+  if (parts) std::copy(b->parts.begin(), b->parts.end(), parts);
   if (grid) *grid = b->L.grid;
   if (lds_bytes) *lds_bytes = static_cast<int32_t>(b->L.lds);
   return GDOCK_OK;

@@ -25,6 +25,7 @@ constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, 
 
 constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup
 constexpr int kMaxAtomsPerLane = 8;
+constexpr int kMaxPoseParts = 8;  // workgroups one pose can be split over
 constexpr int kMaxLdsReceptor = 16000;  // receptor burial counters kept in LDS
 constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away ...
 constexpr float kPadRecCoord = -1.0e4f;  // ... on the other side from padding receptor atoms
@@ -68,6 +69,14 @@ struct DevComplex {
   int32_t n_lig, n_class, n_restr;
   int32_t n_coll;       // pairs in coll_pairs
   int32_t lanes_k;      // ligand atoms per lane chosen for batched launches (0: ceil(n_lig / 256), at most 8)
+  // batched launches: a pose split across `parts` workgroups by receptor range
+  // (1: not split), with the split state of the complex's poses, indexed by
+  // global pose row - split_pose_base
+  int32_t parts;
+  int32_t split_pose_base;
+  int32_t* split_lig;   // n_lig ints per pose
+  double* split_sums;   // 3 doubles per part per pose
+  int32_t* split_sync;  // 2 ints per pose
   double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
   double origin[3];  // FP32 frame origin
   double rec_extent;   // max |rec - origin| component

@@ -99,6 +99,12 @@ void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo,
 
 // Ligand atoms per lane for a complex, measured once per complex shape.
 int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err);
+// Workgroups per pose for a complex's shape (1 = unsplit, at most kMaxPoseParts).
+int pose_parts(const DevComplex& d, int lanes_k);
+// Device bytes of the split state of `poses` poses, and its binding into a
+// table entry (pose rows [pose_base, pose_base + poses), state cleared).
+size_t split_state_bytes(const DevComplex& d, int parts, int poses);
+void bind_split_state(DevComplex* d, int parts, int poses, int pose_base, char* mem);
 
 void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host);
 void bind_contact_tables(gdock_complex* cx);

@@ -23,6 +23,7 @@
 // in FP64 in the reference's order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "common.h"
@@ -354,6 +355,58 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
   }
 }
 
+// The receptor slots a workgroup covers for one pose.  A pose of a large
+// complex can be split across `parts` workgroups by receptor range: each part
+// adds its ligand burial counts into the pose's shared count table and leaves
+// its {vdw, elec, receptor desolvation} sums; the last part to finish
+// combines them in part order, adds the ligand desolvation from the summed
+// counts, finishes the pose and clears the shared state for the next launch.
+struct PosePart {
+  int r0, r1;          // receptor slots [r0, r1)
+  int part, parts;     // parts == 1: the whole pose in this workgroup
+  int32_t* lig_cnt;    // parts > 1: n_lig burial counts of the pose's ligand atoms
+  double* sums;        // parts > 1: 3 doubles per part
+  int32_t* sync;       // parts > 1: {parts finished, overflow seen}
+};
+
+// AIR, restraint penalty and the weighted fitness (thread 0), then the outputs.
+__device__ __forceinline__ void finish_pose(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords,
+                                            const Weights& w, double vdw, double elec, double desolv,
+                                            double* __restrict__ o, int32_t* __restrict__ nsat_out) {
+  // fitness.rs:205-267, restraints.rs:11-40: FP64 CA-CA flat-bottom term
+  double air = 0.0;
+  int32_t nsat = 0;
+  for (int i = 0; i < cx.n_restr; ++i) {
+    double px, py, pz;
+    ligand_atom(cx, pc, coords, cx.restr_lig[i], px, py, pz);
+    const double ex = cx.restr_rec_xyz[3 * i] - px, ey = cx.restr_rec_xyz[3 * i + 1] - py,
+                 ez = cx.restr_rec_xyz[3 * i + 2] - pz;
+    const double e2 = ex * ex + ey * ey + ez * ez;
+    const double dist = sqrt(e2);
+    const double viol = e2 > kD2RestraintIncl ? dist - kAirUpper : 0.0;
+    air += kAirK * viol * viol;
+    nsat += e2 <= kD2RestraintIncl ? 1 : 0;
+  }
+  const double n = static_cast<double>(cx.n_restr);
+  const double ratio = cx.n_restr > 0 ? static_cast<double>(nsat) / n : 0.0;
+  o[0] = vdw;
+  o[1] = elec;
+  o[2] = desolv;
+  o[3] = air;
+  o[4] = (1.0 - ratio) * n;  // chromosome.rs:117-118
+  o[5] = w.vdw * vdw + w.elec * elec + w.desolv * desolv + w.air * air;  // chromosome.rs:121-124
+  if (nsat_out) *nsat_out = nsat;
+}
+
+// fitness.rs:165-181: one ligand atom's desolvation term.
+__device__ __forceinline__ double ligand_desolv(CxRef cx, int l, int32_t cnt) {
+  const double asp = cx.lig_asp[l];
+  if (!(fabs(asp) >= 0.001)) return 0.0;
+  const double burial = fmin(static_cast<double>(cnt) / 10.0, 1.0);
+  const double rad = cx.lig_radius[l];
+  return asp * burial * rad * rad * 4.0 * kPI;
+}
+
 // Scores one pose with the whole workgroup.  pose_rec: the pose record (12
 // doubles) or null; coords: this pose's ligand coordinates (n_lig x 3) when
 // scoring coordinates as given, with `extent` = their largest |x - origin|.
@@ -364,7 +417,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
                                            const double* __restrict__ coords, double extent, const Weights& w,
                                            double* __restrict__ o, int32_t* __restrict__ nsat_out,
                                            int32_t* __restrict__ redo, int32_t* __restrict__ s_rcnt,
-                                           int32_t* s_overflow) {
+                                           int32_t* s_overflow, const PosePart& pp) {
   double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
   SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -386,7 +439,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
   guard_band(144.0, X, B.lo144, B.hi144);
   guard_band(225.0, X, B.lo225, B.hi225);
 
-  for (int i = tid; i < cx.n_rec; i += kScoreThreads) s_rcnt[i] = 0;
+  for (int i = pp.r0 + tid; i < pp.r1; i += kScoreThreads) s_rcnt[i] = 0;
   if (tid == 0) *s_overflow = 0;
   __syncthreads();
 
@@ -421,8 +474,8 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     }
     int nev = 0;
     int cls = -1, cend = 0;
-    for (int g0 = 0; g0 < cx.n_rec; g0 += 64) {
-      const int g1 = min(g0 + 64, cx.n_rec);
+    for (int g0 = pp.r0; g0 < pp.r1; g0 += 64) {
+      const int g1 = min(g0 + 64, pp.r1);
       // Groups beyond every cutoff from all of this wave's atoms add nothing
       // and hold no band pair: skipped (the whole tile when none is near).
       const uint32_t keep = groups_in_reach(cx, g0, box);
@@ -484,18 +537,18 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     for (int k = 0; k < K; ++k) {
       const int l = lig_index<K>(base, k);
       if (l < cx.n_lig) {
-        const double asp = cx.lig_asp[l];
-        if (fabs(asp) >= 0.001) {  // fitness.rs:165-181
-          const double burial = fmin(static_cast<double>(a.cnt[k]) / 10.0, 1.0);
-          const double rad = cx.lig_radius[l];
-          sum_d -= asp * burial * rad * rad * 4.0 * kPI;
+        if (pp.parts > 1) {  // the count over this part's receptor range only
+          if (a.cnt[k]) atomicAdd(&pp.lig_cnt[l], a.cnt[k]);
+        } else {
+          sum_d -= ligand_desolv(cx, l, a.cnt[k]);
         }
       }
     }
   }
   if (overflow && lane == 0) *s_overflow = 1;
   __syncthreads();
-  if (!EXACT && *s_overflow) {  // the exact variant rescores this pose
+  const bool ovf = !EXACT && *s_overflow != 0;
+  if (ovf && pp.parts == 1) {  // the exact variant rescores this pose
     if (tid == 0) *redo = 1;
     __syncthreads();
     return;
@@ -506,6 +559,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
   // value the pair loop added (same coordinates, factors and operations).
   for (int i = tid; i < cx.n_coll; i += kScoreThreads) {
     const int4 P = cx.coll_pairs[i];  // receptor slot, ligand atom, receptor class
+    if (P.x < pp.r0 || P.x >= pp.r1) continue;  // another part's receptor slot
     double px, py, pz;
     ligand_atom(cx, pc, coords, P.y, px, py, pz);
     const float lx = static_cast<float>(px - cx.origin[0]);
@@ -520,7 +574,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       sum_v -= static_cast<double>(t) * static_cast<double>(rho);
     }
   }
-  for (int j = tid; j < cx.n_rec; j += kScoreThreads) {  // fitness.rs:141-162
+  for (int j = pp.r0 + tid; j < pp.r1; j += kScoreThreads) {  // fitness.rs:141-162
     const double asp = cx.rec_asp[j];
     if (fabs(asp) >= 0.001) {
       const double burial = fmin(static_cast<double>(s_rcnt[j]) / 10.0, 1.0);
@@ -541,35 +595,53 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     s_red[3 * wave + 2] = sum_d;
   }
   __syncthreads();
-  if (tid == 0) {
-    double vdw = 0.0, elec = 0.0, desolv = 0.0;
+  double vdw = 0.0, elec = 0.0, desolv = 0.0;
+  if (tid == 0)
     for (int i = 0; i < kScoreThreads / 64; ++i) {
       vdw += s_red[3 * i]; elec += s_red[3 * i + 1]; desolv += s_red[3 * i + 2];
     }
-    // fitness.rs:205-267, restraints.rs:11-40: FP64 CA-CA flat-bottom term
-    double air = 0.0;
-    int32_t nsat = 0;
-    for (int i = 0; i < cx.n_restr; ++i) {
-      double px, py, pz;
-      ligand_atom(cx, pc, coords, cx.restr_lig[i], px, py, pz);
-      const double ex = cx.restr_rec_xyz[3 * i] - px, ey = cx.restr_rec_xyz[3 * i + 1] - py,
-                   ez = cx.restr_rec_xyz[3 * i + 2] - pz;
-      const double e2 = ex * ex + ey * ey + ez * ez;
-      const double dist = sqrt(e2);
-      const double viol = e2 > kD2RestraintIncl ? dist - kAirUpper : 0.0;
-      air += kAirK * viol * viol;
-      nsat += e2 <= kD2RestraintIncl ? 1 : 0;
+  if (pp.parts == 1) {
+    if (tid == 0) {
+      finish_pose(cx, pc, coords, w, vdw, elec, desolv, o, nsat_out);
+      if (!EXACT) *redo = 0;
     }
-    const double n = static_cast<double>(cx.n_restr);
-    const double ratio = cx.n_restr > 0 ? static_cast<double>(nsat) / n : 0.0;
-    o[0] = vdw;
-    o[1] = elec;
-    o[2] = desolv;
-    o[3] = air;
-    o[4] = (1.0 - ratio) * n;  // chromosome.rs:117-118
-    o[5] = w.vdw * vdw + w.elec * elec + w.desolv * desolv + w.air * air;  // chromosome.rs:121-124
-    if (nsat_out) *nsat_out = nsat;
-    if (!EXACT) *redo = 0;
+  } else {
+    int32_t* s_last = s_overflow + 1;
+    if (tid == 0) {
+      pp.sums[3 * pp.part + 0] = vdw;
+      pp.sums[3 * pp.part + 1] = elec;
+      pp.sums[3 * pp.part + 2] = desolv;
+      if (ovf) atomicOr(&pp.sync[1], 1);
+      __threadfence();
+      *s_last = atomicAdd(&pp.sync[0], 1) == pp.parts - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (*s_last) {  // every part has finished: combine
+      __threadfence();
+      double sdl = 0.0;
+      for (int l = tid; l < cx.n_lig; l += kScoreThreads) sdl -= ligand_desolv(cx, l, atomicExch(&pp.lig_cnt[l], 0));
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sdl += __shfl_xor(sdl, off);
+      __syncthreads();
+      if (lane == 0) s_red[wave] = sdl;
+      __syncthreads();
+      if (tid == 0) {
+        double lig_d = 0.0;
+        for (int i = 0; i < kScoreThreads / 64; ++i) lig_d += s_red[i];
+        double v = 0.0, e = 0.0, d = 0.0;
+        for (int q = 0; q < pp.parts; ++q) {
+          v += pp.sums[3 * q]; e += pp.sums[3 * q + 1]; d += pp.sums[3 * q + 2];
+        }
+        const int any_ovf = atomicExch(&pp.sync[1], 0);
+        atomicExch(&pp.sync[0], 0);
+        if (any_ovf) {
+          *redo = 1;  // the exact variant rescores the whole pose
+        } else {
+          finish_pose(cx, pc, coords, w, v, e, d + lig_d, o, nsat_out);
+          *redo = 0;
+        }
+      }
+    }
   }
   __syncthreads();  // LDS is reused by the workgroup's next pose
 }
@@ -582,12 +654,13 @@ __global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
   const int pose = blockIdx.x;
   if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
   extern __shared__ int32_t s_lds[];
-  __shared__ int32_t s_overflow;
+  __shared__ int32_t s_flags[2];
   CxRef cx = *(const DevComplex GDOCK_AS4*)d_cx;
+  const PosePart whole{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
   score_pose<K, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
                              coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr,
                              coords ? pose_extent[pose] : 0.0, w, out + static_cast<size_t>(kOutDoubles) * pose,
-                             nsat_out ? nsat_out + pose : nullptr, redo + pose, s_lds, &s_overflow);
+                             nsat_out ? nsat_out + pose : nullptr, redo + pose, s_lds, s_flags, whole);
 }
 
 int score_atoms_per_lane(int n_lig) {
@@ -602,11 +675,11 @@ template <bool EXACT>
 __device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ pose_rec, const Weights& w,
                                            double* __restrict__ o, int32_t* __restrict__ nsat,
                                            int32_t* __restrict__ redo, int32_t* __restrict__ s_lds,
-                                           int32_t* s_overflow) {
+                                           int32_t* s_flags, const PosePart& pp) {
   switch (cx.lanes_k) {
 #define GDOCK_ITEM(KK)                                                                                   \
   case KK:                                                                                               \
-    score_pose<KK, EXACT>(cx, pose_rec, nullptr, 0.0, w, o, nsat, redo, s_lds, s_overflow);             \
+    score_pose<KK, EXACT>(cx, pose_rec, nullptr, 0.0, w, o, nsat, redo, s_lds, s_flags, pp);            \
     break;
     GDOCK_ITEM(1) GDOCK_ITEM(2) GDOCK_ITEM(3) GDOCK_ITEM(4)
     GDOCK_ITEM(5) GDOCK_ITEM(6) GDOCK_ITEM(7) GDOCK_ITEM(8)
@@ -620,12 +693,27 @@ __device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ 
 // read from device memory so that one captured launch serves any number of
 // items up to the capacity.  The last workgroup to leave resets the counter
 // and the exit count, so the next launch starts from zero with no memset.
+__device__ __forceinline__ PosePart part_of(CxRef cx, int part, int pose_row) {
+  if (cx.parts <= 1) return PosePart{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
+  const int tiles = (cx.n_rec + 63) / 64;
+  const int t0 = part * tiles / cx.parts, t1 = (part + 1) * tiles / cx.parts;
+  const size_t q = static_cast<size_t>(pose_row - cx.split_pose_base);
+  return PosePart{64 * t0, min(64 * t1, cx.n_rec), part, cx.parts, cx.split_lig + q * cx.n_lig,
+                  cx.split_sums + q * 3 * cx.parts, cx.split_sync + q * 2};
+}
+
+// Persistent scoring of a batch: every workgroup takes items from a shared
+// counter until the list (longest first) is exhausted.  An item is
+// {complex slot | part << 24, global pose row}.  The item count is read from
+// device memory so that one captured launch serves any number of items up to
+// the capacity.  The last workgroup to leave resets the counter and the exit
+// count, so the next launch starts from zero with no memset.
 __global__ __launch_bounds__(kScoreThreads) void score_batch_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     int32_t* __restrict__ counter, int32_t* __restrict__ exit_count, const double* __restrict__ poses,
     const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
   extern __shared__ int32_t s_lds[];
-  __shared__ int32_t s_overflow, s_item;
+  __shared__ int32_t s_flags[2], s_item;
   const int n_items = *n_items_p;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -634,9 +722,10 @@ __global__ __launch_bounds__(kScoreThreads) void score_batch_kernel(
     __syncthreads();
     if (it >= n_items) break;
     const int2 I = items[it];
-    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x];
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
+    const PosePart pp = part_of(cx, I.x >> 24, I.y);
     score_item<false>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
-                      out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, &s_overflow);
+                      out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, pp);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -647,21 +736,24 @@ __global__ __launch_bounds__(kScoreThreads) void score_batch_kernel(
   }
 }
 
-// Exact rescoring of the batch items whose settle buffer overflowed (one
-// workgroup per item slot; all others return at once).
+// Exact rescoring of the batch's poses whose settle buffer overflowed: the
+// whole pose in one workgroup (items of part 0; the other parts' items and
+// every clean pose return at once).
 __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     const double* __restrict__ poses, const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out,
     int32_t* __restrict__ redo) {
   extern __shared__ int32_t s_lds[];
-  __shared__ int32_t s_overflow;
-  const int it = blockIdx.x;
-  if (it >= *n_items_p) return;
-  const int2 I = items[it];
-  if (redo[I.y] == 0) return;
-  CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x];
-  score_item<true>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
-                   out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, &s_overflow);
+  __shared__ int32_t s_flags[2];
+  const int n_items = *n_items_p;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int2 I = items[it];
+    if ((I.x >> 24) != 0 || redo[I.y] == 0) continue;  // uniform per workgroup
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
+    const PosePart whole{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
+    score_item<true>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
+                     out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, whole);
+  }
 }
 
 size_t score_lds_bytes(const DevComplex& cx) {
@@ -717,8 +809,8 @@ int batch_grid(size_t lds, int device) {
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream) {
   hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
                      b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo);
-  hipLaunchKernelGGL(score_batch_exact_kernel, dim3(b.capacity), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
-                     b.n_items, b.poses, w, b.out, b.nsat, b.redo);
+  hipLaunchKernelGGL(score_batch_exact_kernel, dim3(std::min(b.capacity, b.grid)), dim3(kScoreThreads), b.lds, stream,
+                     b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo);
   return hipGetLastError();
 }
 

@@ -48,6 +48,8 @@ struct Job {
   size_t bytes = 0;   // device bytes held while active
   int slot = -1;
   int lanes = 0;
+  int parts = 1;
+  void* split_mem = nullptr;  // split-pose state (parts > 1)
   std::unique_ptr<gdock_ga_result> result;
 };
 
@@ -146,10 +148,21 @@ bool gdock_server::step() {
       j->lanes = tune_lanes(ctx, j->cx, &he);
       if (he != hipSuccess) st = hip_fail(ctx, he, "server: autotune");
     }
+    size_t split_bytes = 0;
+    if (st == GDOCK_OK) {
+      j->parts = pose_parts(j->cx->dev, j->lanes);
+      split_bytes = split_state_bytes(j->cx->dev, j->parts, max_pop);
+      if (split_bytes) {
+        he = hipMalloc(&j->split_mem, split_bytes);
+        if (he == hipSuccess) he = hipMemsetAsync(j->split_mem, 0, split_bytes, ctx->stream);
+        if (he != hipSuccess) st = hip_fail(ctx, he, "server: split state");
+      }
+    }
     if (st == GDOCK_OK) {
       j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
       DevComplex d = j->cx->dev;
       d.lanes_k = j->lanes;
+      bind_split_state(&d, j->parts, max_pop, j->slot * max_pop, static_cast<char*>(j->split_mem));
       h_table[j->slot] = d;
       he = hipMemcpyAsync(const_cast<DevComplex*>(L.table) + j->slot, &h_table[j->slot], sizeof(DevComplex),
                           hipMemcpyHostToDevice, ctx->stream);
@@ -157,7 +170,7 @@ bool gdock_server::step() {
     }
     std::lock_guard<std::mutex> g(mu);
     if (st == GDOCK_OK) {  // charge the larger of the estimate and the arena
-      const size_t charged = std::max(j->bytes, j->cx->mem_bytes);
+      const size_t charged = std::max(j->bytes, j->cx->mem_bytes + split_bytes);
       bytes_in_use += charged - j->bytes;
       j->bytes = charged;
     } else {
@@ -166,6 +179,7 @@ bool gdock_server::step() {
       j->message = ctx->err;
       j->t_finish = now_s(t0);
       if (j->cx) { gdock_complex_destroy(ctx, j->cx); j->cx = nullptr; }
+      if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
       bytes_in_use -= j->bytes;
       slots[j->slot] = nullptr;
       ++n_finished;
@@ -185,18 +199,20 @@ bool gdock_server::step() {
   std::stable_sort(active.begin(), active.end(), [](Job* a, Job* b) {
     auto cost = [](Job* j) {
       const int64_t lanes = static_cast<int64_t>(kScoreThreads) * j->lanes;
-      return static_cast<int64_t>(j->cx->dev.n_rec) * ((j->cx->dev.n_lig + lanes - 1) / lanes) * lanes;
+      return static_cast<int64_t>(j->cx->dev.n_rec) * ((j->cx->dev.n_lig + lanes - 1) / lanes) * lanes / j->parts;
     };
     return cost(a) > cost(b);
   });
-  int n_items = 0, max_row = 0;
+  int n_items = 0, max_row = 0, step_poses = 0;
   for (Job* j : active) {
     const GaJob& ga = j->result->job;
     const int pop = ga.population();
     const int row0 = j->slot * max_pop;
     gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
-    for (int p = 0; p < pop; ++p) h_items[n_items++] = make_int2(j->slot, row0 + p);
+    for (int p = 0; p < pop; ++p)
+      for (int q = 0; q < j->parts; ++q) h_items[n_items++] = make_int2(j->slot | (q << 24), row0 + p);
     max_row = std::max(max_row, row0 + pop);
+    step_poses += pop;
   }
   int32_t head = n_items;
   hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
@@ -218,7 +234,7 @@ bool gdock_server::step() {
   float kms = 0.0f;
   if (e == hipSuccess && hipEventElapsedTime(&kms, ev0, ev1) == hipSuccess) kernel_s += kms * 1e-3;
   ++n_steps;
-  n_poses += n_items;
+  n_poses += step_poses;
 
   std::vector<Job*> finished;
   for (Job* j : active) {
@@ -241,6 +257,7 @@ bool gdock_server::step() {
   for (Job* j : finished) {
     gdock_complex_destroy(ctx, j->cx);
     j->cx = nullptr;
+    if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
     std::lock_guard<std::mutex> g(mu);
     if (j->state == kActive) j->state = kFinished;
     j->t_finish = now_s(t0);
@@ -276,7 +293,7 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   *out = nullptr;
   gdock_server_config c;
   if (config) c = *config; else gdock_server_default_config(&c);
-  if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_receptor_atoms < 1 ||
+  if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_population > 65536 || c.max_receptor_atoms < 1 ||
       c.max_receptor_atoms > kMaxLdsReceptor - 64 || c.memory_budget < 0)
     return GDOCK_ERR_INVALID_ARG;
   auto* s = new gdock_server();
@@ -292,15 +309,16 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   s->max_pop = c.max_population;
   s->slots.assign(c.max_jobs, nullptr);
   const size_t cap = static_cast<size_t>(c.max_jobs) * c.max_population;
+  const size_t cap_items = cap * kMaxPoseParts;  // a pose is at most kMaxPoseParts items
   Layout Ly;
-  const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_items = Ly.add<int2>(cap), o_n = Ly.add<int32_t>(4),
+  const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_items = Ly.add<int2>(cap_items), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(cap * kPoseDoubles), o_out = Ly.add<double>(cap * kOutDoubles),
                o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap);
   hipError_t e = hipMalloc(&s->mem, Ly.size);
   if (e == hipSuccess) e = hipMemsetAsync(s->mem, 0, Ly.size, s->ctx->stream);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), cap * kOutDoubles * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items), cap * sizeof(int2));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items), cap_items * sizeof(int2));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_table), c.max_jobs * sizeof(DevComplex));
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
   if (e == hipSuccess) e = hipEventCreate(&s->ev0);
@@ -318,7 +336,7 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
     L.redo = reinterpret_cast<int32_t*>(m + o_redo);
     L.lds = batch_lds_bytes(c.max_receptor_atoms + 64);
-    L.capacity = static_cast<int>(cap);
+    L.capacity = static_cast<int>(cap_items);
     L.grid = std::min(batch_grid(L.lds, device), L.capacity);
     // One graph for every step: the kernels read the item count from device memory.
     hipStream_t cs = nullptr;
@@ -350,8 +368,10 @@ void gdock_server_destroy(gdock_server* s) {
   }
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  for (auto& kv : s->jobs)
+  for (auto& kv : s->jobs) {
     if (kv.second->cx) gdock_complex_destroy(s->ctx, kv.second->cx);
+    if (kv.second->split_mem) (void)hipFree(kv.second->split_mem);
+  }
   if (s->ctx) (void)hipSetDevice(s->ctx->device);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);

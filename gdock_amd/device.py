@@ -178,11 +178,13 @@ class Batch:
         self.n_poses = int(lib().gdock_batch_n_poses(h))
 
     def plan(self) -> dict:
-        k = np.zeros(max(len(self.complexes), 1), np.int32)
+        n = len(self.complexes)
+        k = np.zeros(max(n, 1), np.int32)
+        parts = np.zeros(max(n, 1), np.int32)
         grid, lds = C.c_int32(), C.c_int32()
-        check(lib().gdock_batch_plan(self.handle, ptr(k, C.c_int32), C.byref(grid), C.byref(lds)),
-              what="gdock_batch_plan")
-        return {"lanes_k": k[:len(self.complexes)].tolist(), "grid": grid.value, "lds_bytes": lds.value}
+        check(lib().gdock_batch_plan(self.handle, ptr(k, C.c_int32), ptr(parts, C.c_int32), C.byref(grid),
+                                     C.byref(lds)), what="gdock_batch_plan")
+        return {"lanes_k": k[:n].tolist(), "parts": parts[:n].tolist(), "grid": grid.value, "lds_bytes": lds.value}
 
     def score(self, genes: np.ndarray, weights: constants.EnergyWeights | None = None):
         g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)

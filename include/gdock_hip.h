@@ -151,8 +151,12 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
                                 const int32_t* poses_per_complex, gdock_batch** out);
 void gdock_batch_destroy(gdock_ctx* ctx, gdock_batch* b);
 int32_t gdock_batch_n_poses(const gdock_batch* b);
-/* Chosen ligand atoms per lane (n entries), persistent grid and LDS bytes. */
-gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* grid, int32_t* lds_bytes);
+/* Chosen ligand atoms per lane (n entries), workgroups per pose (n entries:
+ * a pose of a large complex is split by receptor range over up to 8
+ * workgroups; a function of the complex's shape), persistent grid and LDS
+ * bytes.  Any output may be null. */
+gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* parts, int32_t* grid,
+                              int32_t* lds_bytes);
 /* Synchronous: genes (n_poses x 6) in, out (n_poses x 6) and optional
  * n_satisfied out, as gdock_score_poses. */
 gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* genes, const gdock_weights* w,
@@ -236,7 +240,7 @@ gdock_status gdock_calc_metrics(gdock_ctx* ctx, gdock_evaluator* ev, const doubl
  * share its launches.  The server owns its own context on `device`. */
 typedef struct gdock_server_config {
   int32_t max_jobs;            /* active job slots (64) */
-  int32_t max_population;      /* largest population a job may use (150) */
+  int32_t max_population;      /* largest population a job may use (150; at most 65536) */
   int32_t max_receptor_atoms;  /* sizes the kernel's LDS receptor table (8192) */
   int32_t reserved;
   int64_t memory_budget;       /* device bytes for resident jobs; 0: half the free memory */

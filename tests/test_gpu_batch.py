@@ -67,6 +67,81 @@ def test_batch_matches_oracle_and_replays(uneven):
         assert np.array_equal(ns1[a:z], sns)
 
 
+def _lattice(mol):
+    """The molecule with coordinates on a 0.5 A lattice: many pairs sit on the
+    cutoffs, so the settle buffers fill and overflow."""
+    a = mol.atoms.copy()
+    for c in "xyz":
+        a[c] = np.round(a[c] * 2.0) / 2.0
+    return structure.Molecule(a)
+
+
+def test_split_poses_match_oracle():
+    """Poses of large complexes are scored by several workgroups over receptor
+    ranges; the parts' sums, the shared ligand burial counts and the overflow
+    rescoring of the whole pose agree with the oracle and the one-workgroup
+    path, and repeated replays (which reuse the cleared split state) agree."""
+    specs = [(51, 6000, 3000, False), (52, 2600, 3300, False), (53, 5200, 2400, True)]
+    cases = []
+    for seed, nr, nl, lattice in specs:
+        c = synth.make_complex(seed, rec_atoms=nr, lig_atoms=nl, n_restraints=5)
+        rec, lig = _load(c)
+        if lattice:
+            rec, lig = _lattice(rec), _lattice(lig)
+        cases.append((rec, lig, c.restraints))
+    counts = [6, 4, 5]
+    cxs = [Complex(r, l, p) for r, l, p in cases]
+    b = Batch(cxs, counts)
+    plan = b.plan()
+    print("split plan:", plan)
+    assert all(q > 1 for q in plan["parts"]), plan
+    genes = []
+    for i, n in enumerate(counts):
+        g = np.concatenate([np.zeros((1, 6)), synth.ga_genes(700 + i, n - 1)])
+        genes.append(g)
+    genes = np.concatenate(genes)
+    out1, ns1 = b.score(genes)
+    out2, ns2 = b.score(genes)
+    assert np.array_equal(out1, out2) and np.array_equal(ns1, ns2)
+    for i, (r, l, p) in enumerate(cases):
+        a, z = b.offsets[i], b.offsets[i + 1]
+        ref, rns = orc.score_poses(r.atoms, l.atoms, p, genes[a:z], W, nthreads=8)
+        assert_energies_close(out1[a:z], ref, f"split complex {i}")
+        assert np.array_equal(ns1[a:z], rns)
+        single, sns = cxs[i].score_poses(genes[a:z])
+        assert_energies_close(out1[a:z], single, f"split complex {i}: batch vs single")
+        assert np.array_equal(ns1[a:z], sns)
+
+
+def test_split_pose_overflow_rescored():
+    """A receptor and a ligand on one integer lattice, large enough for split
+    poses: every part's settle buffer overflows on pairs exactly at the 8, 12
+    and 15 A cutoffs, so each pose is rescored whole by the exact kernel.
+    Poses: the identity and quarter turns (near-lattice in FP64)."""
+    from helpers import mol, test_atom as A
+    rec = structure.Molecule(mol(*[A(float(x), float(y), float(z), charge=0.5 if (x + y) % 2 else -0.5,
+                                     element="O" if z % 2 else "C", serial=i + 1, resseq=i // 8 + 1)
+                                   for i, (x, y, z) in enumerate((x, y, z) for x in range(16) for y in range(16)
+                                                                  for z in range(16))]))
+    lig = structure.Molecule(mol(*[A(float(x), float(y), float(z) + 17.0, charge=-0.25 if x % 3 else 0.4,
+                                     element="N" if y % 4 else "C", serial=9000 + i, chainid="B",
+                                     resseq=i // 8 + 1)
+                                   for i, (x, y, z) in enumerate((x, y, z) for x in range(16) for y in range(16)
+                                                                  for z in range(8))]))
+    cx = Complex(rec, lig)
+    b = Batch([cx], [4])
+    assert b.plan()["parts"][0] > 1, b.plan()
+    q = np.pi / 2
+    genes = np.array([[0, 0, 0, 0, 0, 0], [q, 0, 0, 0, 0, 0], [0, q, 0, 0, 0, 0], [0, 0, q, 0, 0, 1.0]])
+    out, ns = b.score(genes)
+    ref, rns = orc.score_poses(rec.atoms, lig.atoms, [], genes, W, nthreads=8)
+    assert_energies_close(out, ref, "split lattice storm")
+    single, _ = cx.score_poses(genes)
+    assert_energies_close(out, single, "split lattice storm: batch vs single")
+    out2, _ = b.score(genes)
+    assert np.array_equal(out, out2)
+
+
 def test_batch_device_launch_and_new_weights(uneven):
     import torch
     counts = [4] * len(uneven)

@@ -49,6 +49,32 @@ def test_alone_equals_in_mix():
         srv.close()
 
 
+def test_split_job_alone_equals_in_mix():
+    """A job large enough for split poses (5000 x 2000 atoms: each pose over
+    two workgroups) is bit-identical alone and among small jobs, and takes
+    the same decisions as the plain one-launch-per-generation path (whose
+    poses are unsplit: sums in another order, so near-ties may differ)."""
+    from gdock_amd import ga
+    from gdock_amd.device import Complex
+    c = synth.make_complex(91, rec_atoms=5000, lig_atoms=2000, n_restraints=6)
+    big = (structure.read_pdb_text(c.receptor_pdb)[0], structure.read_pdb_text(c.ligand_pdb)[0], c.restraints)
+    small = [_job(92 + i, 30, 25) for i in range(3)]
+    params = dict(population_size=40, max_generations=10, keep_trace=1)
+    srv = Server(max_jobs=6)
+    try:
+        alone, _ = srv.wait(srv.submit(*big, seed=8, **params), 180.0)
+        ids = [srv.submit(*j, seed=20 + k, **params) for k, j in enumerate(small[:2])]
+        ids.append(srv.submit(*big, seed=8, **params))
+        ids.append(srv.submit(*small[2], seed=30, **params))
+        got = [srv.wait(j, 180.0)[0] for j in ids]
+    finally:
+        srv.close()
+    _same(alone, got[2])
+    rec, lig, pairs = big
+    plain = ga.run(Complex(rec, structure.position_ligand(rec, lig), pairs), seed=8, **params)
+    print("split served vs plain:", _compare(alone, plain, 40))
+
+
 def test_served_job_matches_oracle_ga():
     rec, lig, pairs = _job(0, 60, 76, 6)
     srv = Server(max_jobs=4)
