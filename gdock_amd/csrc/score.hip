@@ -117,10 +117,11 @@ __device__ __forceinline__ int lig_index(int base, int k) {
          static_cast<int>(threadIdx.x & 63);
 }
 
-// Axis-aligned box of a wave's ligand atoms (FP32, origin frame; empty when
-// the wave holds none) and the test that a receptor group's bounding sphere
-// is farther from it than every cutoff.
-struct WaveBox { float x0, y0, z0, x1, y1, z1; };
+// Axis-aligned boxes of a wave's K ligand blocks (FP32, origin frame; empty
+// when a block holds no atom), kept in LDS as 8 floats per block, and the
+// test that a receptor group's bounding sphere is farther from a box than
+// every cutoff.
+constexpr int kBoxFloats = 8;
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -134,22 +135,25 @@ __device__ __forceinline__ float wave_max(float v) {
   return uniform(v);
 }
 
-__device__ __forceinline__ bool beyond_reach(const float4 S, const WaveBox& b) {
-  const float dx = fmaxf(fmaxf(b.x0 - S.x, S.x - b.x1), 0.0f);
-  const float dy = fmaxf(fmaxf(b.y0 - S.y, S.y - b.y1), 0.0f);
-  const float dz = fmaxf(fmaxf(b.z0 - S.z, S.z - b.z1), 0.0f);
+__device__ __forceinline__ bool beyond_reach(const float4 S, const float* __restrict__ b) {
+  const float dx = fmaxf(fmaxf(b[0] - S.x, S.x - b[3]), 0.0f);
+  const float dy = fmaxf(fmaxf(b[1] - S.y, S.y - b[4]), 0.0f);
+  const float dz = fmaxf(fmaxf(b[2] - S.z, S.z - b[5]), 0.0f);
   const float reach = S.w + kCullReach;
   return dx * dx + dy * dy + dz * dz > reach * reach;
 }
 
-// Bit i set: receptor group g0 / kRecGroup + i (i < 16) has atoms within
-// reach of the wave's ligand atoms.  Lanes 0-15 test one group each.
-__device__ __forceinline__ uint32_t groups_in_reach(CxRef cx, int g0, const WaveBox& b) {
+// Receptor groups of the tile at slot g0 (16 groups of kRecGroup) against
+// the wave's ligand blocks 4 h .. 4 h + 3: lane 4 i + b tests group i
+// against block 4 h + b, so bits 4 i .. 4 i + 3 of the ballot say which of
+// those blocks group i can reach.
+template <int K>
+__device__ __forceinline__ uint64_t blocks_in_reach(CxRef cx, int g0, const float* __restrict__ boxes, int h) {
   const int lane = threadIdx.x & 63;
-  const int g = g0 / kRecGroup + lane;
+  const int g = g0 / kRecGroup + (lane >> 2), b = 4 * h + (lane & 3);
   bool near = false;
-  if (lane < 64 / kRecGroup && g < cx.n_rec / kRecGroup) near = !beyond_reach(cx.rec_group[g], b);
-  return static_cast<uint32_t>(__builtin_amdgcn_ballot_w64(near));
+  if (b < K && g < cx.n_rec / kRecGroup) near = !beyond_reach(cx.rec_group[g], boxes + kBoxFloats * b);
+  return __builtin_amdgcn_ballot_w64(near);
 }
 
 template <int K>
@@ -211,13 +215,14 @@ __device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const 
 struct SettleEvent { int32_t j, pad; uint64_t mask; };  // first receptor slot of a group, lanes
 constexpr int kEventsPerWave = 128;
 
-// Fast path of one receptor atom against the lane's K ligand atoms; returns
-// this wave's burial count of the receptor atom and flags band pairs.
+// Fast path of one receptor atom against the lane's ligand atom in slot k;
+// returns this wave's burial count of the receptor atom from that slot and
+// flags band pairs.
 template <int K>
-__device__ __forceinline__ int32_t receptor_atom(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int32_t& ambv) {
+__device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int k,
+                                             int32_t& ambv) {
   int32_t cj = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  {
     const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
     const bool md = d2 <= B.lo64, mv0 = d2 <= B.lo144, me0 = d2 <= B.lo225, far1 = !(d2 <= B.lo1);
     const bool amb = (far1 != !(d2 < B.hi1)) | (md != (d2 < B.hi64)) | (mv0 != (d2 < B.hi144)) |
@@ -420,6 +425,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
                                            int32_t* s_overflow, const PosePart& pp) {
   double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
   SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
+  float* s_box = reinterpret_cast<float*>(s_ev + (kScoreThreads / 64) * kEventsPerWave);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SettleEvent* my_ev = s_ev + wave * kEventsPerWave;
   const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
@@ -461,28 +467,35 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       }
       a.eps[k] = 0.0f; a.n2e[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
-    WaveBox box;
-    {
-      float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
+    float* my_box = s_box + wave * (kMaxAtomsPerLane * kBoxFloats);
+    __syncthreads();  // the previous chunk's boxes are no longer read
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (lig_index<K>(base, k) < cx.n_lig) {
-          x0 = fminf(x0, a.x[k]); y0 = fminf(y0, a.y[k]); z0 = fminf(z0, a.z[k]);
-          x1 = fmaxf(x1, a.x[k]); y1 = fmaxf(y1, a.y[k]); z1 = fmaxf(z1, a.z[k]);
-        }
-      box = WaveBox{wave_min(x0), wave_min(y0), wave_min(z0), wave_max(x1), wave_max(y1), wave_max(z1)};
+    for (int k = 0; k < K; ++k) {
+      const bool in = lig_index<K>(base, k) < cx.n_lig;
+      const float x0 = wave_min(in ? a.x[k] : INFINITY), y0 = wave_min(in ? a.y[k] : INFINITY),
+                  z0 = wave_min(in ? a.z[k] : INFINITY), x1 = wave_max(in ? a.x[k] : -INFINITY),
+                  y1 = wave_max(in ? a.y[k] : -INFINITY), z1 = wave_max(in ? a.z[k] : -INFINITY);
+      if (lane == 0) {
+        float* b = my_box + kBoxFloats * k;
+        b[0] = x0; b[1] = y0; b[2] = z0; b[3] = x1; b[4] = y1; b[5] = z1;
+      }
     }
+    __syncthreads();
     int nev = 0;
     int cls = -1, cend = 0;
     for (int g0 = pp.r0; g0 < pp.r1; g0 += 64) {
       const int g1 = min(g0 + 64, pp.r1);
-      // Groups beyond every cutoff from all of this wave's atoms add nothing
-      // and hold no band pair: skipped (the whole tile when none is near).
-      const uint32_t keep = groups_in_reach(cx, g0, box);
-      if (keep == 0u) continue;
+      // A receptor group beyond every cutoff from all atoms of a ligand block
+      // adds nothing and holds no band pair with it: that block skips the
+      // group (the whole tile when no block is near).
+      const uint64_t keep0 = blocks_in_reach<K>(cx, g0, my_box, 0);
+      const uint64_t keep1 = K > 4 ? blocks_in_reach<K>(cx, g0, my_box, 1) : 0ull;
+      if ((keep0 | keep1) == 0ull) continue;
       int32_t rc = 0;  // lane t: burial count of receptor slot g0 + t from this wave
       for (int j = g0; j < g1; j += kRecGroup) {
-        if (!((keep >> ((j - g0) / kRecGroup)) & 1u)) continue;
+        const int sh = 4 * ((j - g0) / kRecGroup);
+        const uint32_t mk = static_cast<uint32_t>((keep0 >> sh) & 15u) | (static_cast<uint32_t>((keep1 >> sh) & 15u) << 4);
+        if (mk == 0u) continue;
         if (j >= cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
           do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
@@ -496,14 +509,19 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         }
         const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
         int32_t ambv = 0;
+        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!((mk >> k) & 1u)) continue;  // wave-uniform
+          c0 += pair_fast<K>(r0, B, a, k, ambv);
+          c1 += pair_fast<K>(r1, B, a, k, ambv);
+          c2 += pair_fast<K>(r2, B, a, k, ambv);
+          c3 += pair_fast<K>(r3, B, a, k, ambv);
+        }
         const int t = lane - (j - g0);
-        const int32_t c0 = receptor_atom<K>(r0, B, a, ambv);
         rc = t == 0 ? c0 : rc;
-        const int32_t c1 = receptor_atom<K>(r1, B, a, ambv);
         rc = t == 1 ? c1 : rc;
-        const int32_t c2 = receptor_atom<K>(r2, B, a, ambv);
         rc = t == 2 ? c2 : rc;
-        const int32_t c3 = receptor_atom<K>(r3, B, a, ambv);
         rc = t == 3 ? c3 : rc;
         const uint64_t amask = __builtin_amdgcn_ballot_w64(ambv != 0);
         if (amask != 0ull) {  // rare: this group holds band pairs
@@ -758,7 +776,8 @@ __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
 
 size_t score_lds_bytes(const DevComplex& cx) {
   return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
-         (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent);
+         (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent) +
+         (kScoreThreads / 64) * kMaxAtomsPerLane * kBoxFloats * sizeof(float);
 }
 
 template <int K>
