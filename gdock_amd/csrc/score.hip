@@ -28,6 +28,11 @@
 
 #include "common.h"
 
+// The fast scoring kernels run at 4 waves per SIMD (at most 128 VGPRs): the
+// pair loop is VALU-bound and needs the fourth wave to hide its latencies.
+// The few values that do not fit are per-chunk pose-transform temporaries,
+// spilled and reloaded once per ligand chunk, outside the pair loop.
+
 namespace gdock {
 
 // Complex descriptors and receptor tables are read-only for a whole launch;
@@ -158,7 +163,9 @@ __device__ __forceinline__ uint64_t blocks_in_reach(CxRef cx, int g0, const floa
 
 template <int K>
 struct LaneAtoms {
-  float x[K], y[K], z[K], eps[K], n2e[K], ir6[K], acv[K], ace[K];  // n2e = -2 eps
+  // acv: sum of rho (rho - 2) over the current receptor class within the
+  // tile (times eps when folded into FP64); ace: sum of q_r / d^2.
+  float x[K], y[K], z[K], eps[K], ir6[K], acv[K], ace[K];
   int32_t cnt[K];
 };
 
@@ -168,16 +175,23 @@ __device__ __forceinline__ float pair_d2(float rx, float ry, float rz, float lx,
 }
 
 // Soft-core LJ (fitness.rs:57-70): the pair's vdw term is eps (rho^2 - 2 rho)
-// = t rho with t = eps rho - 2 eps, rho = R^6 / (d^6 + R^6/2); rho and 1/d^2
-// come from one reciprocal, 1 / (d^2 (d^6/R^6 + 1/2)).
-__device__ __forceinline__ void pair_terms(float d2, float ir6, float eps, float n2e, float& rho, float& t,
-                                           float& id2) {
+// = eps u rho with u = rho - 2, rho = R^6 / (d^6 + R^6/2); rho and 1/d^2
+// come from one reciprocal, 1 / (d^2 (d^6/R^6 + 1/2)).  The pair loop sums
+// u rho over the receptor atoms of one (rmin2, eps) class and multiplies by
+// eps once per class and tile.
+__device__ __forceinline__ void pair_terms(float d2, float ir6, float& rho, float& u, float& id2) {
   const float d6 = (d2 * d2) * d2;
   const float den = __builtin_fmaf(d6, ir6, 0.5f);
   const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, d2, 1.0e-30f));
   rho = rp * d2;
   id2 = rp * den;
-  t = __builtin_fmaf(eps, rho, n2e);
+  u = rho - 2.0f;
+}
+
+// The vdw term of a pair as the pair loop counts it: eps (u rho), in FP64
+// from the FP32 factors.
+__device__ __forceinline__ double vdw_term(float eps, float u, float rho) {
+  return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho));
 }
 
 __device__ __forceinline__ bool in_band(float d2, const Bands& B) {
@@ -230,9 +244,9 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
     ambv = amb ? 1 : ambv;
     const bool mv = mv0;  // pairs of equal serial are taken out afterwards (collision_correction)
     const bool me = me0 && far1;
-    float rho, t, id2;
-    pair_terms(d2, a.ir6[k], a.eps[k], a.n2e[k], rho, t, id2);
-    a.acv[k] = __builtin_fmaf(mv ? t : 0.0f, rho, a.acv[k]);
+    float rho, u, id2;
+    pair_terms(d2, a.ir6[k], rho, u, id2);
+    a.acv[k] = __builtin_fmaf(mv ? u : 0.0f, rho, a.acv[k]);
     a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
     a.cnt[k] += md ? 1 : 0;
     cj += __popcll(__builtin_amdgcn_ballot_w64(d2 <= B.lo64));
@@ -274,9 +288,9 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
       exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
       const bool f144 = d2 <= B.lo144;
       const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
-      float rho, t, id2;
-      pair_terms(d2, a.ir6[k], a.eps[k], a.n2e[k], rho, t, id2);
-      if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? t : -t, rho, a.acv[k]);
+      float rho, u, id2;
+      pair_terms(d2, a.ir6[k], rho, u, id2);
+      if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? u : -u, rho, a.acv[k]);
       if (x225 != f225) a.ace[k] = __builtin_fmaf(x225 ? id2 : -id2, r.w, a.ace[k]);
       const int dc = static_cast<int>(x64) - static_cast<int>(f64);
       if (dc) {
@@ -337,9 +351,9 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
         exact_flags_at(cx, px, py, pz, l, j, rser, x64, x144, x225);
         const bool f144 = d2 <= B.lo144;
         const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
-        float rho, t, id2;
-        pair_terms(d2, ir6, eps, -2.0f * eps, rho, t, id2);
-        if (x144 != f144) sum_v += static_cast<double>(x144 ? t : -t) * static_cast<double>(rho);
+        float rho, u, id2;
+        pair_terms(d2, ir6, rho, u, id2);
+        if (x144 != f144) sum_v += x144 ? vdw_term(eps, u, rho) : -vdw_term(eps, u, rho);
         if (x225 != f225)
           sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
         const int dc = static_cast<int>(x64) - static_cast<int>(f64);
@@ -465,7 +479,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       } else {
         a.x[k] = a.y[k] = a.z[k] = kPadCoord;
       }
-      a.eps[k] = 0.0f; a.n2e[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
+      a.eps[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
     float* my_box = s_box + wave * (kMaxAtomsPerLane * kBoxFloats);
     __syncthreads();  // the previous chunk's boxes are no longer read
@@ -496,15 +510,14 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         const int sh = 4 * ((j - g0) / kRecGroup);
         const uint32_t mk = static_cast<uint32_t>((keep0 >> sh) & 15u) | (static_cast<uint32_t>((keep1 >> sh) & 15u) << 4);
         if (mk == 0u) continue;
-        if (j >= cend) {  // next (rmin2, epsilon) class: per-lane LJ factors
+        if (j >= cend) {  // next (rmin2, epsilon) class: fold the vdw sum, per-lane LJ factors
           do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
+            sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
+            a.acv[k] = 0.0f;
             const int l = lig_index<K>(base, k);
-            if (l < cx.n_lig) {
-              class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
-              a.n2e[k] = -2.0f * a.eps[k];
-            }
+            if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
           }
         }
         const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
@@ -543,7 +556,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       for (int k = 0; k < K; ++k) {  // fold the tile's FP32 partial sums into FP64
         const int l = lig_index<K>(base, k);
         if (l < cx.n_lig) {
-          sum_v += static_cast<double>(a.acv[k]);
+          sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
           sum_e += static_cast<double>(a.ace[k]) * (kCoulomb * cx.lig_q[l]);
         }
         a.acv[k] = 0.0f; a.ace[k] = 0.0f;
@@ -586,10 +599,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     const float4 r = cx.rec_xyzq[P.x];
     const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
     if (d2 <= B.lo144 && !in_band(d2, B)) {
-      float ir6, eps, rho, t, id2;
+      float ir6, eps, rho, u, id2;
       class_factors(cx, P.z, P.y, ir6, eps);
-      pair_terms(d2, ir6, eps, -2.0f * eps, rho, t, id2);
-      sum_v -= static_cast<double>(t) * static_cast<double>(rho);
+      pair_terms(d2, ir6, rho, u, id2);
+      sum_v -= vdw_term(eps, u, rho);
     }
   }
   for (int j = pp.r0 + tid; j < pp.r1; j += kScoreThreads) {  // fitness.rs:141-162
@@ -665,7 +678,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
 }
 
 template <int K, bool EXACT>
-__global__ __launch_bounds__(kScoreThreads) void score_poses_kernel(
+__global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4, EXACT ? 8 : 4))) void score_poses_kernel(
     const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
     const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
     int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
@@ -726,7 +739,7 @@ __device__ __forceinline__ PosePart part_of(CxRef cx, int part, int pose_row) {
 // device memory so that one captured launch serves any number of items up to
 // the capacity.  The last workgroup to leave resets the counter and the exit
 // count, so the next launch starts from zero with no memset.
-__global__ __launch_bounds__(kScoreThreads) void score_batch_kernel(
+__global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void score_batch_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     int32_t* __restrict__ counter, int32_t* __restrict__ exit_count, const double* __restrict__ poses,
     const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
