@@ -111,7 +111,39 @@ __device__ __forceinline__ void guard_band(double T, double X, float& lo, float&
 // lies inside a band, or at d <= ~1 A, flags the receptor atom; settle_pairs()
 // then re-decides those pairs in FP64 exactly as the reference does and adds
 // the difference to what the fast path accumulated.
-struct Bands { float lo1, hi1, lo64, hi64, lo144, hi144, lo225, hi225; };
+//
+// Bands are kept as integers: for a non-negative float, its bit pattern
+// orders like its value.  c_T = bits(lo_T) + 1, so bits(d2) - c_T borrows
+// exactly when d2 <= lo_T (the FP32 decision), and the same difference, as
+// an unsigned number below w, puts d2 inside the band of T.  w is the widest
+// band of the four in units in the last place; narrower bands are widened to
+// it, which only sends a few more pairs to the FP64 decision.
+struct Bands {
+  float lo1, lo64, lo144, lo225;
+  uint32_t c1, c64, c144, c225, w;
+};
+
+__device__ __forceinline__ Bands make_bands(double X) {
+  Bands B;
+  float h1, h64, h144, h225;
+  guard_band(1.0, X, B.lo1, h1);
+  guard_band(64.0, X, B.lo64, h64);
+  guard_band(144.0, X, B.lo144, h144);
+  guard_band(225.0, X, B.lo225, h225);
+  B.c1 = __float_as_uint(B.lo1) + 1u;
+  B.c64 = __float_as_uint(B.lo64) + 1u;
+  B.c144 = __float_as_uint(B.lo144) + 1u;
+  B.c225 = __float_as_uint(B.lo225) + 1u;
+  B.w = max(max(__float_as_uint(h1) - B.c1, __float_as_uint(h64) - B.c64),
+            max(__float_as_uint(h144) - B.c144, __float_as_uint(h225) - B.c225));
+  return B;
+}
+
+// The smallest of bits(d2) - c_T over the four cutoffs (unsigned).
+__device__ __forceinline__ uint32_t band_offset(float d2, const Bands& B) {
+  const uint32_t b = __float_as_uint(d2);
+  return min(min(b - B.c1, b - B.c64), min(b - B.c144, b - B.c225));
+}
 
 // Ligand atom held in slot k of this lane for the chunk starting at `base`:
 // wave w holds the K consecutive blocks of kLigBlock atoms (each block is
@@ -194,10 +226,7 @@ __device__ __forceinline__ double vdw_term(float eps, float u, float rho) {
   return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho));
 }
 
-__device__ __forceinline__ bool in_band(float d2, const Bands& B) {
-  return ((d2 < B.hi1) != (d2 <= B.lo1)) | ((d2 < B.hi64) != (d2 <= B.lo64)) |
-         ((d2 < B.hi144) != (d2 <= B.lo144)) | ((d2 < B.hi225) != (d2 <= B.lo225));
-}
+__device__ __forceinline__ bool in_band(float d2, const Bands& B) { return band_offset(d2, B) < B.w; }
 
 // Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
 // 1/R^6 and sqrt(eps_r eps_l) in FP64 (fitness.rs:58-63), rounded once.
@@ -234,27 +263,38 @@ constexpr int kEventsPerWave = 128;
 // flags band pairs.
 template <int K>
 __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int k,
-                                             int32_t& ambv) {
-  int32_t cj = 0;
-  {
-    const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
-    const bool md = d2 <= B.lo64, mv0 = d2 <= B.lo144, me0 = d2 <= B.lo225, far1 = !(d2 <= B.lo1);
-    const bool amb = (far1 != !(d2 < B.hi1)) | (md != (d2 < B.hi64)) | (mv0 != (d2 < B.hi144)) |
-                     (me0 != (d2 < B.hi225));
-    ambv = amb ? 1 : ambv;
-    const bool mv = mv0;  // pairs of equal serial are taken out afterwards (collision_correction)
-    const bool me = me0 && far1;
-    float rho, u, id2;
-    pair_terms(d2, a.ir6[k], rho, u, id2);
-    a.acv[k] = __builtin_fmaf(mv ? u : 0.0f, rho, a.acv[k]);
-    a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
-    a.cnt[k] += md ? 1 : 0;
-    cj += __popcll(__builtin_amdgcn_ballot_w64(d2 <= B.lo64));
-    // Pin this pair's results here (an empty asm, no instruction): otherwise
-    // the compiler sinks the math below the group's rare band-pair branch and
-    // keeps every pair's lane masks alive across it, spilling SGPRs.
-    asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+v"(ambv));
-  }
+                                             uint64_t& ambm) {
+  const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
+  // FP32 decisions below each band, as the borrows of bits(d2) - c_T (pairs
+  // of equal serial are included in the vdw sum and taken out afterwards, see
+  // collision_correction), and the band test on the same differences.
+  const uint32_t bd = __float_as_uint(d2);
+  uint32_t q1, q64, q144, q225;
+  const bool le1 = __builtin_sub_overflow(bd, B.c1, &q1);
+  // d <= 8 A as a lane mask (the borrow): it is the carry-in that counts the
+  // ligand atom's burial and, popcounted, the receptor atom's.  Written out
+  // because the compiler would first turn the borrow into a 0/1 value to
+  // take its ballot.  The s_nop gives the 2 wait states a VALU-written SGPR
+  // needs before a VALU reads it.
+  uint64_t md, carry;
+  asm("v_sub_co_u32_e64 %0, %1, %4, %5\n\t"
+      "s_nop 1\n\t"
+      "v_addc_co_u32_e64 %2, %3, 0, %2, %1"
+      : "=&v"(q64), "=&s"(md), "+v"(a.cnt[k]), "=&s"(carry)
+      : "v"(bd), "s"(B.c64));
+  const bool mv = __builtin_sub_overflow(bd, B.c144, &q144);
+  const bool me0 = __builtin_sub_overflow(bd, B.c225, &q225);
+  ambm |= __builtin_amdgcn_ballot_w64(min(min(q1, q64), min(q144, q225)) < B.w);
+  const bool me = me0 && !le1;
+  float rho, u, id2;
+  pair_terms(d2, a.ir6[k], rho, u, id2);
+  a.acv[k] = __builtin_fmaf(mv ? u : 0.0f, rho, a.acv[k]);
+  a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
+  const int32_t cj = __popcll(md);
+  // Pin this pair's results here (an empty asm, no instruction): otherwise
+  // the compiler sinks the math below the group's rare band-pair branch and
+  // keeps every pair's lane masks alive across it, spilling SGPRs.
+  asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+s"(ambm));
   return cj;
 }
 
@@ -453,11 +493,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     pc = load_pose(pose_rec);
     X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
   }
-  Bands B;
-  guard_band(1.0, X, B.lo1, B.hi1);
-  guard_band(64.0, X, B.lo64, B.hi64);
-  guard_band(144.0, X, B.lo144, B.hi144);
-  guard_band(225.0, X, B.lo225, B.hi225);
+  const Bands B = make_bands(X);
 
   for (int i = pp.r0 + tid; i < pp.r1; i += kScoreThreads) s_rcnt[i] = 0;
   if (tid == 0) *s_overflow = 0;
@@ -521,22 +557,22 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           }
         }
         const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
-        int32_t ambv = 0;
+        uint64_t ambm = 0;  // lanes that saw a band pair in this group
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (!((mk >> k) & 1u)) continue;  // wave-uniform
-          c0 += pair_fast<K>(r0, B, a, k, ambv);
-          c1 += pair_fast<K>(r1, B, a, k, ambv);
-          c2 += pair_fast<K>(r2, B, a, k, ambv);
-          c3 += pair_fast<K>(r3, B, a, k, ambv);
+          c0 += pair_fast<K>(r0, B, a, k, ambm);
+          c1 += pair_fast<K>(r1, B, a, k, ambm);
+          c2 += pair_fast<K>(r2, B, a, k, ambm);
+          c3 += pair_fast<K>(r3, B, a, k, ambm);
         }
         const int t = lane - (j - g0);
         rc = t == 0 ? c0 : rc;
         rc = t == 1 ? c1 : rc;
         rc = t == 2 ? c2 : rc;
         rc = t == 3 ? c3 : rc;
-        const uint64_t amask = __builtin_amdgcn_ballot_w64(ambv != 0);
+        const uint64_t amask = ambm;
         if (amask != 0ull) {  // rare: this group holds band pairs
           if (EXACT) {
             band_pairs<K, EXACT>(cx, pc, coords, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
