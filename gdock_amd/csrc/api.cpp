@@ -257,23 +257,18 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     if (hit != lig_by_serial.end())
       for (int32_t l : hit->second) coll.push_back(make_int4(j, l, cls, 0));
   }
-  // bounding spheres of the receptor groups, from the FP32 coordinates the kernel uses
-  std::vector<float4> rgroup(n_pad / kRecGroup);
+  // bounding boxes of the receptor groups (lo, hi), from the FP32 coordinates
+  // the kernel uses; a group of padding slots only gets an empty box
+  std::vector<float4> rgroup(2 * (n_pad / kRecGroup));
   for (int32_t g = 0; g < n_pad / kRecGroup; ++g) {
-    double c[3] = {0, 0, 0};
-    int nreal = 0;
-    for (int32_t j = g * kRecGroup; j < (g + 1) * kRecGroup; ++j)
-      if (perm[j] >= 0) { c[0] += xyzq[j].x; c[1] += xyzq[j].y; c[2] += xyzq[j].z; ++nreal; }
-    const float cf[3] = {static_cast<float>(c[0] / nreal), static_cast<float>(c[1] / nreal),
-                         static_cast<float>(c[2] / nreal)};
-    double r = 0.0;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int32_t j = g * kRecGroup; j < (g + 1) * kRecGroup; ++j)
       if (perm[j] >= 0) {
-        const double dx = static_cast<double>(xyzq[j].x) - cf[0], dy = static_cast<double>(xyzq[j].y) - cf[1],
-                     dz = static_cast<double>(xyzq[j].z) - cf[2];
-        r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+        const float p[3] = {xyzq[j].x, xyzq[j].y, xyzq[j].z};
+        for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], p[d]); hi[d] = std::max(hi[d], p[d]); }
       }
-    rgroup[g] = make_float4(cf[0], cf[1], cf[2], std::nextafter(static_cast<float>(r * (1.0 + 1e-6) + 1e-4), 1e30f));
+    rgroup[2 * g] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    rgroup[2 * g + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
   }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);

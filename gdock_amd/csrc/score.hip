@@ -156,8 +156,8 @@ __device__ __forceinline__ int lig_index(int base, int k) {
 
 // Axis-aligned boxes of a wave's K ligand blocks (FP32, origin frame; empty
 // when a block holds no atom), kept in LDS as 8 floats per block, and the
-// test that a receptor group's bounding sphere is farther from a box than
-// every cutoff.
+// test that a receptor group's box is farther from a block's box than every
+// cutoff.
 constexpr int kBoxFloats = 8;
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -172,12 +172,14 @@ __device__ __forceinline__ float wave_max(float v) {
   return uniform(v);
 }
 
-__device__ __forceinline__ bool beyond_reach(const float4 S, const float* __restrict__ b) {
-  const float dx = fmaxf(fmaxf(b[0] - S.x, S.x - b[3]), 0.0f);
-  const float dy = fmaxf(fmaxf(b[1] - S.y, S.y - b[4]), 0.0f);
-  const float dz = fmaxf(fmaxf(b[2] - S.z, S.z - b[5]), 0.0f);
-  const float reach = S.w + kCullReach;
-  return dx * dx + dy * dy + dz * dz > reach * reach;
+// The gap between two boxes along each axis, and so the distance between any
+// two of their points, is at least the box-to-box distance; FP32 rounding of
+// it is far below the 0.01 A margin of kCullReach over the 15 A cutoff.
+__device__ __forceinline__ bool beyond_reach(const float4 lo, const float4 hi, const float* __restrict__ b) {
+  const float dx = fmaxf(fmaxf(b[0] - hi.x, lo.x - b[3]), 0.0f);
+  const float dy = fmaxf(fmaxf(b[1] - hi.y, lo.y - b[4]), 0.0f);
+  const float dz = fmaxf(fmaxf(b[2] - hi.z, lo.z - b[5]), 0.0f);
+  return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
 }
 
 // Receptor groups of the tile at slot g0 (16 groups of kRecGroup) against
@@ -189,7 +191,8 @@ __device__ __forceinline__ uint64_t blocks_in_reach(CxRef cx, int g0, const floa
   const int lane = threadIdx.x & 63;
   const int g = g0 / kRecGroup + (lane >> 2), b = 4 * h + (lane & 3);
   bool near = false;
-  if (b < K && g < cx.n_rec / kRecGroup) near = !beyond_reach(cx.rec_group[g], boxes + kBoxFloats * b);
+  if (b < K && g < cx.n_rec / kRecGroup)
+    near = !beyond_reach(cx.rec_group[2 * g], cx.rec_group[2 * g + 1], boxes + kBoxFloats * b);
   return __builtin_amdgcn_ballot_w64(near);
 }
 
