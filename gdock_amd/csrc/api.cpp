@@ -257,19 +257,6 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     if (hit != lig_by_serial.end())
       for (int32_t l : hit->second) coll.push_back(make_int4(j, l, cls, 0));
   }
-  // bounding boxes of the receptor groups (lo, hi), from the FP32 coordinates
-  // the kernel uses; a group of padding slots only gets an empty box
-  std::vector<float4> rgroup(2 * (n_pad / kRecGroup));
-  for (int32_t g = 0; g < n_pad / kRecGroup; ++g) {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int32_t j = g * kRecGroup; j < (g + 1) * kRecGroup; ++j)
-      if (perm[j] >= 0) {
-        const float p[3] = {xyzq[j].x, xyzq[j].y, xyzq[j].z};
-        for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], p[d]); hi[d] = std::max(hi[d], p[d]); }
-      }
-    rgroup[2 * g] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-    rgroup[2 * g + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
-  }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);
   double lig_r = 0.0;
@@ -303,7 +290,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   const size_t o_xyzq = L.add<float4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
                o_rz = L.add<double>(n_pad), o_rasp = L.add<double>(n_pad), o_rrad = L.add<double>(n_pad),
                o_meta = L.add<int2>(n_pad), o_coll = L.add<int4>(coll.size()),
-               o_rgrp = L.add<float4>(rgroup.size()), o_lorig = L.add<int32_t>(n_lig), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
+               o_lorig = L.add<int32_t>(n_lig), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
                o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
@@ -319,7 +306,6 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_rasp, rasp.data(), n_pad * 8); put(o_rrad, rrad.data(), n_pad * 8);
   put(o_meta, meta.data(), n_pad * sizeof(int2));
   put(o_coll, coll.data(), coll.size() * sizeof(int4));
-  put(o_rgrp, rgroup.data(), rgroup.size() * sizeof(float4));
   put(o_lorig, lig_perm.data(), lig_perm.size() * 4);
   put(o_cs, class_start.data(), (nc + 1) * 4); put(o_crm, class_rmin.data(), nc * 8); put(o_ceps, class_eps.data(), nc * 8);
   put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
@@ -342,7 +328,6 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.rec_radius = reinterpret_cast<const double*>(m + o_rrad);
   d.rec_meta = reinterpret_cast<const int2*>(m + o_meta);
   d.coll_pairs = reinterpret_cast<const int4*>(m + o_coll);
-  d.rec_group = reinterpret_cast<const float4*>(m + o_rgrp);
   d.lig_orig = reinterpret_cast<const int32_t*>(m + o_lorig);
   d.class_start = reinterpret_cast<const int32_t*>(m + o_cs);
   d.class_rmin2 = reinterpret_cast<const double*>(m + o_crm);

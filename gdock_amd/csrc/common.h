@@ -46,7 +46,6 @@ struct DevComplex {
   const double* rec_radius;
   const int2* rec_meta;        // {serial, 1 if the serial also occurs in the ligand}
   const int4* coll_pairs;      // {receptor slot, ligand atom, receptor class, 0}: equal serials
-  const float4* rec_group;     // per kRecGroup slots: bounding box lo, hi (FP32, origin frame; .w unused)
   const int32_t* class_start;  // n_class + 1
   const double* class_rmin2;
   const double* class_eps;

@@ -172,28 +172,27 @@ __device__ __forceinline__ float wave_max(float v) {
   return uniform(v);
 }
 
-// The gap between two boxes along each axis, and so the distance between any
-// two of their points, is at least the box-to-box distance; FP32 rounding of
-// it is far below the 0.01 A margin of kCullReach over the 15 A cutoff.
-__device__ __forceinline__ bool beyond_reach(const float4 lo, const float4 hi, const float* __restrict__ b) {
-  const float dx = fmaxf(fmaxf(b[0] - hi.x, lo.x - b[3]), 0.0f);
-  const float dy = fmaxf(fmaxf(b[1] - hi.y, lo.y - b[4]), 0.0f);
-  const float dz = fmaxf(fmaxf(b[2] - hi.z, lo.z - b[5]), 0.0f);
+// The distance from a point to any point of a box is at least the
+// point-to-box distance; FP32 rounding of it is far below the 0.01 A margin
+// of kCullReach over the 15 A cutoff.
+__device__ __forceinline__ bool beyond_reach(const float4 r, const float* __restrict__ b) {
+  const float dx = fmaxf(fmaxf(b[0] - r.x, r.x - b[3]), 0.0f);
+  const float dy = fmaxf(fmaxf(b[1] - r.y, r.y - b[4]), 0.0f);
+  const float dz = fmaxf(fmaxf(b[2] - r.z, r.z - b[5]), 0.0f);
   return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
 }
 
-// Receptor groups of the tile at slot g0 (16 groups of kRecGroup) against
-// the wave's ligand blocks 4 h .. 4 h + 3: lane 4 i + b tests group i
-// against block 4 h + b, so bits 4 i .. 4 i + 3 of the ballot say which of
-// those blocks group i can reach.
+// Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
+// t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
+// slot against every block; padding slots sit far from every block).
 template <int K>
-__device__ __forceinline__ uint64_t blocks_in_reach(CxRef cx, int g0, const float* __restrict__ boxes, int h) {
+__device__ __forceinline__ void atoms_in_reach(CxRef cx, int g0, int g1, const float* __restrict__ boxes,
+                                               uint64_t (&keep)[K]) {
   const int lane = threadIdx.x & 63;
-  const int g = g0 / kRecGroup + (lane >> 2), b = 4 * h + (lane & 3);
-  bool near = false;
-  if (b < K && g < cx.n_rec / kRecGroup)
-    near = !beyond_reach(cx.rec_group[2 * g], cx.rec_group[2 * g + 1], boxes + kBoxFloats * b);
-  return __builtin_amdgcn_ballot_w64(near);
+  const bool valid = g0 + lane < g1;
+  const float4 r = cx.rec_xyzq[valid ? g0 + lane : g0];
+#pragma unroll
+  for (int k = 0; k < K; ++k) keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(r, boxes + kBoxFloats * k));
 }
 
 template <int K>
@@ -538,17 +537,19 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     int cls = -1, cend = 0;
     for (int g0 = pp.r0; g0 < pp.r1; g0 += 64) {
       const int g1 = min(g0 + 64, pp.r1);
-      // A receptor group beyond every cutoff from all atoms of a ligand block
+      // A receptor atom beyond every cutoff from all atoms of a ligand block
       // adds nothing and holds no band pair with it: that block skips the
-      // group (the whole tile when no block is near).
-      const uint64_t keep0 = blocks_in_reach<K>(cx, g0, my_box, 0);
-      const uint64_t keep1 = K > 4 ? blocks_in_reach<K>(cx, g0, my_box, 1) : 0ull;
-      if ((keep0 | keep1) == 0ull) continue;
+      // atom (a group when no block is near, the tile when nothing is).
+      uint64_t keep[K];
+      atoms_in_reach<K>(cx, g0, g1, my_box, keep);
+      uint64_t any_keep = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) any_keep |= keep[k];
+      if (any_keep == 0ull) continue;
       int32_t rc = 0;  // lane t: burial count of receptor slot g0 + t from this wave
       for (int j = g0; j < g1; j += kRecGroup) {
-        const int sh = 4 * ((j - g0) / kRecGroup);
-        const uint32_t mk = static_cast<uint32_t>((keep0 >> sh) & 15u) | (static_cast<uint32_t>((keep1 >> sh) & 15u) << 4);
-        if (mk == 0u) continue;
+        const int sh = j - g0;
+        if (((any_keep >> sh) & 15u) == 0u) continue;
         if (j >= cend) {  // next (rmin2, epsilon) class: fold the vdw sum, per-lane LJ factors
           do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
@@ -563,12 +564,12 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         uint64_t ambm = 0;  // lanes that saw a band pair in this group
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if (!((mk >> k) & 1u)) continue;  // wave-uniform
-          c0 += pair_fast<K>(r0, B, a, k, ambm);
-          c1 += pair_fast<K>(r1, B, a, k, ambm);
-          c2 += pair_fast<K>(r2, B, a, k, ambm);
-          c3 += pair_fast<K>(r3, B, a, k, ambm);
+        for (int k = 0; k < K; ++k) {  // wave-uniform branches
+          const uint32_t mk = static_cast<uint32_t>(keep[k] >> sh);
+          if (mk & 1u) c0 += pair_fast<K>(r0, B, a, k, ambm);
+          if (mk & 2u) c1 += pair_fast<K>(r1, B, a, k, ambm);
+          if (mk & 4u) c2 += pair_fast<K>(r2, B, a, k, ambm);
+          if (mk & 8u) c3 += pair_fast<K>(r3, B, a, k, ambm);
         }
         const int t = lane - (j - g0);
         rc = t == 0 ? c0 : rc;
