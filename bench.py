@@ -104,7 +104,7 @@ def main():
     lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
     n = args.poses
     genes = synth.random_genes(cfg["seed"], n)
-    poses = np.zeros((n, 12))
+    poses = np.zeros((n, 16))
     _lib.check(_lib.lib().gdock_prepare_poses(_lib.ptr(genes), n, _lib.ptr(poses)), what="prepare_poses")
 
     ctx = default_context(0)

@@ -379,6 +379,12 @@ gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses) 
     p[6] = std::cos(g[0]); p[7] = std::sin(g[0]);
     p[8] = std::cos(g[1]); p[9] = std::sin(g[1]);
     p[10] = std::cos(g[2]); p[11] = std::sin(g[2]);
+    // the bracketed sums of structure.rs:96-101 (no FMA contraction: -ffp-contract=off)
+    const double ca = p[6], sa = p[7], sb = p[9], cg = p[10], sg = p[11];
+    p[12] = (-sa * cg) + ((ca * sb) * sg);
+    p[13] = (sa * sg) + ((ca * sb) * cg);
+    p[14] = (ca * cg) + ((sa * sb) * sg);
+    p[15] = (-ca * sg) + ((sa * sb) * cg);
   }
   return GDOCK_OK;
 }

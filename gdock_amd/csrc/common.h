@@ -20,7 +20,10 @@ constexpr double kD2ElecMaxIncl = 224.99999999999997;    // dist < 15.0 <=> d2 <
 constexpr double kD2RestraintIncl = 49.0;                // dist <= 7.0 <=> d2 <= this
 constexpr double kD2ContactIncl = 25.000000000000004;    // dist <= 5.0 <=> d2 <= this
 
-constexpr int kPoseDoubles = 12;  // genes[6], cos/sin alpha, beta, gamma
+// Pose record: genes[6], cos/sin of alpha, beta, gamma, then the four
+// trigonometric products of the rotation that do not involve an atom
+// coordinate (m01, m02, m11, m12, structure.rs:96-101), all FP64 from the host.
+constexpr int kPoseDoubles = 16;
 constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, fitness
 
 constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup

@@ -52,10 +52,7 @@ __device__ __forceinline__ PoseMat pose_mat(const double* __restrict__ p) {
   PoseMat c;
   c.dx = p[3]; c.dy = p[4]; c.dz = p[5];
   c.ca = p[6]; c.sa = p[7]; c.cb = p[8]; c.sb = p[9]; c.cg = p[10]; c.sg = p[11];
-  c.m01 = (-c.sa * c.cg) + ((c.ca * c.sb) * c.sg);
-  c.m02 = (c.sa * c.sg) + ((c.ca * c.sb) * c.cg);
-  c.m11 = (c.ca * c.cg) + ((c.sa * c.sb) * c.sg);
-  c.m12 = (-c.ca * c.sg) + ((c.sa * c.sb) * c.cg);
+  c.m01 = p[12]; c.m02 = p[13]; c.m11 = p[14]; c.m12 = p[15];
   return c;
 }
 

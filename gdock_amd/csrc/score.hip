@@ -30,8 +30,6 @@
 
 // The fast scoring kernels run at 4 waves per SIMD (at most 128 VGPRs): the
 // pair loop is VALU-bound and needs the fourth wave to hide its latencies.
-// The few values that do not fit are per-chunk pose-transform temporaries,
-// spilled and reloaded once per ligand chunk, outside the pair loop.
 
 namespace gdock {
 
@@ -53,14 +51,13 @@ struct PoseCoef {
   double dx, dy, dz;
 };
 
+// Every field comes from the pose record (the host forms the products), so a
+// workgroup reads its pose with scalar loads and does no FP64 work for it.
 __device__ __forceinline__ PoseCoef load_pose(const double* __restrict__ p) {
   PoseCoef c;
   c.dx = p[3]; c.dy = p[4]; c.dz = p[5];
   c.ca = p[6]; c.sa = p[7]; c.cb = p[8]; c.sb = p[9]; c.cg = p[10]; c.sg = p[11];
-  c.m01 = (-c.sa * c.cg) + ((c.ca * c.sb) * c.sg);
-  c.m02 = (c.sa * c.sg) + ((c.ca * c.sb) * c.cg);
-  c.m11 = (c.ca * c.cg) + ((c.sa * c.sb) * c.sg);
-  c.m12 = (-c.ca * c.sg) + ((c.sa * c.sb) * c.cg);
+  c.m01 = p[12]; c.m02 = p[13]; c.m11 = p[14]; c.m12 = p[15];
   return c;
 }
 
@@ -115,12 +112,15 @@ __device__ __forceinline__ void guard_band(double T, double X, float& lo, float&
 // Bands are kept as integers: for a non-negative float, its bit pattern
 // orders like its value.  c_T = bits(lo_T) + 1, so bits(d2) - c_T borrows
 // exactly when d2 <= lo_T (the FP32 decision), and the same difference, as
-// an unsigned number below w, puts d2 inside the band of T.  w is the widest
-// band of the four in units in the last place; narrower bands are widened to
-// it, which only sends a few more pairs to the FP64 decision.
+// an unsigned number below a width in units in the last place, puts d2
+// inside the band of T.  The 1 A band has a width of its own (w1): near
+// d2 = 1 the coordinate rounding error spans several hundred ulps, ten times
+// more than at the 8, 12 and 15 A cutoffs, which share the widest of their
+// three widths (w); a narrower band widened to w only sends a few more pairs
+// to the FP64 decision.
 struct Bands {
   float lo1, lo64, lo144, lo225;
-  uint32_t c1, c64, c144, c225, w;
+  uint32_t c1, c64, c144, c225, w1, w;
 };
 
 __device__ __forceinline__ Bands make_bands(double X) {
@@ -134,15 +134,9 @@ __device__ __forceinline__ Bands make_bands(double X) {
   B.c64 = __float_as_uint(B.lo64) + 1u;
   B.c144 = __float_as_uint(B.lo144) + 1u;
   B.c225 = __float_as_uint(B.lo225) + 1u;
-  B.w = max(max(__float_as_uint(h1) - B.c1, __float_as_uint(h64) - B.c64),
-            max(__float_as_uint(h144) - B.c144, __float_as_uint(h225) - B.c225));
+  B.w1 = __float_as_uint(h1) - B.c1;
+  B.w = max(__float_as_uint(h64) - B.c64, max(__float_as_uint(h144) - B.c144, __float_as_uint(h225) - B.c225));
   return B;
-}
-
-// The smallest of bits(d2) - c_T over the four cutoffs (unsigned).
-__device__ __forceinline__ uint32_t band_offset(float d2, const Bands& B) {
-  const uint32_t b = __float_as_uint(d2);
-  return min(min(b - B.c1, b - B.c64), min(b - B.c144, b - B.c225));
 }
 
 // Ligand atom held in slot k of this lane for the chunk starting at `base`:
@@ -228,7 +222,11 @@ __device__ __forceinline__ double vdw_term(float eps, float u, float rho) {
   return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho));
 }
 
-__device__ __forceinline__ bool in_band(float d2, const Bands& B) { return band_offset(d2, B) < B.w; }
+// d2 inside one of the four guard bands (the same test as the pair loop's).
+__device__ __forceinline__ bool in_band(float d2, const Bands& B) {
+  const uint32_t b = __float_as_uint(d2);
+  return b - B.c1 < B.w1 || min(b - B.c64, min(b - B.c144, b - B.c225)) < B.w;
+}
 
 // Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
 // 1/R^6 and sqrt(eps_r eps_l) in FP64 (fitness.rs:58-63), rounded once.
@@ -286,7 +284,10 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
       : "v"(bd), "s"(B.c64));
   const bool mv = __builtin_sub_overflow(bd, B.c144, &q144);
   const bool me0 = __builtin_sub_overflow(bd, B.c225, &q225);
-  ambm |= __builtin_amdgcn_ballot_w64(min(min(q1, q64), min(q144, q225)) < B.w);
+  // The two band tests as lane masks straight from the compares (predicate
+  // 36: unsigned less-than), joined by a scalar OR; a ballot of the combined
+  // bool would first widen it to a 0/1 value.
+  ambm |= __builtin_amdgcn_uicmp(q1, B.w1, 36) | __builtin_amdgcn_uicmp(min(q64, min(q144, q225)), B.w, 36);
   const bool me = me0 && !le1;
   float rho, u, id2;
   pair_terms(d2, a.ir6[k], rho, u, id2);
@@ -468,8 +469,8 @@ __device__ __forceinline__ double ligand_desolv(CxRef cx, int l, int32_t cnt) {
   return asp * burial * rad * rad * 4.0 * kPI;
 }
 
-// Scores one pose with the whole workgroup.  pose_rec: the pose record (12
-// doubles) or null; coords: this pose's ligand coordinates (n_lig x 3) when
+// Scores one pose with the whole workgroup.  pose_rec: the pose record
+// (kPoseDoubles doubles) or null; coords: this pose's ligand coordinates (n_lig x 3) when
 // scoring coordinates as given, with `extent` = their largest |x - origin|.
 // o: 6 outputs; *redo is set when the fast variant could not settle every
 // band pair (the exact variant then rescores the pose).

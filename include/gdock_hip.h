@@ -127,8 +127,12 @@ gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double*
                                 const gdock_weights* w, double* out, int32_t* n_satisfied);
 
 /* Device-resident form for callers that keep populations in HBM.  A pose
- * record is 12 doubles: the 6 genes, then cos/sin of alpha, beta, gamma as
- * gdock_prepare_poses computes them on the host (glibc, like the reference).
+ * record is 16 doubles: the 6 genes, then cos/sin of alpha, beta, gamma as
+ * gdock_prepare_poses computes them on the host (glibc, like the reference),
+ * then the rotation's four pure trigonometric products of structure.rs:96-101
+ * (-sin a cos g + cos a sin b sin g, sin a sin g + cos a sin b cos g,
+ * cos a cos g + sin a sin b sin g, -cos a sin g + sin a sin b cos g), FP64
+ * in that operation order.
  * d_poses, d_out (n x 6) and d_nsat (n) are device pointers; the launch is
  * queued on `stream` (hipStream_t, 0 = the context's stream) and not waited on. */
 gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses);
@@ -161,7 +165,7 @@ gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* p
  * n_satisfied out, as gdock_score_poses. */
 gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* genes, const gdock_weights* w,
                                double* out, int32_t* n_satisfied);
-/* Device-resident form: the batch's own HBM buffers (pose records of 12
+/* Device-resident form: the batch's own HBM buffers (pose records of 16
  * doubles as gdock_prepare_poses writes them, outputs n_poses x 6, satisfied
  * counts) and an asynchronous graph replay on `stream`. */
 gdock_status gdock_batch_device_buffers(gdock_batch* b, double** d_poses, double** d_out, int32_t** d_nsat);

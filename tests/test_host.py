@@ -82,12 +82,16 @@ def test_position_ligand_matches_oracle():
 
 def test_prepare_poses_trig_is_glibc():
     g = synth.random_genes(5, 64)
-    poses = np.zeros((64, 12))
+    poses = np.zeros((64, 16))
     assert _lib.lib().gdock_prepare_poses(_lib.ptr(g), 64, _lib.ptr(poses)) == 0
     assert np.array_equal(poses[:, :6], g)
     for i in range(64):
         a, b, c = g[i, :3]
-        assert tuple(poses[i, 6:]) == (math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(c), math.sin(c))
+        ca, sa, cb, sb, cg, sg = math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(c), math.sin(c)
+        assert tuple(poses[i, 6:12]) == (ca, sa, cb, sb, cg, sg)
+        # structure.rs:96-101, the bracketed sums (Python floats: IEEE FP64, no FMA)
+        assert tuple(poses[i, 12:]) == ((-sa * cg) + ((ca * sb) * sg), (sa * sg) + ((ca * sb) * cg),
+                                        (ca * cg) + ((sa * sb) * sg), (-ca * sg) + ((sa * sb) * cg))
     bad = g.copy()
     bad[3, 4] = np.nan
     assert _lib.lib().gdock_prepare_poses(_lib.ptr(bad), 64, _lib.ptr(poses)) == _lib.INVALID_ARG
