@@ -46,6 +46,7 @@ def parse():
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "score_poses_kernel_2000x1500x4096_traffic.txt"
+VALU_FILE = ROOT / "profiles" / "score_poses_kernel_2000x1500x4096_valu.txt"
 
 
 def measured_traffic():
@@ -57,6 +58,19 @@ def measured_traffic():
             if line.startswith("traffic_bytes_per_launch"):
                 return float(line.split("=")[1])
     except OSError:
+        pass
+    return None
+
+
+def measured_valu_issue():
+    """Fraction of the chip's VALU issue slots the scoring kernel fills, from
+    the committed rocprofv3 counter summary (SQ_INSTS_VALU x 4 cycles over
+    1024 SIMDs x GRBM_GUI_ACTIVE cycles); None when no summary is present."""
+    try:
+        for line in VALU_FILE.read_text().splitlines():
+            if line.startswith("VALU issue ="):
+                return float(line.rsplit("=", 1)[1].strip().rstrip("%")) / 100.0
+    except (OSError, ValueError):
         pass
     return None
 
@@ -177,6 +191,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": measured_traffic(),
+            "valu_issue_frac": measured_valu_issue(),
             "kernel": "score_poses_kernel",
             "flop_per_launch": flop,
         },
