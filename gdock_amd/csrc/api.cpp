@@ -4,13 +4,17 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <string_view>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -111,6 +115,104 @@ void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo,
   bisect_order(atoms, idx, lo, lo + k, block);
   bisect_order(atoms, idx, lo + k, hi, block);
 }
+
+
+namespace {
+
+// Pose records [lo, hi) from genes (genes already checked finite).
+void pose_records(const double* genes, double* poses, int64_t lo, int64_t hi) {
+  for (int64_t i = lo; i < hi; ++i) {
+    const double* g = genes + 6 * i;
+    double* p = poses + kPoseDoubles * i;
+    for (int k = 0; k < 6; ++k) p[k] = g[k];
+    // glibc cos/sin, as the reference's f64::cos/sin (structure.rs:96-103)
+    p[6] = std::cos(g[0]); p[7] = std::sin(g[0]);
+    p[8] = std::cos(g[1]); p[9] = std::sin(g[1]);
+    p[10] = std::cos(g[2]); p[11] = std::sin(g[2]);
+    // the bracketed sums of structure.rs:96-101 (no FMA contraction: -ffp-contract=off)
+    const double ca = p[6], sa = p[7], sb = p[9], cg = p[10], sg = p[11];
+    p[12] = (-sa * cg) + ((ca * sb) * sg);
+    p[13] = (sa * sg) + ((ca * sb) * cg);
+    p[14] = (ca * cg) + ((sa * sb) * sg);
+    p[15] = (-ca * sg) + ((sa * sb) * cg);
+  }
+}
+
+// Fork-join over [0, n) for large host loops: a few worker threads, created
+// on first use and kept for the life of the process, each take one slice and
+// the caller takes the first.  One call at a time uses the workers; a call
+// that finds them busy runs its loop alone.
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* pool = new HostPool();  // never destroyed: workers live as long as the process
+    return *pool;
+  }
+  template <class F>
+  void run(int64_t n, F&& f) {
+    std::unique_lock<std::mutex> busy(call_mu_, std::try_to_lock);
+    const int parts = static_cast<int>(std::min<int64_t>(n, workers_ + 1));
+    if (!busy.owns_lock() || parts <= 1) {
+      f(0, n);
+      return;
+    }
+    auto slice = [n, parts](int q, int64_t* lo, int64_t* hi) {
+      *lo = n * q / parts;
+      *hi = n * (q + 1) / parts;
+    };
+    std::function<void(int)> job = [&](int q) {
+      int64_t lo, hi;
+      slice(q, &lo, &hi);
+      f(lo, hi);
+    };
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      job_ = &job;
+      parts_ = parts;
+      pending_ = parts - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    job(0);
+    std::unique_lock<std::mutex> g(mu_);
+    done_cv_.wait(g, [this] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  HostPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    workers_ = static_cast<int>(std::min(7u, hw > 1 ? hw - 1 : 0u));
+    for (int w = 0; w < workers_; ++w) std::thread([this, w] { loop(w + 1); }).detach();
+  }
+  void loop(int q) {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_.wait(g, [&] { return gen_ != seen; });
+      seen = gen_;
+      if (q >= parts_) continue;  // this call uses fewer slices
+      const std::function<void(int)>* job = job_;
+      g.unlock();
+      (*job)(q);
+      g.lock();
+      if (--pending_ == 0) done_cv_.notify_one();
+    }
+  }
+  int workers_ = 0;
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  int parts_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
+template <class F>
+void host_pool_for(int64_t n, F&& f) {
+  HostPool::get().run(n, std::forward<F>(f));
+}
+
+}  // namespace
 
 }  // namespace gdock
 
@@ -369,22 +471,15 @@ int32_t gdock_complex_n_restraints(const gdock_complex* cx) { return cx ? cx->n_
 
 gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses) {
   if (n < 0 || (n && (!genes || !poses))) return GDOCK_ERR_INVALID_ARG;
-  for (int32_t i = 0; i < n; ++i) {
-    const double* g = genes + 6 * i;
-    for (int k = 0; k < 6; ++k)
-      if (!std::isfinite(g[k])) return GDOCK_ERR_INVALID_ARG;
-    double* p = poses + kPoseDoubles * i;
-    for (int k = 0; k < 6; ++k) p[k] = g[k];
-    // glibc cos/sin, as the reference's f64::cos/sin (structure.rs:96-103)
-    p[6] = std::cos(g[0]); p[7] = std::sin(g[0]);
-    p[8] = std::cos(g[1]); p[9] = std::sin(g[1]);
-    p[10] = std::cos(g[2]); p[11] = std::sin(g[2]);
-    // the bracketed sums of structure.rs:96-101 (no FMA contraction: -ffp-contract=off)
-    const double ca = p[6], sa = p[7], sb = p[9], cg = p[10], sg = p[11];
-    p[12] = (-sa * cg) + ((ca * sb) * sg);
-    p[13] = (sa * sg) + ((ca * sb) * cg);
-    p[14] = (ca * cg) + ((sa * sb) * sg);
-    p[15] = (-ca * sg) + ((sa * sb) * cg);
+  for (int64_t i = 0; i < static_cast<int64_t>(n) * 6; ++i)
+    if (!std::isfinite(genes[i])) return GDOCK_ERR_INVALID_ARG;
+  // A generation of a large batch is thousands of poses: split it over the
+  // host pool (each pose is independent, so the split changes no value).
+  constexpr int32_t kSplitAt = 2048;
+  if (n >= kSplitAt) {
+    host_pool_for(n, [&](int64_t lo, int64_t hi) { pose_records(genes, poses, lo, hi); });
+  } else {
+    pose_records(genes, poses, 0, n);
   }
   return GDOCK_OK;
 }

@@ -462,6 +462,7 @@ __device__ __forceinline__ void finish_pose(CxRef cx, const PoseCoef& pc, const 
 
 // fitness.rs:165-181: one ligand atom's desolvation term.
 __device__ __forceinline__ double ligand_desolv(CxRef cx, int l, int32_t cnt) {
+  if (cnt == 0) return 0.0;  // burial 0: the term is exactly zero
   const double asp = cx.lig_asp[l];
   if (!(fabs(asp) >= 0.001)) return 0.0;
   const double burial = fmin(static_cast<double>(cnt) / 10.0, 1.0);
@@ -647,9 +648,11 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     }
   }
   for (int j = pp.r0 + tid; j < pp.r1; j += kScoreThreads) {  // fitness.rs:141-162
+    const int32_t cnt = s_rcnt[j];
+    if (cnt == 0) continue;  // burial 0: the term is exactly zero
     const double asp = cx.rec_asp[j];
     if (fabs(asp) >= 0.001) {
-      const double burial = fmin(static_cast<double>(s_rcnt[j]) / 10.0, 1.0);
+      const double burial = fmin(static_cast<double>(cnt) / 10.0, 1.0);
       const double rad = cx.rec_radius[j];
       sum_d -= asp * burial * rad * rad * 4.0 * kPI;
     }

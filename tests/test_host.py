@@ -97,6 +97,24 @@ def test_prepare_poses_trig_is_glibc():
     assert _lib.lib().gdock_prepare_poses(_lib.ptr(bad), 64, _lib.ptr(poses)) == _lib.INVALID_ARG
 
 
+def test_prepare_poses_split_over_threads_is_identical():
+    """A large call is split over the host pool; every record equals the one
+    prepared alone, and a non-finite gene anywhere still rejects the call."""
+    g = synth.random_genes(9, 5000)
+    whole = np.zeros((5000, 16))
+    assert _lib.lib().gdock_prepare_poses(_lib.ptr(g), 5000, _lib.ptr(whole)) == 0
+    parts = np.zeros((5000, 16))
+    for a in range(0, 5000, 100):
+        sub = np.ascontiguousarray(g[a:a + 100])
+        out = np.zeros((100, 16))
+        assert _lib.lib().gdock_prepare_poses(_lib.ptr(sub), 100, _lib.ptr(out)) == 0
+        parts[a:a + 100] = out
+    assert np.array_equal(whole, parts)
+    bad = g.copy()
+    bad[4321, 2] = np.inf
+    assert _lib.lib().gdock_prepare_poses(_lib.ptr(bad), 5000, _lib.ptr(whole)) == _lib.INVALID_ARG
+
+
 def test_synthetic_complex_deterministic_and_sized():
     a = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
     b = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
