@@ -12,17 +12,18 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libgdock_hip.so"
 
-# gdock_atom (structure.rs:28-49 fields), 80 bytes, 8-byte aligned
+# gdock_atom (structure.rs:28-49 fields), 88 bytes, 8-byte aligned
 ATOM_DTYPE = np.dtype(
     [
         ("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("charge", "<f8"), ("vdw_radius", "<f8"),
-        ("epsilon", "<f8"), ("rmin2", "<f8"), ("serial", "<i4"), ("resseq", "<i2"),
+        ("epsilon", "<f8"), ("rmin2", "<f8"), ("occupancy", "<f4"), ("tempfactor", "<f4"),
+        ("serial", "<i4"), ("resseq", "<i2"),
         ("chainid", "S1"), ("altloc", "S1"), ("icode", "S1"), ("name", "S5"), ("resname", "S5"),
         ("element", "S5"),
     ],
     align=True,
 )
-assert ATOM_DTYPE.itemsize == 80, ATOM_DTYPE.itemsize
+assert ATOM_DTYPE.itemsize == 88, ATOM_DTYPE.itemsize
 
 OK, INVALID_ARG, OOM, DEVICE, BUSY, NOT_FOUND = range(6)
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE", 4: "BUSY", 5: "NOT_FOUND"}
@@ -147,6 +148,10 @@ def lib() -> C.CDLL:
         "gdock_evaluator_destroy": (None, [P, P]),
         "gdock_evaluator_info": (C.c_int, [P, I32, I32, I32]),
         "gdock_calc_metrics": (C.c_int, [P, P, D, C.c_int32, D]),
+        "gdock_format_pdb": (C.c_int, [P, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "gdock_apply_genes": (C.c_int, [P, C.c_int32, D, P]),
+        "gdock_write_run_outputs": (C.c_int, [P, P, P, D, D, C.c_int32, C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_int64), C.c_int32, C.c_char_p]),
         "gdock_server_default_config": (C.c_int, [C.POINTER(ServerConfig)]),
         "gdock_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(P)]),
         "gdock_server_destroy": (None, [P]),

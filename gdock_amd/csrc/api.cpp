@@ -125,10 +125,13 @@ void pose_records(const double* genes, double* poses, int64_t lo, int64_t hi) {
     const double* g = genes + 6 * i;
     double* p = poses + kPoseDoubles * i;
     for (int k = 0; k < 6; ++k) p[k] = g[k];
-    // glibc cos/sin, as the reference's f64::cos/sin (structure.rs:96-103)
-    p[6] = std::cos(g[0]); p[7] = std::sin(g[0]);
-    p[8] = std::cos(g[1]); p[9] = std::sin(g[1]);
-    p[10] = std::cos(g[2]); p[11] = std::sin(g[2]);
+    // structure.rs:96-103 takes f64::cos and f64::sin of each angle; a sin and
+    // a cos of one operand compile to one glibc sincos call (LLVM on GNU
+    // targets, as g++ does too), which can differ from separate cos and sin
+    // calls in the last bit.  So: sincos, explicitly.
+    ::sincos(g[0], &p[7], &p[6]);
+    ::sincos(g[1], &p[9], &p[8]);
+    ::sincos(g[2], &p[11], &p[10]);
     // the bracketed sums of structure.rs:96-101 (no FMA contraction: -ffp-contract=off)
     const double ca = p[6], sa = p[7], sb = p[9], cg = p[10], sg = p[11];
     p[12] = (-sa * cg) + ((ca * sb) * sg);

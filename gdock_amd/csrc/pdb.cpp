@@ -65,6 +65,17 @@ bool to_f64(std::string_view s, double* v) {
   return end == buf + s.size();
 }
 
+// Rust's f32 parse of the same forms: correctly rounded straight to f32.
+bool to_f32(std::string_view s, float* v) {
+  double probe;
+  if (!to_f64(s, &probe)) return false;
+  char buf[64];
+  std::memcpy(buf, s.data(), s.size());
+  buf[s.size()] = 0;
+  *v = std::strtof(buf, nullptr);
+  return true;
+}
+
 void copy_field(char* dst, std::string_view s) {
   const size_t n = s.size() < 4 ? s.size() : 4;
   std::memcpy(dst, s.data(), n);
@@ -92,6 +103,8 @@ bool parse_atom_line(std::string_view line, gdock_atom* a) {
   if (!to_f64(trim(col(30, 38)), &a->x) || !to_f64(trim(col(38, 46)), &a->y) ||
       !to_f64(trim(col(46, 54)), &a->z))
     return false;
+  if (!to_f32(trim(col(54, 60)), &a->occupancy)) a->occupancy = 1.0f;  // structure.rs:236-240
+  if (!to_f32(trim(col(60, 66)), &a->tempfactor)) a->tempfactor = 0.0f;
   std::string_view el = trim(col(76, 78));
   if (el.empty()) {
     a->element[0] = name.empty() ? ' ' : name[0];  // structure.rs:245-247

@@ -79,6 +79,19 @@ def take_result(h, p: GaParams) -> GaResult:
         L.gdock_ga_free(h)
 
 
+def write_outputs(cx: Complex, result: GaResult, out_dir, evaluator=None) -> None:
+    """commands::run's output files after clustering (run.rs:738-880):
+    model_<i>.pdb, ranked_<i>.pdb and metrics.tsv in out_dir."""
+    hg = np.ascontiguousarray(result.hof_genes, dtype=np.float64).reshape(-1, 6)
+    hf = np.ascontiguousarray(result.hof_fitness, dtype=np.float64)
+    si = np.ascontiguousarray([s[0] for s in result.selected] or [0], dtype=np.int64)
+    ss = np.ascontiguousarray([s[1] for s in result.selected] or [0], dtype=np.int64)
+    check(lib().gdock_write_run_outputs(cx.ctx.handle, cx.handle, evaluator.handle if evaluator else None, ptr(hg),
+                                        ptr(hf), len(hf), ptr(si, C.c_int64), ptr(ss, C.c_int64),
+                                        len(result.selected), str(out_dir).encode()),
+          cx.ctx.handle, "gdock_write_run_outputs")
+
+
 def rng_stream(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.uint64)
     lib().gdock_rng_stream(seed, n, ptr(out, C.c_uint64))

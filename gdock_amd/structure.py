@@ -80,6 +80,30 @@ def read_pdb(path: str | Path) -> list[Molecule]:
     return read_pdb_text(Path(path).read_text())
 
 
+def to_pdb_string(mol: Molecule) -> str:
+    """Atom::to_pdb_string for every atom, concatenated (structure.rs:159-176)."""
+    n = C.c_size_t()
+    L = lib()
+    check(L.gdock_format_pdb(mol.atoms.ctypes.data, len(mol), None, 0, C.byref(n)), what="format_pdb")
+    buf = C.create_string_buffer(n.value + 1)
+    check(L.gdock_format_pdb(mol.atoms.ctypes.data, len(mol), buf, n.value + 1, C.byref(n)), what="format_pdb")
+    return buf.raw[:n.value].decode()
+
+
+def write_pdb(mol: Molecule, path: str | Path) -> None:
+    """structure.rs:303-310."""
+    Path(path).write_text(to_pdb_string(mol))
+
+
+def apply_genes(ligand: Molecule, genes) -> Molecule:
+    """Chromosome::apply_genes (chromosome.rs:79-83): rotate about the COM, then displace."""
+    g = np.ascontiguousarray(np.asarray(genes, dtype=np.float64).reshape(6))
+    out = Molecule(np.zeros(len(ligand), ATOM_DTYPE))
+    check(lib().gdock_apply_genes(ligand.atoms.ctypes.data, len(ligand), g.ctypes.data_as(C.POINTER(C.c_double)),
+                                  out.atoms.ctypes.data), what="apply_genes")
+    return out
+
+
 def combine_molecules(receptor: Molecule, ligand: Molecule) -> Molecule:
     """run.rs:35-49."""
     return Molecule(np.concatenate([receptor.atoms, ligand.atoms]))

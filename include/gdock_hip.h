@@ -46,6 +46,8 @@ typedef struct gdock_atom {
   double vdw_radius;  /* Angstrom (H 1.2, C 1.7, N 1.55, O 1.52, else 1.0) */
   double epsilon;     /* kcal/mol, <= 0 */
   double rmin2;       /* Angstrom */
+  float occupancy;    /* PDB columns 55-60 (1.0 when absent) */
+  float tempfactor;   /* PDB columns 61-66 (0.0 when absent) */
   int32_t serial;
   int16_t resseq;
   char chainid, altloc, icode;
@@ -215,6 +217,28 @@ int32_t gdock_ga_clusters(const gdock_ga_result* r, int64_t* center, int64_t* si
 int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t* cluster_size);
 /* The first n outputs of the GA's random stream for a seed. */
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
+
+/* ---- output stage ------------------------------------------------------------ */
+/* Atom::to_pdb_string over n atoms, concatenated as write_pdb writes them
+ * (src/structure.rs:159-176,303-310): one "ATOM  " record per atom with the
+ * reference's field widths, alignment and rounding.  *len receives the text
+ * length; with out non-null the text and a terminating NUL are written when
+ * cap > *len (else GDOCK_ERR_INVALID_ARG). */
+gdock_status gdock_format_pdb(const gdock_atom* atoms, int32_t n, char* out, size_t cap, size_t* len);
+/* Chromosome::apply_genes (src/chromosome.rs:79-83): the ligand rotated about
+ * its centre of mass by genes[0..3] and displaced by genes[3..6], FP64 in the
+ * reference's operation order with glibc cos/sin; every other field copied. */
+gdock_status gdock_apply_genes(const gdock_atom* lig, int32_t n, const double* genes, gdock_atom* out);
+/* The files commands::run writes after clustering (src/commands/run.rs:
+ * 738-880) into out_dir (created if missing): model_<i>.pdb for each selected
+ * model (Hall of Fame index, cluster size, as gdock_ga_selected returns them),
+ * ranked_<i>.pdb for the five best Hall of Fame entries by fitness, each the
+ * complex's receptor followed by its positioned ligand moved by the entry's
+ * genes, and metrics.tsv (model, cluster_size, score; with an evaluator also
+ * dockq, rmsd, irmsd, fnat), every number with four decimals. */
+gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const double* hof_genes,
+                                     const double* hof_fitness, int32_t n_hof, const int64_t* selected_hof_index,
+                                     const int64_t* selected_cluster_size, int32_t n_selected, const char* out_dir);
 
 /* ---- DockQ evaluator --------------------------------------------------------- */
 /* Evaluator::new (src/evaluator.rs:50-66) for the complex's receptor and a

@@ -79,12 +79,13 @@ using Molecule = std::vector<Atom>;
 // Interchange record, same layout as the C-ABI's gdock_atom.
 struct RecordAtom {
   double x, y, z, charge, vdw_radius, epsilon, rmin2;
+  float occupancy, tempfactor;
   int32_t serial;
   int16_t resseq;
   char chainid, altloc, icode;
   char name[5], resname[5], element[5];
 };
-static_assert(sizeof(RecordAtom) == 80, "record layout");
+static_assert(sizeof(RecordAtom) == 88, "record layout");
 
 // ---------------------------------------------------------------- toppar
 struct ToppType { bool has_eps, has_rmin, has_q; double eps, rmin, q; };
@@ -169,6 +170,15 @@ bool parse_f64(const std::string& s, double* out) {
   return true;
 }
 
+// Rust's f32 parse of the same forms: correctly rounded straight to f32
+// (not through f64, which could round twice).
+bool parse_f32(const std::string& s, float* out) {
+  double probe;
+  if (!parse_f64(s, &probe)) return false;
+  *out = std::strtof(s.c_str(), nullptr);
+  return true;
+}
+
 // structure.rs:187-270 process_atom_line
 bool process_atom_line(const std::string& line, Atom* atom) {
   if (line.compare(0, 4, "ATOM") != 0 || line.size() < 54) return false;
@@ -189,9 +199,8 @@ bool process_atom_line(const std::string& line, Atom* atom) {
   if (!parse_f64(trim(get(30, 38)), &t.x)) return false;
   if (!parse_f64(trim(get(38, 46)), &t.y)) return false;
   if (!parse_f64(trim(get(46, 54)), &t.z)) return false;
-  double d;
-  t.occupancy = parse_f64(trim(get(54, 60)), &d) ? static_cast<float>(d) : 1.0f;
-  t.tempfactor = parse_f64(trim(get(60, 66)), &d) ? static_cast<float>(d) : 0.0f;
+  if (!parse_f32(trim(get(54, 60)), &t.occupancy)) t.occupancy = 1.0f;
+  if (!parse_f32(trim(get(60, 66)), &t.tempfactor)) t.tempfactor = 0.0f;
   t.element = trim(get(76, 78));
   if (t.element.empty()) t.element = std::string(1, t.name.empty() ? ' ' : t.name[0]);
   const std::string el = trim(t.element);
@@ -247,9 +256,12 @@ Molecule rotate(Molecule m, double alpha, double beta, double gamma) {
   double cx, cy, cz;
   center_of_mass(m, &cx, &cy, &cz);
   for (auto& a : m) { a.x -= cx; a.y -= cy; a.z -= cz; }
-  const double ca = std::cos(alpha), sa = std::sin(alpha);
-  const double cb = std::cos(beta), sb = std::sin(beta);
-  const double cg = std::cos(gamma), sg = std::sin(gamma);
+  // f64::cos and f64::sin of one angle compile to one glibc sincos call
+  // (LLVM on GNU targets; g++ -O3 does the same): called explicitly here.
+  double ca, sa, cb, sb, cg, sg;
+  ::sincos(alpha, &sa, &ca);
+  ::sincos(beta, &sb, &cb);
+  ::sincos(gamma, &sg, &cg);
   for (auto& a : m) {
     const double x = a.x, y = a.y, z = a.z;
     const double nx = x * ca * cb + y * (-sa * cg + ca * sb * sg) + z * (sa * sg + ca * sb * cg);
@@ -795,6 +807,7 @@ Molecule from_records(const RecordAtom* r, int n) {
     Atom& a = m[i];
     a.x = r[i].x; a.y = r[i].y; a.z = r[i].z;
     a.charge = r[i].charge; a.vdw_radius = r[i].vdw_radius; a.epsilon = r[i].epsilon; a.rmin2 = r[i].rmin2;
+    a.occupancy = r[i].occupancy; a.tempfactor = r[i].tempfactor;
     a.serial = r[i].serial; a.resseq = r[i].resseq; a.chainid = r[i].chainid;
     a.altloc = r[i].altloc; a.icode = r[i].icode;
     a.name = std::string(r[i].name, strnlen(r[i].name, 5));
@@ -810,6 +823,7 @@ void to_records(const Molecule& m, RecordAtom* r) {
     std::memset(&r[i], 0, sizeof(RecordAtom));
     r[i].x = a.x; r[i].y = a.y; r[i].z = a.z;
     r[i].charge = a.charge; r[i].vdw_radius = a.vdw_radius; r[i].epsilon = a.epsilon; r[i].rmin2 = a.rmin2;
+    r[i].occupancy = a.occupancy; r[i].tempfactor = a.tempfactor;
     r[i].serial = a.serial; r[i].resseq = a.resseq; r[i].chainid = a.chainid;
     r[i].altloc = a.altloc; r[i].icode = a.icode;
     std::strncpy(r[i].name, a.name.c_str(), 4);

@@ -17,7 +17,7 @@ LIB = HERE / "libgdock_oracle.so"
 
 ATOM_DTYPE = np.dtype(
     [("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("charge", "<f8"), ("vdw_radius", "<f8"), ("epsilon", "<f8"),
-     ("rmin2", "<f8"), ("serial", "<i4"), ("resseq", "<i2"), ("chainid", "S1"), ("altloc", "S1"), ("icode", "S1"),
+     ("rmin2", "<f8"), ("occupancy", "<f4"), ("tempfactor", "<f4"), ("serial", "<i4"), ("resseq", "<i2"), ("chainid", "S1"), ("altloc", "S1"), ("icode", "S1"),
      ("name", "S5"), ("resname", "S5"), ("element", "S5")], align=True)
 
 _lib = None

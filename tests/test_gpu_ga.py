@@ -97,3 +97,44 @@ def test_ga_small_population_parameters():
     ora = orc.GaRun(rec.atoms, lig.atoms, c.restraints, W, seed=11, pop_size=40, max_gen=30, nthreads=16)
     _compare(res, ora, 40)
     assert res.generations <= 30 and len(res.selected) <= 5
+
+
+def test_run_outputs_written_as_the_reference(tmp_path):
+    """run.rs:738-880 after a GA run: model_<i>.pdb for the selected models,
+    ranked_<i>.pdb for the five best Hall of Fame entries, metrics.tsv; each
+    PDB is the receptor then the ligand moved by the entry's genes (equal to
+    the oracle's apply_genes up to the three written decimals).  With an
+    evaluator the metrics carry DockQ, L-RMSD, iRMSD and fnat."""
+    from gdock_amd.evaluator import Evaluator
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    native = structure.read_pdb_text(read_text("2oob_B.pdb"))[0]
+    lig = structure.position_ligand(rec, native)
+    cx = Complex(rec, lig, RESTRAINTS_2OOB)
+    res = ga.run(cx, seed=9, max_generations=20)
+    assert len(res.selected) >= 1
+    order = np.argsort(res.hof_fitness, kind="stable")[:5]
+    for ev, sub in ((None, "plain"), (Evaluator(cx, native), "dockq")):
+        out = tmp_path / sub
+        ga.write_outputs(cx, res, out, evaluator=ev)
+        rows = (out / "metrics.tsv").read_text().splitlines()
+        head = "model\tcluster_size\tscore" + ("\tdockq\trmsd\tirmsd\tfnat" if ev else "")
+        assert rows[0] == head
+        models = [(f"model_{i + 1}", h, str(s)) for i, (h, s) in enumerate(res.selected)]
+        models += [(f"ranked_{r + 1}", int(h), "-") for r, h in enumerate(order)]
+        assert len(rows) == 1 + len(models)
+        met = ev.calc_metrics(res.hof_genes[[h for _, h, _ in models]]) if ev else None
+        for k, ((name, h, size), row) in enumerate(zip(models, rows[1:])):
+            want = [name, size, f"{res.hof_fitness[h]:.4f}"]
+            if ev:
+                rmsd, irmsd, fnat, dockq = met[k]
+                want += [f"{dockq:.4f}", f"{rmsd:.4f}", f"{irmsd:.4f}", f"{fnat:.4f}"]
+            assert row.split("\t") == want
+            text = (out / f"{name}.pdb").read_text()
+            moved = structure.apply_genes(lig, res.hof_genes[h])
+            assert text == structure.to_pdb_string(structure.combine_molecules(rec, moved))
+            back = structure.read_pdb_text(text)[0]
+            ref = orc.apply_genes(lig.atoms, res.hof_genes[h])
+            assert len(back) == len(rec) + len(lig)
+            for f in ("x", "y", "z"):
+                assert np.abs(back.atoms[f][len(rec):] - ref[f]).max() <= 5.0001e-4
+                assert np.array_equal(back.atoms[f][:len(rec)], rec.atoms[f])

@@ -15,8 +15,8 @@ from gdock_amd import _lib, structure, synth
 from helpers import read_text
 
 ROOT = Path(__file__).resolve().parent.parent
-FIELDS = ["x", "y", "z", "charge", "vdw_radius", "epsilon", "rmin2", "serial", "resseq", "chainid", "name",
-          "resname", "element"]
+FIELDS = ["x", "y", "z", "charge", "vdw_radius", "epsilon", "rmin2", "occupancy", "tempfactor", "serial", "resseq",
+          "chainid", "name", "resname", "element"]
 
 
 def header_functions():
@@ -80,14 +80,29 @@ def test_position_ligand_matches_oracle():
         assert np.array_equal(mine.atoms[f], ref[f])
 
 
-def test_prepare_poses_trig_is_glibc():
+def _glibc_sincos(x: float) -> tuple[float, float]:
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.sincos.restype = None
+    libm.sincos.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    s, c = ctypes.c_double(), ctypes.c_double()
+    libm.sincos(x, ctypes.byref(s), ctypes.byref(c))
+    return c.value, s.value
+
+
+def test_prepare_poses_trig_is_glibc_sincos():
+    """Pose records hold glibc sincos of each angle (the reference's cos/sin
+    pair of one operand compiles to one sincos call); 4.85373032658... is an
+    angle where sincos and separate cos/sin differ in the last bit."""
     g = synth.random_genes(5, 64)
+    g[0, 0] = 4.8537303265858345
     poses = np.zeros((64, 16))
     assert _lib.lib().gdock_prepare_poses(_lib.ptr(g), 64, _lib.ptr(poses)) == 0
     assert np.array_equal(poses[:, :6], g)
+    assert _glibc_sincos(g[0, 0])[0] != math.cos(g[0, 0])
     for i in range(64):
         a, b, c = g[i, :3]
-        ca, sa, cb, sb, cg, sg = math.cos(a), math.sin(a), math.cos(b), math.sin(b), math.cos(c), math.sin(c)
+        (ca, sa), (cb, sb), (cg, sg) = _glibc_sincos(a), _glibc_sincos(b), _glibc_sincos(c)
         assert tuple(poses[i, 6:12]) == (ca, sa, cb, sb, cg, sg)
         # structure.rs:96-101, the bracketed sums (Python floats: IEEE FP64, no FMA)
         assert tuple(poses[i, 12:]) == ((-sa * cg) + ((ca * sb) * sg), (sa * sg) + ((ca * sb) * cg),
