@@ -222,3 +222,33 @@ def test_serial_collisions_at_cutoffs():
     # and moved by a pose, through the transform path
     genes = np.array([[0.3, 0.2, 0.1, 0.05, -0.02, 0.01], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]])
     _check_poses(rec, lig, cx, [], genes, "serial collisions, posed")
+
+
+def test_component_errors_against_fp64_oracle():
+    """Each energy component's error against the FP64 oracle on the benchmark
+    complex (receptor 2000 x ligand 1500 atoms, seed 1, poses as the benchmark
+    draws them) and on a large uneven-batch complex with GA-initial poses:
+    the largest absolute and relative errors are printed (the numbers the
+    design notes quote) and every one is inside the tolerance."""
+    cases = []
+    cfg = synth.CONFIGS["batched_scoring"]
+    c = synth.make_complex(cfg["seed"], rec_atoms=cfg["rec_atoms"], lig_atoms=cfg["lig_atoms"],
+                           n_restraints=cfg["n_restraints"])
+    cases.append(("2000x1500 benchmark poses", c, synth.random_genes(cfg["seed"], 96)))
+    big = max(synth.uneven_batch(64, 100), key=lambda cc: cc.receptor_pdb.count("\n") * cc.ligand_pdb.count("\n"))
+    cases.append(("largest uneven-batch complex, GA poses", big, synth.ga_genes(100, 96)))
+    for what, cc, genes in cases:
+        rec = structure.read_pdb_text(cc.receptor_pdb)[0]
+        lig = structure.position_ligand(rec, structure.read_pdb_text(cc.ligand_pdb)[0])
+        cx = Complex(rec, lig, cc.restraints)
+        got, ns = cx.score_poses(genes)
+        ref, rns = orc.score_poses(rec.atoms, lig.atoms, cc.restraints, genes, nthreads=16)
+        assert_energies_close(got, ref, what)
+        assert np.array_equal(ns, rns)
+        err = np.abs(got - ref)
+        rel = err / np.maximum(np.abs(ref), 1e-300)
+        parts = []
+        for j, name in enumerate(["vdw", "elec", "desolv", "air"]):
+            nz = np.abs(ref[:, j]) > 1e-3
+            parts.append(f"{name} max abs {err[:, j].max():.2e} max rel {rel[nz, j].max() if nz.any() else 0.0:.2e}")
+        print(f"{what} ({len(rec)} x {len(lig)} atoms, {len(genes)} poses): " + "; ".join(parts))
