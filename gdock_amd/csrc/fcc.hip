@@ -33,6 +33,8 @@
 
 namespace gdock {
 
+size_t contacts_lds_bytes(const DevContacts& c);
+
 // ---------------------------------------------------------------- tables
 void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host) {
   cx->con = DevContacts{};
@@ -112,8 +114,8 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
   c.n_lig_res = static_cast<int32_t>(cx->lig_res_key.size());
   const int64_t bits = static_cast<int64_t>(c.n_rec_res) * c.n_lig_res;
   c.words64 = static_cast<int32_t>((bits + 63) / 64);
-  if (c.words64 * 8 > 60 * 1024) {
-    cx->con_unsupported = "contacts: residue-pair bitset exceeds the 60 KiB workgroup budget";
+  if (contacts_lds_bytes(c) > 64 * 1024) {
+    cx->con_unsupported = "contacts: residue-pair bitset and residue masks exceed the 64 KiB workgroup budget";
     c.words64 = 0;
     return;
   }
@@ -213,16 +215,30 @@ __device__ __forceinline__ void band_expanded(double T, double X, float& lo, flo
 // Wave w holds ligand heavy atoms base + w * 64 K + k * 64 + lane (K blocks
 // of kLigBlock spatially compact atoms).  A receptor residue whose bounding
 // sphere lies farther than `reach` (the cutoff plus a margin well above the
-// FP32 error) from the box of the wave's atoms has no contact with them.
+// FP32 error) from the box of a block has no contact with its atoms; each
+// residue is tested against each block's box.
+constexpr int kContactBoxFloats = 8;
+
+// Dynamic LDS of contacts_kernel: the pose's bitset (32-bit words), then per
+// wave kMaxAtomsPerLane block boxes and one mask byte per receptor residue.
+size_t contacts_lds_bytes(const DevContacts& c) {
+  return static_cast<size_t>(c.words64) * 8 + 4 * kContactBoxFloats * kMaxAtomsPerLane * sizeof(float) +
+         4 * static_cast<size_t>((c.n_rec_res + 3) & ~3);
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, const DevContacts ct,
                                                        const double* __restrict__ poses, double d2_incl, float reach,
                                                        unsigned long long* __restrict__ bits_out,
                                                        int32_t* __restrict__ sizes) {
-  extern __shared__ uint32_t s_bits[];
+  extern __shared__ uint32_t s_bits[];  // words32 bitset words, then the per-wave block boxes and residue masks
   __shared__ int32_t s_cnt[4];
   const int pose = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int words32 = 2 * ct.words64;
+  float* my_box = reinterpret_cast<float*>(s_bits + words32) + wave * (kContactBoxFloats * kMaxAtomsPerLane);
+  uint8_t* my_mask = reinterpret_cast<uint8_t*>(reinterpret_cast<float*>(s_bits + words32) +
+                                                4 * kContactBoxFloats * kMaxAtomsPerLane) +
+                     wave * ((ct.n_rec_res + 3) & ~3);
   for (int i = tid; i < words32; i += 256) s_bits[i] = 0u;
   const PoseT pc = pose_of(poses + static_cast<size_t>(kPoseDoubles) * pose);
   const double X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
@@ -249,26 +265,43 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
       }
       ll[k] = __builtin_fmaf(lz[k], lz[k], __builtin_fmaf(ly[k], ly[k], lx[k] * lx[k]));
     }
-    float x0 = INFINITY, y0 = INFINITY, z0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, z1 = -INFINITY;
+    // Box of each of the wave's ligand blocks (empty: +inf / -inf, never in reach).
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (base + wave * (kLigBlock * K) + k * kLigBlock + lane < ct.n_lig_heavy) {
-        x0 = fminf(x0, lx[k]); y0 = fminf(y0, ly[k]); z0 = fminf(z0, lz[k]);
-        x1 = fmaxf(x1, lx[k]); y1 = fmaxf(y1, ly[k]); z1 = fmaxf(z1, lz[k]);
+    for (int k = 0; k < K; ++k) {
+      const bool in = base + wave * (kLigBlock * K) + k * kLigBlock + lane < ct.n_lig_heavy;
+      float x0 = in ? lx[k] : INFINITY, y0 = in ? ly[k] : INFINITY, z0 = in ? lz[k] : INFINITY;
+      float x1 = in ? lx[k] : -INFINITY, y1 = in ? ly[k] : -INFINITY, z1 = in ? lz[k] : -INFINITY;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        x0 = fminf(x0, __shfl_xor(x0, off)); y0 = fminf(y0, __shfl_xor(y0, off)); z0 = fminf(z0, __shfl_xor(z0, off));
+        x1 = fmaxf(x1, __shfl_xor(x1, off)); y1 = fmaxf(y1, __shfl_xor(y1, off)); z1 = fmaxf(z1, __shfl_xor(z1, off));
       }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      x0 = fminf(x0, __shfl_xor(x0, off)); y0 = fminf(y0, __shfl_xor(y0, off)); z0 = fminf(z0, __shfl_xor(z0, off));
-      x1 = fmaxf(x1, __shfl_xor(x1, off)); y1 = fmaxf(y1, __shfl_xor(y1, off)); z1 = fmaxf(z1, __shfl_xor(z1, off));
+      if (lane == 0) {
+        float* bx = my_box + kContactBoxFloats * k;
+        bx[0] = x0; bx[1] = y0; bx[2] = z0; bx[3] = x1; bx[4] = y1; bx[5] = z1;
+      }
     }
-    x0 = uniform_f(x0); y0 = uniform_f(y0); z0 = uniform_f(z0);
-    x1 = uniform_f(x1); y1 = uniform_f(y1); z1 = uniform_f(z1);
-    for (int ri = 0; ri < ct.n_rec_res; ++ri) {
+    // Bit k of a residue's mask: its bounding sphere reaches block k.  A
+    // residue beyond reach of a block has no atom pair within the cutoff or
+    // its guard band with it, so skipping the pair is exact.
+    for (int r0 = 0; r0 < ct.n_rec_res; r0 += 64) {
+      const int ri = r0 + lane;
+      if (ri >= ct.n_rec_res) break;
       const float4 S = ct.rec_res_sphere[ri];
-      const float ex = fmaxf(fmaxf(x0 - S.x, S.x - x1), 0.0f), ey = fmaxf(fmaxf(y0 - S.y, S.y - y1), 0.0f),
-                  ez = fmaxf(fmaxf(z0 - S.z, S.z - z1), 0.0f);
       const float rr = S.w + reach;
-      if (ex * ex + ey * ey + ez * ez > rr * rr) continue;
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float* bx = my_box + kContactBoxFloats * k;
+        const float ex = fmaxf(fmaxf(bx[0] - S.x, S.x - bx[3]), 0.0f), ey = fmaxf(fmaxf(bx[1] - S.y, S.y - bx[4]), 0.0f),
+                    ez = fmaxf(fmaxf(bx[2] - S.z, S.z - bx[5]), 0.0f);
+        m |= (ex * ex + ey * ey + ez * ez <= rr * rr ? 1u : 0u) << k;
+      }
+      my_mask[ri] = static_cast<uint8_t>(m);
+    }
+    for (int ri = 0; ri < ct.n_rec_res; ++ri) {
+      const uint32_t mk = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(my_mask[ri]));
+      if (mk == 0u) continue;
       // closest receptor atom of the residue to each of the lane's atoms
       float mn[K];
 #pragma unroll
@@ -278,7 +311,8 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
         const float4 r = ct.rec_c4[j];
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          mn[k] = fminf(mn[k], __builtin_fmaf(r.x, lx[k], __builtin_fmaf(r.y, ly[k], __builtin_fmaf(r.z, lz[k], r.w + ll[k]))));
+          if ((mk >> k) & 1u)
+            mn[k] = fminf(mn[k], __builtin_fmaf(r.x, lx[k], __builtin_fmaf(r.y, ly[k], __builtin_fmaf(r.z, lz[k], r.w + ll[k]))));
       }
       bool hit[K];
 #pragma unroll
@@ -539,7 +573,7 @@ template <int K>
 hipError_t launch_contacts_k(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
                              unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
   const float reach = static_cast<float>(std::sqrt(std::max(d2_incl, 0.0))) + 0.01f;
-  hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), static_cast<size_t>(cx->con.words64) * 8, s, cx->dev,
+  hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), contacts_lds_bytes(cx->con), s, cx->dev,
                      cx->con, d_poses, d2_incl, reach, d_bits, d_sizes);
   return hipGetLastError();
 }
@@ -675,6 +709,11 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   e = launch_contacts(cx, static_cast<double*>(P(o_p)), n, contact_d2_threshold(cfg->contact_distance),
                       static_cast<unsigned long long*>(P(o_b)), static_cast<int32_t*>(P(o_s)), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: contacts");
+  // Integer overlap minima per set size (host), computed while the device
+  // finds the contacts.
+  std::vector<int32_t> ca, cb;
+  fcc_minima(cfg->fcc_cutoff, max_set, &ca);
+  fcc_minima(cfg->fcc_cutoff * cfg->strictness, max_set, &cb);
   e = hipMemsetAsync(P(o_u), 0, W32 * 4, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
   hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256, (n + kUnionPoses - 1) / kUnionPoses), dim3(256), 0,
@@ -686,9 +725,6 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   std::vector<int32_t> col;
   for (int w = 0; w < W32; ++w) if (uni[w]) col.push_back(w);
   const int wc = static_cast<int>(col.size());
-  std::vector<int32_t> ca, cb;
-  fcc_minima(cfg->fcc_cutoff, max_set, &ca);
-  fcc_minima(cfg->fcc_cutoff * cfg->strictness, max_set, &cb);
   const int zero_ok = (0.0 >= cfg->fcc_cutoff && 0.0 >= cfg->fcc_cutoff * cfg->strictness) ? 1 : 0;
   e = hipMemcpyAsync(P(o_col), col.data(), std::max(1, wc) * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(P(o_ca), ca.data(), ca.size() * 4, hipMemcpyHostToDevice, ctx->stream);
