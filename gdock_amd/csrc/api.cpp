@@ -2,7 +2,10 @@
 // resident receptor + positioned ligand), population scoring entry points.
 #include <hip/hip_runtime.h>
 
+#include <pthread.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <cstdarg>
@@ -144,7 +147,10 @@ void pose_records(const double* genes, double* poses, int64_t lo, int64_t hi) {
 // Fork-join over [0, n) for large host loops: a few worker threads, created
 // on first use and kept for the life of the process, each take one slice and
 // the caller takes the first.  One call at a time uses the workers; a call
-// that finds them busy runs its loop alone.
+// that finds them busy runs its loop alone, and so does every call in a
+// child process forked after the pool started (the workers are not there).
+std::atomic<bool> g_pool_forked{false};
+
 class HostPool {
  public:
   static HostPool& get() {
@@ -153,6 +159,10 @@ class HostPool {
   }
   template <class F>
   void run(int64_t n, F&& f) {
+    if (g_pool_forked.load(std::memory_order_relaxed)) {
+      f(0, n);
+      return;
+    }
     std::unique_lock<std::mutex> busy(call_mu_, std::try_to_lock);
     const int parts = static_cast<int>(std::min<int64_t>(n, workers_ + 1));
     if (!busy.owns_lock() || parts <= 1) {
@@ -184,6 +194,7 @@ class HostPool {
 
  private:
   HostPool() {
+    pthread_atfork(nullptr, nullptr, [] { g_pool_forked.store(true, std::memory_order_relaxed); });
     const unsigned hw = std::thread::hardware_concurrency();
     workers_ = static_cast<int>(std::min(7u, hw > 1 ? hw - 1 : 0u));
     for (int w = 0; w < workers_; ++w) std::thread([this, w] { loop(w + 1); }).detach();

@@ -130,6 +130,28 @@ def test_prepare_poses_split_over_threads_is_identical():
     assert _lib.lib().gdock_prepare_poses(_lib.ptr(bad), 5000, _lib.ptr(whole)) == _lib.INVALID_ARG
 
 
+def _prepare_in_child(genes, q):
+    out = np.zeros((len(genes), 16))
+    q.put((_lib.lib().gdock_prepare_poses(_lib.ptr(genes), len(genes), _lib.ptr(out)), out))
+
+
+def test_prepare_poses_in_a_forked_child():
+    """A process forked after the host pool started has no pool workers: a
+    large call there runs on the calling thread and gives the same records."""
+    import multiprocessing as mp
+    g = synth.random_genes(10, 4096)
+    parent = np.zeros((4096, 16))
+    assert _lib.lib().gdock_prepare_poses(_lib.ptr(g), 4096, _lib.ptr(parent)) == 0
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    p = ctx.Process(target=_prepare_in_child, args=(g, q))
+    p.start()
+    status, child = q.get(timeout=60)
+    p.join(60)
+    assert p.exitcode == 0 and status == 0
+    assert np.array_equal(child, parent)
+
+
 def test_synthetic_complex_deterministic_and_sized():
     a = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
     b = synth.make_complex(1, rec_atoms=2000, lig_atoms=1500)
