@@ -176,6 +176,15 @@ void GaJob::absorb(const double* out) {
 }
 
 // population.rs:74-158 with ga.rs:7-39 and chromosome.rs:56-77.
+const std::vector<double>& GaJob::final_genes() {
+  if (done_ && !final_bred_ && converged_at_ < 0 && !pop_.empty()) {
+    breed();
+    refresh_genes();
+  }
+  final_bred_ = true;
+  return genes_;
+}
+
 void GaJob::breed() {
   const size_t n = pop_.size();
   std::vector<Individual> elite = pop_;
@@ -361,6 +370,13 @@ gdock_status gdock_ga_info(const gdock_ga_result* r, int32_t* generations, int32
   if (n_clusters) *n_clusters = static_cast<int32_t>(r->clusters.size());
   if (n_selected) *n_selected = static_cast<int32_t>(r->job.selected().size());
   return GDOCK_OK;
+}
+
+int32_t gdock_ga_final_population(gdock_ga_result* r, double* genes) {
+  if (!r || !r->job.done()) return -1;
+  const std::vector<double>& g = r->job.final_genes();
+  if (genes) std::memcpy(genes, g.data(), g.size() * sizeof(double));
+  return static_cast<int32_t>(g.size() / 6);
 }
 
 gdock_status gdock_ga_trace(const gdock_ga_result* r, double* genes, double* fitness, int32_t* best_idx) {

@@ -68,6 +68,11 @@ class GaJob {
   // run.rs:694-733: cluster centres sorted by fitness, then the best-scored
   // remaining entries, up to five models.
   void select_models(const std::vector<gdock_cluster_result>& clusters);
+  // The population the generation loop leaves behind (run.rs:381-531): the
+  // last one scored after an early stop; after the last generation the loop
+  // has evolved once more (run.rs:530), so that unscored population.  Breeds
+  // it on first call once the job is done.
+  const std::vector<double>& final_genes();
   const std::vector<GaSelected>& selected() const { return selected_; }
   // per evaluated generation: genes (pop x 6), fitness (pop), best index
   std::vector<double> trace_genes, trace_fitness;
@@ -84,7 +89,7 @@ class GaJob {
   std::vector<GaSelected> selected_;
   int generation_ = 0, generations_ = 0, converged_at_ = -1, stall_ = 0;
   double last_best_ = 1.7976931348623157e308;
-  bool done_ = false;
+  bool done_ = false, final_bred_ = false;
 };
 
 gdock_status validate_ga_params(gdock_ctx* ctx, const gdock_ga_params* p);

@@ -107,6 +107,32 @@ bool write_text(const std::filesystem::path& p, const std::string& text) {
   return static_cast<bool>(f);
 }
 
+// <dir>/<name>.pdb: the complex's receptor followed by its positioned ligand
+// moved by genes (run.rs:35-49 combine_molecules, structure.rs:303-310).
+bool write_complex(gdock_ctx* ctx, const gdock_complex* cx, const std::filesystem::path& dir, const std::string& name,
+                   const double* genes, std::vector<gdock_atom>* scratch) {
+  const std::vector<gdock_atom>& rec = cx->rec;
+  const std::vector<gdock_atom>& lig = cx->lig;
+  scratch->resize(rec.size() + lig.size());
+  std::copy(rec.begin(), rec.end(), scratch->begin());
+  apply_genes(lig.data(), static_cast<int32_t>(lig.size()), genes, scratch->data() + rec.size());
+  std::string text;
+  text.reserve(scratch->size() * 80);
+  for (const gdock_atom& a : *scratch) pdb_line(text, a);
+  if (!write_text(dir / (name + ".pdb"), text)) {
+    set_error(ctx, "write outputs: cannot write %s.pdb", name.c_str());
+    return false;
+  }
+  return true;
+}
+
+bool make_dir(gdock_ctx* ctx, const char* out_dir) {
+  std::error_code ec;
+  std::filesystem::create_directories(std::filesystem::path(out_dir), ec);  // run.rs:540-543
+  if (ec) set_error(ctx, "write outputs: cannot create %s: %s", out_dir, ec.message().c_str());
+  return !ec;
+}
+
 }  // namespace
 
 extern "C" {
@@ -147,12 +173,7 @@ gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_ev
       return GDOCK_ERR_INVALID_ARG;
     }
   const std::filesystem::path dir(out_dir);
-  std::error_code ec;
-  std::filesystem::create_directories(dir, ec);  // run.rs:540-543
-  if (ec) {
-    set_error(ctx, "write_run_outputs: cannot create %s: %s", out_dir, ec.message().c_str());
-    return GDOCK_ERR_INVALID_ARG;
-  }
+  if (!make_dir(ctx, out_dir)) return GDOCK_ERR_INVALID_ARG;
   // run.rs:826-831: Hall of Fame indices by fitness, best first (stable)
   std::vector<int32_t> ranked(n_hof);
   std::iota(ranked.begin(), ranked.end(), 0);
@@ -176,21 +197,8 @@ gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_ev
     }
   }
 
-  const std::vector<gdock_atom>& rec = cx->rec;
-  const std::vector<gdock_atom>& lig = cx->lig;
-  std::vector<gdock_atom> complex(rec.size() + lig.size());
-  std::copy(rec.begin(), rec.end(), complex.begin());
-  auto write_model = [&](const std::string& name, const double* g) -> bool {
-    apply_genes(lig.data(), static_cast<int32_t>(lig.size()), g, complex.data() + rec.size());  // run.rs:35-49
-    std::string text;
-    text.reserve(complex.size() * 80);
-    for (const gdock_atom& a : complex) pdb_line(text, a);
-    if (!write_text(dir / (name + ".pdb"), text)) {
-      set_error(ctx, "write_run_outputs: cannot write %s.pdb", name.c_str());
-      return false;
-    }
-    return true;
-  };
+  std::vector<gdock_atom> complex;
+  auto write_model = [&](const std::string& name, const double* g) { return write_complex(ctx, cx, dir, name, g, &complex); };
   // metrics row: model, cluster size or '-', score, then DockQ, L-RMSD, iRMSD, fnat with an evaluator
   auto row = [&](const std::string& name, const std::string& size, double fitness, int32_t m) {
     std::string r = name + "\t" + size + "\t" + fixed(fitness, 4);
@@ -214,6 +222,45 @@ gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_ev
   }
   if (!write_text(dir / "metrics.tsv", tsv)) {
     set_error(ctx, "write_run_outputs: cannot write metrics.tsv");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  return GDOCK_OK;
+}
+
+gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const gdock_weights* w,
+                                      const double* genes, int32_t n, const char* out_dir) {
+  if (!ctx || !cx || !w || !out_dir || n <= 0 || !genes) {
+    if (ctx) set_error(ctx, "write_best_outputs: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  const std::filesystem::path dir(out_dir);
+  if (!make_dir(ctx, out_dir)) return GDOCK_ERR_INVALID_ARG;
+  // run.rs:553-565: score the final population, the first individual of least fitness
+  std::vector<double> out(static_cast<size_t>(n) * kOutDoubles);
+  gdock_status st = gdock_score_poses(ctx, cx, genes, n, w, out.data(), nullptr);
+  if (st != GDOCK_OK) return st;
+  int32_t best = 0;
+  for (int32_t i = 1; i < n; ++i)
+    if (out[kOutDoubles * i + 5] < out[kOutDoubles * best + 5]) best = i;
+  std::vector<gdock_atom> complex;
+  if (!write_complex(ctx, cx, dir, "best_by_score", genes + 6 * best, &complex)) return GDOCK_ERR_INVALID_ARG;
+  if (!ev) return GDOCK_OK;
+  // run.rs:577-634: the last individual of greatest DockQ, and metrics.tsv
+  std::vector<double> m(static_cast<size_t>(n) * 4);  // {rmsd, irmsd, fnat, dockq}
+  st = gdock_calc_metrics(ctx, ev, genes, n, m.data());
+  if (st != GDOCK_OK) return st;
+  int32_t top = 0;
+  for (int32_t i = 1; i < n; ++i)
+    if (m[4 * i + 3] >= m[4 * top + 3]) top = i;
+  if (!write_complex(ctx, cx, dir, "best_by_dockq", genes + 6 * top, &complex)) return GDOCK_ERR_INVALID_ARG;
+  auto row = [&](const char* name, int32_t i) {
+    const double* q = &m[4 * static_cast<size_t>(i)];
+    return std::string(name) + "\t" + fixed(q[3], 4) + "\t" + fixed(q[0], 4) + "\t" + fixed(q[1], 4) + "\t" +
+           fixed(q[2], 4) + "\t" + fixed(out[kOutDoubles * i + 5], 4) + "\n";
+  };
+  const std::string tsv = "model\tdockq\trmsd\tirmsd\tfnat\tscore\n" + row("best_by_score", best) + row("best_by_dockq", top);
+  if (!write_text(dir / "metrics.tsv", tsv)) {
+    set_error(ctx, "write_best_outputs: cannot write metrics.tsv");
     return GDOCK_ERR_INVALID_ARG;
   }
   return GDOCK_OK;

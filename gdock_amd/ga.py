@@ -33,6 +33,7 @@ class GaResult:
     cluster_center: np.ndarray = field(repr=False)
     cluster_size: np.ndarray = field(repr=False)
     selected: list
+    final_genes: np.ndarray = field(repr=False)  # the population the generation loop leaves behind (pop, 6)
 
 
 def run(cx: Complex, seed: int = 42, weights: constants.EnergyWeights | None = None, **params) -> GaResult:
@@ -73,8 +74,10 @@ def take_result(h, p: GaParams) -> GaResult:
         L.gdock_ga_clusters(h, ptr(cc, C.c_int64), ptr(cs, C.c_int64))
         si, ss = np.zeros(max(ns.value, 1), np.int64), np.zeros(max(ns.value, 1), np.int64)
         L.gdock_ga_selected(h, ptr(si, C.c_int64), ptr(ss, C.c_int64))
+        fin = np.zeros((pop, 6))
+        L.gdock_ga_final_population(h, ptr(fin))
         return GaResult(G, conv.value, genes, fit, best, hg, hf, hgen, cc, cs,
-                        list(zip(si[:ns.value].tolist(), ss[:ns.value].tolist())))
+                        list(zip(si[:ns.value].tolist(), ss[:ns.value].tolist())), fin)
     finally:
         L.gdock_ga_free(h)
 
@@ -90,6 +93,18 @@ def write_outputs(cx: Complex, result: GaResult, out_dir, evaluator=None) -> Non
                                         ptr(hf), len(hf), ptr(si, C.c_int64), ptr(ss, C.c_int64),
                                         len(result.selected), str(out_dir).encode()),
           cx.ctx.handle, "gdock_write_run_outputs")
+
+
+def write_best_outputs(cx: Complex, result: GaResult, out_dir, evaluator=None,
+                       weights: constants.EnergyWeights | None = None) -> None:
+    """commands::run's `--no-clust` output (run.rs:549-637): best_by_score.pdb,
+    and with an evaluator best_by_dockq.pdb and metrics.tsv, from the final
+    population."""
+    g = np.ascontiguousarray(result.final_genes, dtype=np.float64).reshape(-1, 6)
+    w = weights_c(weights)
+    check(lib().gdock_write_best_outputs(cx.ctx.handle, cx.handle, evaluator.handle if evaluator else None,
+                                         C.byref(w), ptr(g), len(g), str(out_dir).encode()),
+          cx.ctx.handle, "gdock_write_best_outputs")
 
 
 def rng_stream(seed: int, n: int) -> np.ndarray:

@@ -215,6 +215,13 @@ gdock_status gdock_ga_trace(const gdock_ga_result* r, double* genes, double* fit
 int32_t gdock_ga_hof(const gdock_ga_result* r, double* genes, double* fitness, int32_t* generation);
 int32_t gdock_ga_clusters(const gdock_ga_result* r, int64_t* center, int64_t* size);
 int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t* cluster_size);
+/* The population the generation loop of src/commands/run.rs:381-531 leaves
+ * behind, which `--no-clust` scores (run.rs:553-554): after an early stop the
+ * last population scored; after the last generation the loop has evolved
+ * once more (run.rs:530), and that unscored population is bred on the first
+ * call.  genes: population_size x 6 (may be null); returns the population
+ * size, or -1 for a null or unfinished result. */
+int32_t gdock_ga_final_population(gdock_ga_result* r, double* genes);
 /* The first n outputs of the GA's random stream for a seed. */
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
 
@@ -239,6 +246,14 @@ gdock_status gdock_apply_genes(const gdock_atom* lig, int32_t n, const double* g
 gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const double* hof_genes,
                                      const double* hof_fitness, int32_t n_hof, const int64_t* selected_hof_index,
                                      const int64_t* selected_cluster_size, int32_t n_selected, const char* out_dir);
+/* The `--no-clust` outputs (src/commands/run.rs:549-637) into out_dir: the
+ * n poses (the final population, gdock_ga_final_population) are scored with
+ * w; best_by_score.pdb is the first pose of least fitness.  With an
+ * evaluator also best_by_dockq.pdb (the last pose of greatest DockQ, as
+ * Iterator::max_by picks) and metrics.tsv (model, dockq, rmsd, irmsd, fnat,
+ * score for the two models, four decimals). */
+gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const gdock_weights* w,
+                                      const double* genes, int32_t n, const char* out_dir);
 
 /* ---- DockQ evaluator --------------------------------------------------------- */
 /* Evaluator::new (src/evaluator.rs:50-66) for the complex's receptor and a

@@ -723,6 +723,7 @@ struct GaTrace {
   HallOfFame hof;
   std::vector<ClusterResult> clusters;
   std::vector<std::pair<int64_t, int64_t>> selected;  // (hof index, cluster size)
+  std::vector<double> final_genes;  // the population at loop exit (run.rs:553-554 scores it for --no-clust)
 };
 
 void score_population(const Molecule& rec, const Molecule& lig, const std::vector<Restraint>& rs,
@@ -772,6 +773,7 @@ void run_ga(const Molecule& rec, const Molecule& lig_pos, const std::vector<Rest
     ++gen;
     pop = evolve(pop, rng);
   }
+  for (auto& c : pop) tr->final_genes.insert(tr->final_genes.end(), c.genes, c.genes + 6);
   if (!cluster) return;
   std::vector<Molecule> structs;
   for (auto& e : tr->hof.entries) {
@@ -1127,6 +1129,12 @@ int or_ga_clusters(void* h, int64_t* center, int64_t* size) {
   }
   return static_cast<int>(t->clusters.size());
 }
+int or_ga_final_population(void* h, double* genes) {
+  const auto* t = static_cast<const GaTrace*>(h);
+  if (genes) std::memcpy(genes, t->final_genes.data(), t->final_genes.size() * sizeof(double));
+  return static_cast<int>(t->final_genes.size() / 6);
+}
+
 int or_ga_selected(void* h, int64_t* idx, int64_t* size) {
   auto* t = static_cast<GaTrace*>(h);
   for (size_t i = 0; i < t->selected.size(); ++i) { idx[i] = t->selected[i].first; size[i] = t->selected[i].second; }

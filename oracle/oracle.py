@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
             "or_ga_hof": (C.c_int, [P, D, D, I32]),
             "or_ga_clusters": (C.c_int, [P, I64, I64]),
             "or_ga_selected": (C.c_int, [P, I64, I64]),
+            "or_ga_final_population": (C.c_int, [P, D]),
             "or_hof_new": (P, [C.c_int]),
             "or_hof_try_add": (C.c_int, [P, D, C.c_double, C.c_int]),
             "or_hof_entries": (C.c_int, [P, D]),
@@ -253,6 +254,8 @@ class GaRun:
         ss = np.zeros(5, np.int64)
         ns = L.or_ga_selected(self.h, _p(si, C.c_int64), _p(ss, C.c_int64))
         self.selected = list(zip(si[:ns].tolist(), ss[:ns].tolist()))
+        self.final_genes = np.zeros((pop_size, 6))
+        L.or_ga_final_population(self.h, _p(self.final_genes))
         L.or_ga_free(self.h)
         self.h = None
 

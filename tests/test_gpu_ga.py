@@ -138,3 +138,41 @@ def test_run_outputs_written_as_the_reference(tmp_path):
             for f in ("x", "y", "z"):
                 assert np.abs(back.atoms[f][len(rec):] - ref[f]).max() <= 5.0001e-4
                 assert np.array_equal(back.atoms[f][:len(rec)], rec.atoms[f])
+
+
+def test_no_clust_outputs_from_the_final_population(tmp_path):
+    """`--no-clust` (run.rs:549-637): the population the generation loop
+    leaves behind (after the last generation the loop has evolved once more)
+    is scored; best_by_score.pdb is its first pose of least fitness, and with
+    an evaluator best_by_dockq.pdb is its last pose of greatest DockQ, with
+    both in metrics.tsv.  The final population equals the oracle GA's."""
+    from gdock_amd.evaluator import Evaluator
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    native = structure.read_pdb_text(read_text("2oob_B.pdb"))[0]
+    lig = structure.position_ligand(rec, native)
+    cx = Complex(rec, lig, RESTRAINTS_2OOB)
+    res = ga.run(cx, seed=3, keep_trace=1, max_generations=25, cluster=0)
+    ora = orc.GaRun(rec.atoms, lig.atoms, RESTRAINTS_2OOB, W, seed=3, pop_size=150, max_gen=25, cluster=False,
+                    nthreads=16)
+    if _compare(res, ora, 150) == "identical" and res.converged_at < 0:
+        assert res.generations == 25
+        assert np.array_equal(res.final_genes, ora.final_genes)
+        assert not np.array_equal(res.final_genes, res.genes[-1])  # bred once more after generation 25
+    ev = Evaluator(cx, native)
+    out = tmp_path / "noclust"
+    ga.write_best_outputs(cx, res, out, evaluator=ev)
+    fit = cx.score_poses(res.final_genes)[0][:, 5]
+    met = ev.calc_metrics(res.final_genes)
+    best = int(np.argmin(fit))                                        # first minimum
+    top = len(met) - 1 - int(np.argmax(met[::-1, 3]))                 # last maximum
+    for name, i in (("best_by_score", best), ("best_by_dockq", top)):
+        moved = structure.apply_genes(lig, res.final_genes[i])
+        assert (out / f"{name}.pdb").read_text() == structure.to_pdb_string(structure.combine_molecules(rec, moved))
+    rows = (out / "metrics.tsv").read_text().splitlines()
+    assert rows[0] == "model\tdockq\trmsd\tirmsd\tfnat\tscore"
+    for row, (name, i) in zip(rows[1:], (("best_by_score", best), ("best_by_dockq", top))):
+        rmsd, irmsd, fnat, dockq = met[i]
+        assert row.split("\t") == [name, f"{dockq:.4f}", f"{rmsd:.4f}", f"{irmsd:.4f}", f"{fnat:.4f}", f"{fit[i]:.4f}"]
+    plain = tmp_path / "plain"
+    ga.write_best_outputs(cx, res, plain)
+    assert sorted(p.name for p in plain.iterdir()) == ["best_by_score.pdb"]
