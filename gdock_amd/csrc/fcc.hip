@@ -212,11 +212,11 @@ __device__ __forceinline__ void band_expanded(double T, double X, float& lo, flo
 }
 
 // ---------------------------------------------------------------- K6 contacts
-// Wave w holds ligand heavy atoms base + w * 64 K + k * 64 + lane (K blocks
-// of kLigBlock spatially compact atoms).  A receptor residue whose bounding
-// sphere lies farther than `reach` (the cutoff plus a margin well above the
-// FP32 error) from the box of a block has no contact with its atoms; each
-// residue is tested against each block's box.
+// Wave w holds K blocks of kLigBlock spatially compact ligand heavy atoms
+// (contact_slot).  A receptor residue whose bounding sphere lies farther than
+// `reach` (the cutoff plus a margin well above the FP32 error) from the box
+// of a block has no contact with its atoms; each residue is tested against
+// each block's box.
 constexpr int kContactBoxFloats = 8;
 
 // Dynamic LDS of contacts_kernel: the pose's bitset (32-bit words), then per
@@ -224,6 +224,13 @@ constexpr int kContactBoxFloats = 8;
 size_t contacts_lds_bytes(const DevContacts& c) {
   return static_cast<size_t>(c.words64) * 8 + 4 * kContactBoxFloats * kMaxAtomsPerLane * sizeof(float) +
          4 * static_cast<size_t>((c.n_rec_res + 3) & ~3);
+}
+
+// Ligand heavy atom in slot k of a lane for the chunk at `base`: the chunk's
+// blocks are dealt to the waves in turn (wave w: blocks w, w + 4, ...), so
+// every wave holds parts of the whole ligand and the waves' work is even.
+__device__ __forceinline__ int contact_slot(int base, int k, int wave, int lane) {
+  return base + (k * 4 + wave) * kLigBlock + lane;
 }
 
 template <int K>
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
     int32_t lres[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
+      const int h = contact_slot(base, k, wave, lane);
       if (h < ct.n_lig_heavy) {
         double px, py, pz;
         pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
     // Box of each of the wave's ligand blocks (empty: +inf / -inf, never in reach).
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const bool in = base + wave * (kLigBlock * K) + k * kLigBlock + lane < ct.n_lig_heavy;
+      const bool in = contact_slot(base, k, wave, lane) < ct.n_lig_heavy;
       float x0 = in ? lx[k] : INFINITY, y0 = in ? ly[k] : INFINITY, z0 = in ? lz[k] : INFINITY;
       float x1 = in ? lx[k] : -INFINITY, y1 = in ? ly[k] : -INFINITY, z1 = in ? lz[k] : -INFINITY;
 #pragma unroll
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
         hit[k] = mn[k] <= lo;
         const bool amb = !hit[k] && mn[k] < hi;
         if (__any(amb)) {  // rare: decide this residue in FP64 (clustering.rs:98-99)
-          const int h = base + wave * (kLigBlock * K) + k * kLigBlock + lane;
+          const int h = contact_slot(base, k, wave, lane);
           if (amb && h < ct.n_lig_heavy) {
             double px, py, pz;
             pose_atom(cx, pc, ct.lig_atom[h], px, py, pz);

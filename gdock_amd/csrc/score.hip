@@ -140,11 +140,15 @@ __device__ __forceinline__ Bands make_bands(double X) {
 }
 
 // Ligand atom held in slot k of this lane for the chunk starting at `base`:
-// wave w holds the K consecutive blocks of kLigBlock atoms (each block is
-// spatially compact) at base + w * kLigBlock * K.
+// the chunk's blocks of kLigBlock atoms (each spatially compact) are dealt to
+// the waves in turn, so wave w holds blocks w, w + 4, w + 8, ...  Every wave
+// then holds parts of the whole ligand: buried and exposed regions, and so
+// the pair work, spread evenly over the waves (consecutive blocks per wave
+// left the wave with the buried region as the pose's critical path, 10%
+// slower), and a small ligand's few blocks keep every wave busy.
 template <int K>
 __device__ __forceinline__ int lig_index(int base, int k) {
-  return base + static_cast<int>(threadIdx.x >> 6) * (kLigBlock * K) + k * kLigBlock +
+  return base + (k * (kScoreThreads / 64) + static_cast<int>(threadIdx.x >> 6)) * kLigBlock +
          static_cast<int>(threadIdx.x & 63);
 }
 
