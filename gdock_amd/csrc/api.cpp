@@ -120,6 +120,31 @@ void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo,
 }
 
 
+// Reorders the full blocks of `block` entries of idx (the partial block, if
+// any, stays last) by bit-reversed block index, so that any run of
+// consecutive blocks samples the whole order evenly.  Applied after
+// bisect_order: each block stays spatially compact, while consecutive blocks
+// come from all over the molecule.
+void spread_blocks(std::vector<int32_t>& idx, size_t block) {
+  const size_t nb = idx.size() / block;
+  if (nb < 3) return;
+  int bits = 0;
+  while ((size_t{1} << bits) < nb) ++bits;
+  auto rev = [bits](size_t b) {
+    size_t r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((b >> i) & 1u) << (bits - 1 - i);
+    return r;
+  };
+  std::vector<size_t> order(nb);
+  for (size_t b = 0; b < nb; ++b) order[b] = b;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return rev(a) < rev(b); });
+  std::vector<int32_t> out(idx.size());
+  for (size_t p = 0; p < nb; ++p)
+    std::copy(idx.begin() + order[p] * block, idx.begin() + (order[p] + 1) * block, out.begin() + p * block);
+  std::copy(idx.begin() + nb * block, idx.end(), out.begin() + nb * block);
+  idx.swap(out);
+}
+
 namespace {
 
 // Pose records [lo, hi) from genes (genes already checked finite).
@@ -342,6 +367,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   std::vector<int32_t> lig_perm(n_lig);  // device index -> input index
   for (int32_t i = 0; i < n_lig; ++i) lig_perm[i] = i;
   bisect_order(cx->lig.data(), lig_perm, 0, lig_perm.size(), kLigBlock);
+  spread_blocks(lig_perm, kLigBlock);
   cx->lig_slot.assign(n_lig, 0);
   for (int32_t q = 0; q < n_lig; ++q) cx->lig_slot[lig_perm[q]] = q;
   std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;  // serial -> device indices
@@ -411,7 +437,20 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
                o_lser = L.add<int32_t>(n_lig), o_rrx = L.add<double>(rr_xyz.size()),
-               o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1);
+               o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1),
+               o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig);
+  // Lennard-Jones factors of every (receptor class, ligand atom) pair, with
+  // the FP64 operations of fitness.rs:58-63 (R = rmin2_r + rmin2_l, R^6 as
+  // R^2 (R^2 R^2), 1 / R^6; sqrt(eps_r eps_l)) rounded once to FP32.
+  std::vector<float2> lcls(static_cast<size_t>(nc) * n_lig);
+  for (int c = 0; c < nc; ++c)
+    for (int32_t i = 0; i < n_lig; ++i) {
+      const double R = class_rmin[c] + lrm[i];
+      const double R2 = R * R, R6 = R2 * (R2 * R2);
+      float2& f = lcls[static_cast<size_t>(c) * n_lig + i];
+      f.x = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
+      f.y = static_cast<float>(std::sqrt(class_eps[c] * leps[i]));
+    }
   cx->contacts_layout_begin = L.size;
   std::vector<uint8_t> host;
   build_contact_tables(cx, &L, &host);  // FCC residue tables appended to the same arena
@@ -428,6 +467,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
   put(o_lasp, lasp.data(), n_lig * 8); put(o_lrad, lrad.data(), n_lig * 8); put(o_lser, lser.data(), n_lig * 4);
   put(o_rrx, rr_xyz.data(), rr_xyz.size() * 8); put(o_rli, rl_idx.data(), rl_idx.size() * 4);
+  put(o_lcls, lcls.data(), lcls.size() * sizeof(float2));
   if (!host.empty()) std::memcpy(blob.data() + cx->contacts_layout_begin, host.data(), host.size());
 
   hipError_t e = hipSetDevice(ctx->device);
@@ -459,6 +499,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.lig_serial = reinterpret_cast<const int32_t*>(m + o_lser);
   d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
+  d.lig_class = reinterpret_cast<const float2*>(m + o_lcls);
   d.n_rec = n_pad; d.n_rec_atoms = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
   d.n_coll = static_cast<int32_t>(coll.size());
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }

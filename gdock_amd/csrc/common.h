@@ -52,6 +52,7 @@ struct DevComplex {
   const int32_t* class_start;  // n_class + 1
   const double* class_rmin2;
   const double* class_eps;
+  const float2* lig_class;     // n_class x n_lig: {1/R^6, sqrt(eps_r eps_l)} per receptor class and ligand atom
   // ligand (positioned), in spatial block order (lig_orig: index in the input)
   const double* lig_x;
   const double* lig_y;

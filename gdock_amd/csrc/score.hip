@@ -140,15 +140,16 @@ __device__ __forceinline__ Bands make_bands(double X) {
 }
 
 // Ligand atom held in slot k of this lane for the chunk starting at `base`:
-// the chunk's blocks of kLigBlock atoms (each spatially compact) are dealt to
-// the waves in turn, so wave w holds blocks w, w + 4, w + 8, ...  Every wave
-// then holds parts of the whole ligand: buried and exposed regions, and so
-// the pair work, spread evenly over the waves (consecutive blocks per wave
-// left the wave with the buried region as the pose's critical path, 10%
-// slower), and a small ligand's few blocks keep every wave busy.
+// wave w holds the K consecutive blocks of kLigBlock atoms at base + w * 64 K.
+// Each block is spatially compact, and the complex stores its blocks in
+// bit-reversed order, so any K consecutive blocks come from all over the
+// ligand: every wave holds buried and exposed parts alike and the waves' pair
+// work is even (spatially consecutive blocks per wave made the wave with the
+// buried region the pose's critical path, 10% slower).  Consecutive slots
+// keep the K addresses of a lane within load-offset range of one base.
 template <int K>
 __device__ __forceinline__ int lig_index(int base, int k) {
-  return base + (k * (kScoreThreads / 64) + static_cast<int>(threadIdx.x >> 6)) * kLigBlock +
+  return base + static_cast<int>(threadIdx.x >> 6) * (kLigBlock * K) + k * kLigBlock +
          static_cast<int>(threadIdx.x & 63);
 }
 
@@ -233,12 +234,12 @@ __device__ __forceinline__ bool in_band(float d2, const Bands& B) {
 }
 
 // Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
-// 1/R^6 and sqrt(eps_r eps_l) in FP64 (fitness.rs:58-63), rounded once.
+// 1/R^6 and sqrt(eps_r eps_l) from FP64 (fitness.rs:58-63), rounded once,
+// tabulated per complex (one coalesced 8-byte load per lane).
 __device__ __forceinline__ void class_factors(CxRef cx, int c, int l, float& ir6, float& eps) {
-  const double R = cx.class_rmin2[c] + cx.lig_rmin2[l];
-  const double R2 = R * R, R6 = R2 * (R2 * R2);
-  ir6 = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
-  eps = static_cast<float>(sqrt(cx.class_eps[c] * cx.lig_eps[l]));
+  const float2 f = cx.lig_class[static_cast<size_t>(c) * cx.n_lig + l];
+  ir6 = f.x;
+  eps = f.y;
 }
 
 // The reference's decisions for one pair, in FP64 on the pose's coordinates
