@@ -247,7 +247,7 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   Layout Ly;
   const size_t o_tab = Ly.add<DevComplex>(table.size()), o_items = Ly.add<int2>(ni), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(np * kPoseDoubles), o_out = Ly.add<double>(np * kOutDoubles),
-               o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np);
+               o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np), o_order = Ly.add<uint8_t>(order_bytes(ni));
   std::vector<size_t> o_split(n, 0);
   const size_t split_begin = Ly.size;
   for (int32_t i = 0; i < n; ++i) {
@@ -274,6 +274,7 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   std::memcpy(blob.data() + o_n, head, sizeof head);
   e = hipMemcpyAsync(m, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemsetAsync(m + o_ns, 0, (o_redo - o_ns) + np * sizeof(int32_t), ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m + o_order, 0, order_bytes(ni), ctx->stream);
   if (e == hipSuccess && split_end > split_begin)  // split state starts cleared; the kernel clears it after use
     e = hipMemsetAsync(m + split_begin, 0, split_end - split_begin, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -291,6 +292,7 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   L.out = reinterpret_cast<double*>(m + o_out);
   L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
   L.redo = reinterpret_cast<int32_t*>(m + o_redo);
+  bind_order(&L, m + o_order, ni);
   L.lds = batch_lds_bytes(max_rec);
   L.capacity = static_cast<int>(std::max<size_t>(b->items.size(), 1));
   L.grid = std::min(batch_grid(L.lds, ctx->device), L.capacity);

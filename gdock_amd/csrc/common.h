@@ -88,6 +88,10 @@ struct DevComplex {
 
 struct Weights { double vdw, elec, desolv, air; };
 
+// Cost buckets of the per-launch item order (half-octave steps of the
+// estimated pair work).
+constexpr int kCostBuckets = 64;
+
 // Device buffers of one batched (persistent) scoring launch.
 struct BatchLaunch {
   const DevComplex* table;  // complex slots (device)
@@ -99,6 +103,10 @@ struct BatchLaunch {
   double* out;              // kOutDoubles per pose
   int32_t* nsat;
   int32_t* redo;
+  // per launch, the items reordered by estimated cost (most expensive first)
+  int2* sorted;             // capacity items
+  uint8_t* bucket;          // capacity: cost bucket of each item
+  int32_t* hist;            // kCostBuckets counts, then kCostBuckets cursors: zero between launches
   int grid;                 // persistent workgroups
   int capacity;             // item capacity (grid of the exact pass)
   size_t lds;

@@ -82,6 +82,11 @@ int score_atoms_per_lane(int n_lig);
 size_t batch_lds_bytes(int max_n_rec);
 int batch_grid(size_t lds, int device);
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream);
+// Device bytes of the per-launch item order for `capacity` items, and the
+// binding of BatchLaunch::sorted / bucket / hist into such a block (which
+// must start zeroed).
+size_t order_bytes(size_t capacity);
+void bind_order(BatchLaunch* b, char* mem, size_t capacity);
 
 // Host-side bump layout of one device allocation (256-byte aligned pieces).
 struct Layout {

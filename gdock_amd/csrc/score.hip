@@ -887,8 +887,96 @@ int batch_grid(size_t lds, int device) {
   return per_cu * cus;
 }
 
+// Per-launch order of the items, most expensive first, so that the
+// persistent workgroups start the long items early and the launch does not
+// end on a long item taken late.  The cost of a pose varies twentyfold
+// within one complex (a pose inside the receptor against one far from it),
+// so it is estimated from the pose itself: the receptor slots of the item's
+// range within the ligand's radius + kCullReach of the pose's ligand centre
+// (the rotation keeps the centre; the genes move it), plus a share of the
+// range for the per-tile tests, times the ligand lane-slots walked.  Items
+// are counting-sorted into half-octave buckets of that cost.  The order
+// changes nothing in the results: every pose is computed by its own items.
+__global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __restrict__ table,
+                                                        const int2* __restrict__ items,
+                                                        const int32_t* __restrict__ n_items_p,
+                                                        const double* __restrict__ poses,
+                                                        uint8_t* __restrict__ bucket, int32_t* __restrict__ hist) {
+  const int n_items = *n_items_p;
+  const int lane = threadIdx.x & 63;
+  for (int it = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6))); it < n_items;
+       it += gridDim.x * 4) {  // one wave per item
+    const int2 I = items[it];
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
+    const PosePart pp = part_of(cx, I.x >> 24, I.y);
+    const double* p = poses + static_cast<size_t>(kPoseDoubles) * I.y;
+    const double ox = cx.com[0] - cx.origin[0], oy = cx.com[1] - cx.origin[1], oz = cx.com[2] - cx.origin[2];
+    const double lig_r = cx.lig_extent - fmax(fabs(ox), fmax(fabs(oy), fabs(oz)));
+    const float px = static_cast<float>(ox + p[3]), py = static_cast<float>(oy + p[4]),
+                pz = static_cast<float>(oz + p[5]);
+    const float reach = static_cast<float>(lig_r) + kCullReach;
+    int32_t near = 0;
+    for (int j = pp.r0 + lane; j < pp.r1; j += 64) {
+      const float4 r = cx.rec_xyzq[j];
+      const float dx = r.x - px, dy = r.y - py, dz = r.z - pz;
+      near += dx * dx + dy * dy + dz * dz <= reach * reach ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) near += __shfl_xor(near, off);
+    const int k = cx.lanes_k > 0 ? cx.lanes_k : 1;
+    const int per_chunk = kScoreThreads * k;
+    const double slots = static_cast<double>((cx.n_lig + per_chunk - 1) / per_chunk) * per_chunk;
+    const double cost = (near + 0.0625 * (pp.r1 - pp.r0)) * slots;
+    const int b = cost < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(cost)));
+    if (lane == 0) {
+      bucket[it] = static_cast<uint8_t>(b);
+      atomicAdd(&hist[b], 1);
+    }
+  }
+}
+
+// One workgroup of kCostBuckets threads: bucket b starts after every item of
+// the costlier buckets; the counts are cleared for the next launch.
+__global__ __launch_bounds__(kCostBuckets) void item_order_scan_kernel(int32_t* __restrict__ hist) {
+  __shared__ int32_t s_n[kCostBuckets];
+  const int b = threadIdx.x;
+  s_n[b] = hist[b];
+  hist[b] = 0;
+  __syncthreads();
+  int32_t start = 0;
+  for (int c = kCostBuckets - 1; c > b; --c) start += s_n[c];
+  hist[kCostBuckets + b] = start;  // cursor
+}
+
+__global__ __launch_bounds__(256) void item_scatter_kernel(const int2* __restrict__ items,
+                                                           const int32_t* __restrict__ n_items_p,
+                                                           const uint8_t* __restrict__ bucket,
+                                                           int32_t* __restrict__ hist, int2* __restrict__ sorted) {
+  const int n_items = *n_items_p;
+  for (int it = blockIdx.x * 256 + threadIdx.x; it < n_items; it += gridDim.x * 256)
+    sorted[atomicAdd(&hist[kCostBuckets + bucket[it]], 1)] = items[it];
+}
+
+size_t order_bytes(size_t capacity) {
+  const size_t c = std::max<size_t>(capacity, 1);
+  return (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256 + 2 * kCostBuckets * sizeof(int32_t);
+}
+
+void bind_order(BatchLaunch* b, char* mem, size_t capacity) {
+  const size_t c = std::max<size_t>(capacity, 1);
+  b->sorted = reinterpret_cast<int2*>(mem);
+  b->bucket = reinterpret_cast<uint8_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256);
+  b->hist = reinterpret_cast<int32_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256);
+}
+
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream) {
-  hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
+  const int cap = std::max(b.capacity, 1);
+  hipLaunchKernelGGL(item_cost_kernel, dim3(std::min((cap + 3) / 4, 4096)), dim3(256), 0, stream, b.table, b.items,
+                     b.n_items, b.poses, b.bucket, b.hist);
+  hipLaunchKernelGGL(item_order_scan_kernel, dim3(1), dim3(kCostBuckets), 0, stream, b.hist);
+  hipLaunchKernelGGL(item_scatter_kernel, dim3(std::min((cap + 255) / 256, 1024)), dim3(256), 0, stream, b.items,
+                     b.n_items, b.bucket, b.hist, b.sorted);
+  hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.sorted,
                      b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo);
   hipLaunchKernelGGL(score_batch_exact_kernel, dim3(std::min(b.capacity, b.grid)), dim3(kScoreThreads), b.lds, stream,
                      b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo);

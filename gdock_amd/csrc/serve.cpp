@@ -313,7 +313,8 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   Layout Ly;
   const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_items = Ly.add<int2>(cap_items), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(cap * kPoseDoubles), o_out = Ly.add<double>(cap * kOutDoubles),
-               o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap);
+               o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap),
+               o_order = Ly.add<uint8_t>(order_bytes(cap_items));
   hipError_t e = hipMalloc(&s->mem, Ly.size);
   if (e == hipSuccess) e = hipMemsetAsync(s->mem, 0, Ly.size, s->ctx->stream);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
@@ -335,6 +336,7 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     L.out = reinterpret_cast<double*>(m + o_out);
     L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
     L.redo = reinterpret_cast<int32_t*>(m + o_redo);
+    bind_order(&L, m + o_order, cap_items);
     L.lds = batch_lds_bytes(c.max_receptor_atoms + 64);
     L.capacity = static_cast<int>(cap_items);
     L.grid = std::min(batch_grid(L.lds, device), L.capacity);
