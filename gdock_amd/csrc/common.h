@@ -107,6 +107,7 @@ struct BatchLaunch {
   int2* sorted;             // capacity items
   uint8_t* bucket;          // capacity: cost bucket of each item
   int32_t* hist;            // kCostBuckets counts, then kCostBuckets cursors: zero between launches
+  int32_t* redo_count;      // poses of the launch flagged for the exact variant
   int grid;                 // persistent workgroups
   int capacity;             // item capacity (grid of the exact pass)
   size_t lds;

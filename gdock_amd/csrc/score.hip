@@ -434,6 +434,7 @@ struct PosePart {
   int32_t* lig_cnt;    // parts > 1: n_lig burial counts of the pose's ligand atoms
   double* sums;        // parts > 1: 3 doubles per part
   int32_t* sync;       // parts > 1: {parts finished, overflow seen}
+  int32_t* redo_count;  // batched launches: counts the poses flagged for the exact variant
 };
 
 // AIR, restraint penalty and the weighted fitness (thread 0), then the outputs.
@@ -627,7 +628,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
   __syncthreads();
   const bool ovf = !EXACT && *s_overflow != 0;
   if (ovf && pp.parts == 1) {  // the exact variant rescores this pose
-    if (tid == 0) *redo = 1;
+    if (tid == 0) {
+      *redo = 1;
+      if (pp.redo_count) atomicAdd(pp.redo_count, 1);
+    }
     __syncthreads();
     return;
   }
@@ -716,6 +720,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         atomicExch(&pp.sync[0], 0);
         if (any_ovf) {
           *redo = 1;  // the exact variant rescores the whole pose
+          if (pp.redo_count) atomicAdd(pp.redo_count, 1);
         } else {
           finish_pose(cx, pc, coords, w, v, e, d + lig_d, o, nsat_out);
           *redo = 0;
@@ -791,7 +796,8 @@ __device__ __forceinline__ PosePart part_of(CxRef cx, int part, int pose_row) {
 __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void score_batch_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     int32_t* __restrict__ counter, int32_t* __restrict__ exit_count, const double* __restrict__ poses,
-    const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
+    const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo,
+    int32_t* __restrict__ redo_count) {
   extern __shared__ int32_t s_lds[];
   __shared__ int32_t s_flags[2], s_item;
   const int n_items = *n_items_p;
@@ -803,7 +809,8 @@ __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4
     if (it >= n_items) break;
     const int2 I = items[it];
     CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
-    const PosePart pp = part_of(cx, I.x >> 24, I.y);
+    PosePart pp = part_of(cx, I.x >> 24, I.y);
+    pp.redo_count = redo_count;
     score_item<false>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
                       out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, pp);
   }
@@ -822,9 +829,10 @@ __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4
 __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     const double* __restrict__ poses, const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out,
-    int32_t* __restrict__ redo) {
+    int32_t* __restrict__ redo, const int32_t* __restrict__ redo_count) {
   extern __shared__ int32_t s_lds[];
   __shared__ int32_t s_flags[2];
+  if (*redo_count == 0) return;  // no pose flagged in this launch
   const int n_items = *n_items_p;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int2 I = items[it];
@@ -902,6 +910,9 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
                                                         const int32_t* __restrict__ n_items_p,
                                                         const double* __restrict__ poses,
                                                         uint8_t* __restrict__ bucket, int32_t* __restrict__ hist) {
+  __shared__ int32_t s_hist[kCostBuckets];
+  if (threadIdx.x < kCostBuckets) s_hist[threadIdx.x] = 0;
+  __syncthreads();
   const int n_items = *n_items_p;
   const int lane = threadIdx.x & 63;
   for (int it = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6))); it < n_items;
@@ -916,10 +927,11 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
                 pz = static_cast<float>(oz + p[5]);
     const float reach = static_cast<float>(lig_r) + kCullReach;
     int32_t near = 0;
-    for (int j = pp.r0 + lane; j < pp.r1; j += 64) {
+#pragma unroll 4
+    for (int j = pp.r0 + lane; j < pp.r1; j += 256) {  // every fourth slot: an estimate
       const float4 r = cx.rec_xyzq[j];
       const float dx = r.x - px, dy = r.y - py, dz = r.z - pz;
-      near += dx * dx + dy * dy + dz * dz <= reach * reach ? 1 : 0;
+      near += dx * dx + dy * dy + dz * dz <= reach * reach ? 4 : 0;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) near += __shfl_xor(near, off);
@@ -930,9 +942,11 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
     const int b = cost < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(cost)));
     if (lane == 0) {
       bucket[it] = static_cast<uint8_t>(b);
-      atomicAdd(&hist[b], 1);
+      atomicAdd(&s_hist[b], 1);
     }
   }
+  __syncthreads();
+  if (threadIdx.x < kCostBuckets && s_hist[threadIdx.x] != 0) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
 
 // One workgroup of kCostBuckets threads: bucket b starts after every item of
@@ -942,24 +956,38 @@ __global__ __launch_bounds__(kCostBuckets) void item_order_scan_kernel(int32_t* 
   const int b = threadIdx.x;
   s_n[b] = hist[b];
   hist[b] = 0;
+  if (b == 0) hist[2 * kCostBuckets] = 0;  // poses flagged for the exact variant
   __syncthreads();
   int32_t start = 0;
   for (int c = kCostBuckets - 1; c > b; --c) start += s_n[c];
   hist[kCostBuckets + b] = start;  // cursor
 }
 
+// 256 items per workgroup: places within each bucket from LDS atomics, one
+// global reservation per bucket present in the workgroup.
 __global__ __launch_bounds__(256) void item_scatter_kernel(const int2* __restrict__ items,
                                                            const int32_t* __restrict__ n_items_p,
                                                            const uint8_t* __restrict__ bucket,
                                                            int32_t* __restrict__ hist, int2* __restrict__ sorted) {
+  __shared__ int32_t s_cnt[kCostBuckets], s_base[kCostBuckets];
   const int n_items = *n_items_p;
-  for (int it = blockIdx.x * 256 + threadIdx.x; it < n_items; it += gridDim.x * 256)
-    sorted[atomicAdd(&hist[kCostBuckets + bucket[it]], 1)] = items[it];
+  if (static_cast<int>(blockIdx.x) * 256 >= n_items) return;  // whole workgroup
+  if (threadIdx.x < kCostBuckets) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int it = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = it < n_items;
+  const int b = valid ? bucket[it] : 0;
+  const int local = valid ? atomicAdd(&s_cnt[b], 1) : 0;
+  __syncthreads();
+  if (threadIdx.x < kCostBuckets && s_cnt[threadIdx.x] != 0)
+    s_base[threadIdx.x] = atomicAdd(&hist[kCostBuckets + threadIdx.x], s_cnt[threadIdx.x]);
+  __syncthreads();
+  if (valid) sorted[s_base[b] + local] = items[it];
 }
 
 size_t order_bytes(size_t capacity) {
   const size_t c = std::max<size_t>(capacity, 1);
-  return (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256 + 2 * kCostBuckets * sizeof(int32_t);
+  return (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256 + (2 * kCostBuckets + 1) * sizeof(int32_t);
 }
 
 void bind_order(BatchLaunch* b, char* mem, size_t capacity) {
@@ -967,6 +995,7 @@ void bind_order(BatchLaunch* b, char* mem, size_t capacity) {
   b->sorted = reinterpret_cast<int2*>(mem);
   b->bucket = reinterpret_cast<uint8_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256);
   b->hist = reinterpret_cast<int32_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256);
+  b->redo_count = b->hist + 2 * kCostBuckets;
 }
 
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream) {
@@ -974,12 +1003,12 @@ hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stre
   hipLaunchKernelGGL(item_cost_kernel, dim3(std::min((cap + 3) / 4, 4096)), dim3(256), 0, stream, b.table, b.items,
                      b.n_items, b.poses, b.bucket, b.hist);
   hipLaunchKernelGGL(item_order_scan_kernel, dim3(1), dim3(kCostBuckets), 0, stream, b.hist);
-  hipLaunchKernelGGL(item_scatter_kernel, dim3(std::min((cap + 255) / 256, 1024)), dim3(256), 0, stream, b.items,
-                     b.n_items, b.bucket, b.hist, b.sorted);
+  hipLaunchKernelGGL(item_scatter_kernel, dim3((cap + 255) / 256), dim3(256), 0, stream, b.items, b.n_items,
+                     b.bucket, b.hist, b.sorted);
   hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.sorted,
-                     b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo);
+                     b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo, b.redo_count);
   hipLaunchKernelGGL(score_batch_exact_kernel, dim3(std::min(b.capacity, b.grid)), dim3(kScoreThreads), b.lds, stream,
-                     b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo);
+                     b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo, b.redo_count);
   return hipGetLastError();
 }
 

@@ -67,6 +67,28 @@ def test_batch_matches_oracle_and_replays(uneven):
         assert np.array_equal(ns1[a:z], sns)
 
 
+def test_results_do_not_depend_on_the_item_order(uneven):
+    """Each launch orders its items by the poses' estimated cost (and within a
+    cost bucket in no fixed order): the same pose gives the same bits whatever
+    the order, its row or the other poses of the launch."""
+    counts = [6, 6, 0, 6, 6, 6, 6]
+    b = Batch([Complex(r, l, p) for r, l, p in uneven], counts)
+    genes = np.concatenate([_genes(i, n) if n else np.zeros((0, 6)) for i, n in enumerate(counts)])
+    out1, ns1 = b.score(genes)
+    rev = genes.copy()
+    for i in range(len(counts)):
+        a, z = b.offsets[i], b.offsets[i + 1]
+        rev[a:z] = genes[a:z][::-1]
+    out_r, ns_r = b.score(rev)
+    other = np.concatenate([_genes(40 + i, n) if n else np.zeros((0, 6)) for i, n in enumerate(counts)])
+    b.score(other)
+    out3, ns3 = b.score(genes)
+    for i in range(len(counts)):
+        a, z = b.offsets[i], b.offsets[i + 1]
+        assert np.array_equal(out_r[a:z][::-1], out1[a:z]) and np.array_equal(ns_r[a:z][::-1], ns1[a:z]), i
+    assert np.array_equal(out3, out1) and np.array_equal(ns3, ns1)
+
+
 def _lattice(mol):
     """The molecule with coordinates on a 0.5 A lattice: many pairs sit on the
     cutoffs, so the settle buffers fill and overflow."""
