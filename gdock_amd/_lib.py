@@ -131,6 +131,7 @@ def lib() -> C.CDLL:
         "gdock_batch_destroy": (None, [P, P]),
         "gdock_batch_n_poses": (C.c_int32, [P]),
         "gdock_batch_plan": (C.c_int, [P, I32, I32, I32, I32]),
+        "gdock_batch_last_items": (C.c_int, [P, I32]),
         "gdock_batch_score": (C.c_int, [P, P, D, C.POINTER(Weights), D, I32]),
         "gdock_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "gdock_batch_launch": (C.c_int, [P, P, C.POINTER(Weights), P]),

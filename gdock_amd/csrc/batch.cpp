@@ -2,10 +2,12 @@
 // captured HIP graph, with a measured choice of ligand atoms per lane.
 //
 // A batch is a fixed set of complexes with a fixed number of poses each (one
-// GA generation of every complex).  Its items (complex, pose) are ordered
-// longest first; one persistent kernel drains them from a device counter, and
-// a second kernel rescores the rare items whose settle buffer overflowed.
-// Both launches are captured once into a graph; every generation replays it.
+// GA generation of every complex).  Each launch plans its work items from the
+// poses (a pose split over receptor ranges when its estimated cost is high,
+// items ordered most expensive first); one persistent kernel drains them from
+// a device counter, and a second kernel rescores the rare poses whose settle
+// buffer overflowed.  The launches are captured once into a graph; every
+// generation replays it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,11 +30,12 @@ struct gdock_batch {
   std::vector<int32_t> pose_off;  // n + 1
   std::vector<int32_t> lanes_k;
   std::vector<int32_t> parts;
-  std::vector<int2> items;
+  std::vector<int2> pose_list;  // {complex, pose row} per pose
+  int64_t item_capacity = 0;
   int32_t n_poses = 0;
   int32_t max_n_rec = 0;
   BatchLaunch L{};
-  void* mem = nullptr;  // device: table, items, n_items, counters, poses, out, nsat, redo
+  void* mem = nullptr;  // device: table, pose list, counts, poses, out, nsat, redo, items, plan
   double* h_poses = nullptr;  // pinned host staging
   double* h_out = nullptr;
   int32_t* h_nsat = nullptr;
@@ -71,19 +74,16 @@ std::vector<int> lane_candidates(int n_lig) {
 int rec_size_class(int n_rec) { return static_cast<int>(std::lround(2.0 * std::log2(std::max(n_rec, 1)))); }
 }  // namespace
 
-// Workgroups per pose in batched launches: a pose of 8e6 padded pairs or more
-// is split by receptor range into parts of about 4e6 pairs (at least 512
-// receptor slots each, at most kMaxPoseParts), so that a large complex's poses spread over more CUs when a
-// launch holds few of them.  A function of the complex's shape only, so a
-// complex is scored the same way in any batch.
-#ifndef GDOCK_SPLIT_PAIRS
-#define GDOCK_SPLIT_PAIRS 4000000
-#endif
+// Largest number of workgroups one pose of a complex may be split over in
+// batched launches: enough parts of kSplitCost for a pose with every
+// receptor slot within reach, at least 512 receptor slots each, at most
+// kMaxPoseParts.  Each launch picks every pose's own count up to this
+// (pose_plan_kernel).  A function of the complex's shape only.
 int pose_parts(const DevComplex& d, int lanes_k) {
   const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * std::max(lanes_k, 1);
-  const int64_t chunks = (d.n_lig + per_chunk - 1) / per_chunk;
-  const int64_t pairs = static_cast<int64_t>(d.n_rec) * chunks * per_chunk;
-  int64_t p = std::min<int64_t>(pairs / GDOCK_SPLIT_PAIRS, d.n_rec / 512);
+  const double slots = static_cast<double>((d.n_lig + per_chunk - 1) / per_chunk * per_chunk);
+  const double worst = 1.0625 * d.n_rec * slots;
+  const int64_t p = std::min<int64_t>(static_cast<int64_t>(std::ceil(worst / kSplitCost)), d.n_rec / 512);
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p, kMaxPoseParts)));
 }
 
@@ -170,7 +170,7 @@ extern "C" {
 
 gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes, int32_t n,
                                 const int32_t* poses_per_complex, gdock_batch** out) {
-  if (!ctx || !out || n < 0 || n >= (1 << 24) || (n > 0 && (!complexes || !poses_per_complex))) {
+  if (!ctx || !out || n < 0 || n > kItemSlotMask + 1 || (n > 0 && (!complexes || !poses_per_complex))) {
     if (ctx) set_error(ctx, "batch_create: invalid arguments");
     return GDOCK_ERR_INVALID_ARG;
   }
@@ -211,30 +211,18 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   }
 
   b->parts.resize(n);
-  int64_t n_items = 0;
   for (int32_t i = 0; i < n; ++i) {
     b->parts[i] = pose_parts(b->cx[i]->dev, b->lanes_k[i]);
-    n_items += static_cast<int64_t>(poses_per_complex[i]) * b->parts[i];
+    b->item_capacity += static_cast<int64_t>(poses_per_complex[i]) * b->parts[i];
   }
-  if (n_items > (1 << 30)) {
+  if (b->item_capacity > (1 << 30)) {
+    set_error(ctx, "batch_create: %lld work items is too many", static_cast<long long>(b->item_capacity));
     delete b;
-    set_error(ctx, "batch_create: %lld work items is too many", static_cast<long long>(n_items));
     return GDOCK_ERR_INVALID_ARG;
   }
-  // Items longest first: cost = receptor slots x ligand lane-slots walked,
-  // per part; the parts of a pose are consecutive items.
-  std::vector<int32_t> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  auto cost = [&](int32_t i) {
-    const DevComplex& d = b->cx[i]->dev;
-    const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * b->lanes_k[i];
-    const int64_t chunks = (d.n_lig + per_chunk - 1) / per_chunk;
-    return static_cast<int64_t>(d.n_rec) * chunks * per_chunk / b->parts[i];
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost(x) > cost(y); });
-  for (int32_t i : order)
-    for (int32_t p = b->pose_off[i]; p < b->pose_off[i + 1]; ++p)
-      for (int32_t q = 0; q < b->parts[i]; ++q) b->items.push_back(make_int2(i | (q << 24), p));
+  // Every launch plans its own work items from this list (pose_plan_kernel).
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t p = b->pose_off[i]; p < b->pose_off[i + 1]; ++p) b->pose_list.push_back(make_int2(i, p));
 
   // Device memory.
   std::vector<DevComplex> table(std::max(n, 1));
@@ -243,11 +231,12 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
     table[i].lanes_k = b->lanes_k[i];
   }
   const size_t np = std::max<int64_t>(total, 1);
-  const size_t ni = std::max<size_t>(b->items.size(), 1);
+  const size_t ni = std::max<int64_t>(b->item_capacity, 1);
   Layout Ly;
-  const size_t o_tab = Ly.add<DevComplex>(table.size()), o_items = Ly.add<int2>(ni), o_n = Ly.add<int32_t>(4),
+  const size_t o_tab = Ly.add<DevComplex>(table.size()), o_list = Ly.add<int2>(np), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(np * kPoseDoubles), o_out = Ly.add<double>(np * kOutDoubles),
-               o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np), o_order = Ly.add<uint8_t>(order_bytes(ni));
+               o_ns = Ly.add<int32_t>(np), o_redo = Ly.add<int32_t>(np), o_items = Ly.add<int2>(ni),
+               o_order = Ly.add<uint8_t>(order_bytes(np));
   std::vector<size_t> o_split(n, 0);
   const size_t split_begin = Ly.size;
   for (int32_t i = 0; i < n; ++i) {
@@ -267,14 +256,16 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   for (int32_t i = 0; i < n; ++i)
     bind_split_state(&table[i], b->parts[i], poses_per_complex[i], b->pose_off[i],
                      b->parts[i] > 1 ? m + o_split[i] : nullptr);
-  int32_t head[4] = {static_cast<int32_t>(b->items.size()), 0, 0, 0};  // n_items, counter, exit count
+  // n_items (written by each launch's plan), counter, exit count, pose count
+  int32_t head[4] = {0, 0, 0, static_cast<int32_t>(b->pose_list.size())};
   std::vector<uint8_t> blob(o_poses, 0);
   std::memcpy(blob.data() + o_tab, table.data(), table.size() * sizeof(DevComplex));
-  if (!b->items.empty()) std::memcpy(blob.data() + o_items, b->items.data(), b->items.size() * sizeof(int2));
+  if (!b->pose_list.empty())
+    std::memcpy(blob.data() + o_list, b->pose_list.data(), b->pose_list.size() * sizeof(int2));
   std::memcpy(blob.data() + o_n, head, sizeof head);
   e = hipMemcpyAsync(m, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemsetAsync(m + o_ns, 0, (o_redo - o_ns) + np * sizeof(int32_t), ctx->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(m + o_order, 0, order_bytes(ni), ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(m + o_order, 0, order_bytes(np), ctx->stream);
   if (e == hipSuccess && split_end > split_begin)  // split state starts cleared; the kernel clears it after use
     e = hipMemsetAsync(m + split_begin, 0, split_end - split_begin, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -284,18 +275,21 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   }
   BatchLaunch& L = b->L;
   L.table = reinterpret_cast<const DevComplex*>(m + o_tab);
-  L.items = reinterpret_cast<const int2*>(m + o_items);
-  L.n_items = reinterpret_cast<const int32_t*>(m + o_n);
+  L.pose_list = reinterpret_cast<const int2*>(m + o_list);
+  L.n_pose_list = reinterpret_cast<const int32_t*>(m + o_n) + 3;
+  L.items = reinterpret_cast<int2*>(m + o_items);
+  L.n_items = reinterpret_cast<int32_t*>(m + o_n);
   L.counter = reinterpret_cast<int32_t*>(m + o_n) + 1;
   L.exit_count = reinterpret_cast<int32_t*>(m + o_n) + 2;
   L.poses = reinterpret_cast<const double*>(m + o_poses);
   L.out = reinterpret_cast<double*>(m + o_out);
   L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
   L.redo = reinterpret_cast<int32_t*>(m + o_redo);
-  bind_order(&L, m + o_order, ni);
+  bind_order(&L, m + o_order, np);
   L.lds = batch_lds_bytes(max_rec);
-  L.capacity = static_cast<int>(std::max<size_t>(b->items.size(), 1));
-  L.grid = std::min(batch_grid(L.lds, ctx->device), L.capacity);
+  L.capacity = static_cast<int>(np);
+  L.item_capacity = static_cast<int>(ni);
+  L.grid = std::min(batch_grid(L.lds, ctx->device), L.item_capacity);
   *out = b;
   return GDOCK_OK;
 }
@@ -324,6 +318,16 @@ gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* p
   return GDOCK_OK;
 }
 
+gdock_status gdock_batch_last_items(const gdock_batch* b, int32_t* n_items) {
+  if (!b || !n_items) return GDOCK_ERR_INVALID_ARG;
+  *n_items = 0;
+  if (b->pose_list.empty()) return GDOCK_OK;
+  hipError_t e = hipSetDevice(b->ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(n_items, b->L.n_items, sizeof(int32_t), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(b->ctx, e, "batch_last_items");
+}
+
 gdock_status gdock_batch_device_buffers(gdock_batch* b, double** d_poses, double** d_out, int32_t** d_nsat) {
   if (!b) return GDOCK_ERR_INVALID_ARG;
   if (d_poses) *d_poses = const_cast<double*>(b->L.poses);
@@ -338,7 +342,7 @@ gdock_status gdock_batch_launch(gdock_ctx* ctx, gdock_batch* b, const gdock_weig
     if (ctx) set_error(ctx, "batch_launch: invalid arguments");
     return GDOCK_ERR_INVALID_ARG;
   }
-  if (b->items.empty()) return GDOCK_OK;
+  if (b->pose_list.empty()) return GDOCK_OK;
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
   hipError_t e = hipSetDevice(ctx->device);

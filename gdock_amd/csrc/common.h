@@ -72,13 +72,14 @@ struct DevComplex {
   int32_t n_lig, n_class, n_restr;
   int32_t n_coll;       // pairs in coll_pairs
   int32_t lanes_k;      // ligand atoms per lane chosen for batched launches (0: ceil(n_lig / 256), at most 8)
-  // batched launches: a pose split across `parts` workgroups by receptor range
-  // (1: not split), with the split state of the complex's poses, indexed by
-  // global pose row - split_pose_base
+  // batched launches: a pose may be split across up to `parts` workgroups by
+  // receptor range (1: never split; each launch picks every pose's count),
+  // with the split state of the complex's poses, indexed by global pose row -
+  // split_pose_base
   int32_t parts;
   int32_t split_pose_base;
   int32_t* split_lig;   // n_lig ints per pose
-  double* split_sums;   // 3 doubles per part per pose
+  double* split_sums;   // 3 doubles per part (up to `parts`) per pose
   int32_t* split_sync;  // 2 ints per pose
   double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
   double origin[3];  // FP32 frame origin
@@ -91,25 +92,35 @@ struct Weights { double vdw, elec, desolv, air; };
 // Cost buckets of the per-launch item order (half-octave steps of the
 // estimated pair work).
 constexpr int kCostBuckets = 64;
+// Estimated cost of one work item (receptor slots within reach of the
+// ligand, plus 1/16 of the slots walked for the per-tile tests, times the
+// ligand lane-slots): a pose estimated above it is split by receptor range
+// into ceil(cost / kSplitCost) parts, up to the complex's largest part count.
+constexpr double kSplitCost = 1.5e6;
+constexpr int kItemPartShift = 20, kItemPartsShift = 24;
+constexpr int kItemSlotMask = (1 << kItemPartShift) - 1;  // complex slots per launch table
 
 // Device buffers of one batched (persistent) scoring launch.
 struct BatchLaunch {
-  const DevComplex* table;  // complex slots (device)
-  const int2* items;        // {slot, global pose}, longest first
-  const int32_t* n_items;   // device-resident item count
+  const DevComplex* table;     // complex slots (device)
+  const int2* pose_list;       // {complex slot, pose row} per pose of the launch
+  const int32_t* n_pose_list;  // device-resident pose count
+  // Work items, written per launch by the planning kernels: {slot | part <<
+  // kItemPartShift | parts << kItemPartsShift, pose row}, most expensive first.
+  int2* items;
+  int32_t* n_items;
   int32_t* counter;         // work counter and exit count: zero between launches
   int32_t* exit_count;
   const double* poses;      // pose records, kPoseDoubles each
   double* out;              // kOutDoubles per pose
   int32_t* nsat;
   int32_t* redo;
-  // per launch, the items reordered by estimated cost (most expensive first)
-  int2* sorted;             // capacity items
-  uint8_t* bucket;          // capacity: cost bucket of each item
+  uint16_t* plan;           // per listed pose: cost bucket | parts << 8
   int32_t* hist;            // kCostBuckets counts, then kCostBuckets cursors: zero between launches
   int32_t* redo_count;      // poses of the launch flagged for the exact variant
   int grid;                 // persistent workgroups
-  int capacity;             // item capacity (grid of the exact pass)
+  int capacity;             // pose capacity of the list
+  int item_capacity;        // item capacity (capacity x the largest part count)
   size_t lds;
 };
 

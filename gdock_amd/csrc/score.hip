@@ -773,26 +773,28 @@ __device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ 
   }
 }
 
-// Persistent scoring of a batch: every workgroup takes items from a shared
-// counter until the list (longest first) is exhausted.  The item count is
-// read from device memory so that one captured launch serves any number of
-// items up to the capacity.  The last workgroup to leave resets the counter
-// and the exit count, so the next launch starts from zero with no memset.
-__device__ __forceinline__ PosePart part_of(CxRef cx, int part, int pose_row) {
-  if (cx.parts <= 1) return PosePart{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
+// Part `part` of a pose split into `parts` (1: the whole pose): an equal share
+// of the receptor tiles; the split state has room for cx.parts parts.
+__device__ __forceinline__ PosePart part_of(CxRef cx, int part, int parts, int pose_row) {
+  if (parts <= 1) return PosePart{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
   const int tiles = (cx.n_rec + 63) / 64;
-  const int t0 = part * tiles / cx.parts, t1 = (part + 1) * tiles / cx.parts;
+  const int t0 = part * tiles / parts, t1 = (part + 1) * tiles / parts;
   const size_t q = static_cast<size_t>(pose_row - cx.split_pose_base);
-  return PosePart{64 * t0, min(64 * t1, cx.n_rec), part, cx.parts, cx.split_lig + q * cx.n_lig,
+  return PosePart{64 * t0, min(64 * t1, cx.n_rec), part, parts, cx.split_lig + q * cx.n_lig,
                   cx.split_sums + q * 3 * cx.parts, cx.split_sync + q * 2};
 }
 
+__device__ __forceinline__ int item_slot(int2 I) { return I.x & kItemSlotMask; }
+__device__ __forceinline__ int item_part(int2 I) { return (I.x >> kItemPartShift) & 15; }
+__device__ __forceinline__ int item_parts(int2 I) { return (I.x >> kItemPartsShift) & 15; }
+
 // Persistent scoring of a batch: every workgroup takes items from a shared
-// counter until the list (longest first) is exhausted.  An item is
-// {complex slot | part << 24, global pose row}.  The item count is read from
-// device memory so that one captured launch serves any number of items up to
-// the capacity.  The last workgroup to leave resets the counter and the exit
-// count, so the next launch starts from zero with no memset.
+// counter until the list (most expensive first) is exhausted.  An item is
+// {complex slot | part << kItemPartShift | parts << kItemPartsShift, global
+// pose row}.  The item count is read from device memory so that one captured
+// launch serves any number of items up to the capacity.  The last workgroup
+// to leave resets the counter and the exit count, so the next launch starts
+// from zero with no memset.
 __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void score_batch_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     int32_t* __restrict__ counter, int32_t* __restrict__ exit_count, const double* __restrict__ poses,
@@ -808,8 +810,8 @@ __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4
     __syncthreads();
     if (it >= n_items) break;
     const int2 I = items[it];
-    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
-    PosePart pp = part_of(cx, I.x >> 24, I.y);
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[item_slot(I)];
+    PosePart pp = part_of(cx, item_part(I), item_parts(I), I.y);
     pp.redo_count = redo_count;
     score_item<false>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
                       out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, pp);
@@ -836,8 +838,8 @@ __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
   const int n_items = *n_items_p;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int2 I = items[it];
-    if ((I.x >> 24) != 0 || redo[I.y] == 0) continue;  // uniform per workgroup
-    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
+    if (item_part(I) != 0 || redo[I.y] == 0) continue;  // uniform per workgroup
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[item_slot(I)];
     const PosePart whole{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
     score_item<true>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
                      out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, whole);
@@ -895,32 +897,35 @@ int batch_grid(size_t lds, int device) {
   return per_cu * cus;
 }
 
-// Per-launch order of the items, most expensive first, so that the
+// Per-launch plan of the work items, most expensive first, so that the
 // persistent workgroups start the long items early and the launch does not
 // end on a long item taken late.  The cost of a pose varies twentyfold
 // within one complex (a pose inside the receptor against one far from it),
-// so it is estimated from the pose itself: the receptor slots of the item's
-// range within the ligand's radius + kCullReach of the pose's ligand centre
-// (the rotation keeps the centre; the genes move it), plus a share of the
-// range for the per-tile tests, times the ligand lane-slots walked.  Items
-// are counting-sorted into half-octave buckets of that cost.  The order
-// changes nothing in the results: every pose is computed by its own items.
-__global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __restrict__ table,
-                                                        const int2* __restrict__ items,
-                                                        const int32_t* __restrict__ n_items_p,
+// so it is estimated from the pose itself: the receptor slots within the
+// ligand's radius + kCullReach of the pose's ligand centre (the rotation keeps
+// the centre; the genes move it; every fourth slot is tested), plus 1/16 of
+// the slots for the per-tile tests, times the ligand lane-slots walked.  A pose
+// estimated above kSplitCost is split into ceil(cost / kSplitCost) items over
+// receptor ranges (at most the complex's part count), so that no single item
+// holds the launch; cheap poses stay whole, as splitting has a cost of its
+// own.  Items are counting-sorted into half-octave buckets of their share of
+// the cost.  The plan is a function of the pose and the complex only, so a
+// pose gives the same bits in any launch and any mix.
+__global__ __launch_bounds__(256) void pose_plan_kernel(const DevComplex* __restrict__ table,
+                                                        const int2* __restrict__ pose_list,
+                                                        const int32_t* __restrict__ n_list_p,
                                                         const double* __restrict__ poses,
-                                                        uint8_t* __restrict__ bucket, int32_t* __restrict__ hist) {
+                                                        uint16_t* __restrict__ plan, int32_t* __restrict__ hist) {
   __shared__ int32_t s_hist[kCostBuckets];
   if (threadIdx.x < kCostBuckets) s_hist[threadIdx.x] = 0;
   __syncthreads();
-  const int n_items = *n_items_p;
+  const int n_list = *n_list_p;
   const int lane = threadIdx.x & 63;
-  for (int it = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6))); it < n_items;
-       it += gridDim.x * 4) {  // one wave per item
-    const int2 I = items[it];
-    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[I.x & 0xFFFFFF];
-    const PosePart pp = part_of(cx, I.x >> 24, I.y);
-    const double* p = poses + static_cast<size_t>(kPoseDoubles) * I.y;
+  for (int i = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6))); i < n_list;
+       i += gridDim.x * 4) {  // one wave per pose
+    const int2 P = pose_list[i];
+    CxRef cx = ((const DevComplex GDOCK_AS4*)table)[P.x];
+    const double* p = poses + static_cast<size_t>(kPoseDoubles) * P.y;
     const double ox = cx.com[0] - cx.origin[0], oy = cx.com[1] - cx.origin[1], oz = cx.com[2] - cx.origin[2];
     const double lig_r = cx.lig_extent - fmax(fabs(ox), fmax(fabs(oy), fabs(oz)));
     const float px = static_cast<float>(ox + p[3]), py = static_cast<float>(oy + p[4]),
@@ -928,7 +933,7 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
     const float reach = static_cast<float>(lig_r) + kCullReach;
     int32_t near = 0;
 #pragma unroll 4
-    for (int j = pp.r0 + lane; j < pp.r1; j += 256) {  // every fourth slot: an estimate
+    for (int j = lane; j < cx.n_rec; j += 256) {  // every fourth slot: an estimate
       const float4 r = cx.rec_xyzq[j];
       const float dx = r.x - px, dy = r.y - py, dz = r.z - pz;
       near += dx * dx + dy * dy + dz * dz <= reach * reach ? 4 : 0;
@@ -938,11 +943,13 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
     const int k = cx.lanes_k > 0 ? cx.lanes_k : 1;
     const int per_chunk = kScoreThreads * k;
     const double slots = static_cast<double>((cx.n_lig + per_chunk - 1) / per_chunk) * per_chunk;
-    const double cost = (near + 0.0625 * (pp.r1 - pp.r0)) * slots;
-    const int b = cost < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(cost)));
+    const double cost = (near + 0.0625 * cx.n_rec) * slots;
+    const int parts = min(max(cx.parts, 1), max(1, static_cast<int>(ceil(cost / kSplitCost))));
+    const double share = cost / parts;
+    const int b = share < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(share)));
     if (lane == 0) {
-      bucket[it] = static_cast<uint8_t>(b);
-      atomicAdd(&s_hist[b], 1);
+      plan[i] = static_cast<uint16_t>(b | (parts << 8));
+      atomicAdd(&s_hist[b], parts);
     }
   }
   __syncthreads();
@@ -950,8 +957,10 @@ __global__ __launch_bounds__(256) void item_cost_kernel(const DevComplex* __rest
 }
 
 // One workgroup of kCostBuckets threads: bucket b starts after every item of
-// the costlier buckets; the counts are cleared for the next launch.
-__global__ __launch_bounds__(kCostBuckets) void item_order_scan_kernel(int32_t* __restrict__ hist) {
+// the costlier buckets; the item count of the launch is the sum; the counts
+// are cleared for the next launch.
+__global__ __launch_bounds__(kCostBuckets) void item_order_scan_kernel(int32_t* __restrict__ hist,
+                                                                       int32_t* __restrict__ n_items) {
   __shared__ int32_t s_n[kCostBuckets];
   const int b = threadIdx.x;
   s_n[b] = hist[b];
@@ -961,54 +970,60 @@ __global__ __launch_bounds__(kCostBuckets) void item_order_scan_kernel(int32_t* 
   int32_t start = 0;
   for (int c = kCostBuckets - 1; c > b; --c) start += s_n[c];
   hist[kCostBuckets + b] = start;  // cursor
+  if (b == 0) *n_items = start + s_n[0];
 }
 
-// 256 items per workgroup: places within each bucket from LDS atomics, one
-// global reservation per bucket present in the workgroup.
-__global__ __launch_bounds__(256) void item_scatter_kernel(const int2* __restrict__ items,
-                                                           const int32_t* __restrict__ n_items_p,
-                                                           const uint8_t* __restrict__ bucket,
-                                                           int32_t* __restrict__ hist, int2* __restrict__ sorted) {
+// 256 poses per workgroup: each writes its items at places within its bucket
+// from LDS atomics, with one global reservation per bucket present in the
+// workgroup.
+__global__ __launch_bounds__(256) void item_scatter_kernel(const int2* __restrict__ pose_list,
+                                                           const int32_t* __restrict__ n_list_p,
+                                                           const uint16_t* __restrict__ plan,
+                                                           int32_t* __restrict__ hist, int2* __restrict__ items) {
   __shared__ int32_t s_cnt[kCostBuckets], s_base[kCostBuckets];
-  const int n_items = *n_items_p;
-  if (static_cast<int>(blockIdx.x) * 256 >= n_items) return;  // whole workgroup
+  const int n_list = *n_list_p;
+  if (static_cast<int>(blockIdx.x) * 256 >= n_list) return;  // whole workgroup
   if (threadIdx.x < kCostBuckets) s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int it = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = it < n_items;
-  const int b = valid ? bucket[it] : 0;
-  const int local = valid ? atomicAdd(&s_cnt[b], 1) : 0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < n_list;
+  const int pl = valid ? plan[i] : 0;
+  const int b = pl & 0xFF, parts = pl >> 8;
+  const int local = valid ? atomicAdd(&s_cnt[b], parts) : 0;
   __syncthreads();
   if (threadIdx.x < kCostBuckets && s_cnt[threadIdx.x] != 0)
     s_base[threadIdx.x] = atomicAdd(&hist[kCostBuckets + threadIdx.x], s_cnt[threadIdx.x]);
   __syncthreads();
-  if (valid) sorted[s_base[b] + local] = items[it];
+  if (valid) {
+    const int2 P = pose_list[i];
+    for (int q = 0; q < parts; ++q)
+      items[s_base[b] + local + q] = make_int2(P.x | (q << kItemPartShift) | (parts << kItemPartsShift), P.y);
+  }
 }
 
 size_t order_bytes(size_t capacity) {
   const size_t c = std::max<size_t>(capacity, 1);
-  return (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256 + (2 * kCostBuckets + 1) * sizeof(int32_t);
+  return (c * sizeof(uint16_t) + 255) / 256 * 256 + (2 * kCostBuckets + 1) * sizeof(int32_t);
 }
 
 void bind_order(BatchLaunch* b, char* mem, size_t capacity) {
   const size_t c = std::max<size_t>(capacity, 1);
-  b->sorted = reinterpret_cast<int2*>(mem);
-  b->bucket = reinterpret_cast<uint8_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256);
-  b->hist = reinterpret_cast<int32_t*>(mem + (c * sizeof(int2) + 255) / 256 * 256 + (c + 255) / 256 * 256);
+  b->plan = reinterpret_cast<uint16_t*>(mem);
+  b->hist = reinterpret_cast<int32_t*>(mem + (c * sizeof(uint16_t) + 255) / 256 * 256);
   b->redo_count = b->hist + 2 * kCostBuckets;
 }
 
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream) {
   const int cap = std::max(b.capacity, 1);
-  hipLaunchKernelGGL(item_cost_kernel, dim3(std::min((cap + 3) / 4, 4096)), dim3(256), 0, stream, b.table, b.items,
-                     b.n_items, b.poses, b.bucket, b.hist);
-  hipLaunchKernelGGL(item_order_scan_kernel, dim3(1), dim3(kCostBuckets), 0, stream, b.hist);
-  hipLaunchKernelGGL(item_scatter_kernel, dim3((cap + 255) / 256), dim3(256), 0, stream, b.items, b.n_items,
-                     b.bucket, b.hist, b.sorted);
-  hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.sorted,
+  hipLaunchKernelGGL(pose_plan_kernel, dim3(std::min((cap + 3) / 4, 4096)), dim3(256), 0, stream, b.table,
+                     b.pose_list, b.n_pose_list, b.poses, b.plan, b.hist);
+  hipLaunchKernelGGL(item_order_scan_kernel, dim3(1), dim3(kCostBuckets), 0, stream, b.hist, b.n_items);
+  hipLaunchKernelGGL(item_scatter_kernel, dim3((cap + 255) / 256), dim3(256), 0, stream, b.pose_list, b.n_pose_list,
+                     b.plan, b.hist, b.items);
+  hipLaunchKernelGGL(score_batch_kernel, dim3(b.grid), dim3(kScoreThreads), b.lds, stream, b.table, b.items,
                      b.n_items, b.counter, b.exit_count, b.poses, w, b.out, b.nsat, b.redo, b.redo_count);
-  hipLaunchKernelGGL(score_batch_exact_kernel, dim3(std::min(b.capacity, b.grid)), dim3(kScoreThreads), b.lds, stream,
-                     b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo, b.redo_count);
+  hipLaunchKernelGGL(score_batch_exact_kernel, dim3(std::min(b.item_capacity, b.grid)), dim3(kScoreThreads), b.lds,
+                     stream, b.table, b.items, b.n_items, b.poses, w, b.out, b.nsat, b.redo, b.redo_count);
   return hipGetLastError();
 }
 

@@ -68,7 +68,7 @@ struct gdock_server {
   void* mem = nullptr;
   double* h_poses = nullptr;  // pinned
   double* h_out = nullptr;
-  int2* h_items = nullptr;
+  int2* h_list = nullptr;  // {slot, pose row} per pose of a step
   DevComplex* h_table = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -195,34 +195,28 @@ bool gdock_server::step() {
   if (active.empty()) return true;
   const auto t_step = std::chrono::steady_clock::now();
 
-  // Items longest first; pose rows slot * max_pop + p.
-  std::stable_sort(active.begin(), active.end(), [](Job* a, Job* b) {
-    auto cost = [](Job* j) {
-      const int64_t lanes = static_cast<int64_t>(kScoreThreads) * j->lanes;
-      return static_cast<int64_t>(j->cx->dev.n_rec) * ((j->cx->dev.n_lig + lanes - 1) / lanes) * lanes / j->parts;
-    };
-    return cost(a) > cost(b);
-  });
-  int n_items = 0, max_row = 0, step_poses = 0;
+  // The poses of the step (rows slot * max_pop + p); the launch plans its
+  // work items from them.
+  int n_list = 0, max_row = 0, step_poses = 0;
   for (Job* j : active) {
     const GaJob& ga = j->result->job;
     const int pop = ga.population();
     const int row0 = j->slot * max_pop;
     gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
-    for (int p = 0; p < pop; ++p)
-      for (int q = 0; q < j->parts; ++q) h_items[n_items++] = make_int2(j->slot | (q << 24), row0 + p);
+    for (int p = 0; p < pop; ++p) h_list[n_list++] = make_int2(j->slot, row0 + p);
     max_row = std::max(max_row, row0 + pop);
     step_poses += pop;
   }
-  int32_t head = n_items;
+  int32_t head = n_list;
   hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
                                 static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(const_cast<int2*>(L.items), h_items, static_cast<size_t>(n_items) * sizeof(int2),
+    e = hipMemcpyAsync(const_cast<int2*>(L.pose_list), h_list, static_cast<size_t>(n_list) * sizeof(int2),
                        hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(const_cast<int32_t*>(L.n_items), &head, sizeof head, hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(const_cast<int32_t*>(L.n_pose_list), &head, sizeof head, hipMemcpyHostToDevice,
+                       ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
   if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
@@ -294,7 +288,8 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   gdock_server_config c;
   if (config) c = *config; else gdock_server_default_config(&c);
   if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_population > 65536 || c.max_receptor_atoms < 1 ||
-      c.max_receptor_atoms > kMaxLdsReceptor - 64 || c.memory_budget < 0)
+      c.max_receptor_atoms > kMaxLdsReceptor - 64 || c.memory_budget < 0 ||
+      static_cast<int64_t>(c.max_jobs) * c.max_population * kMaxPoseParts > (int64_t{1} << 30))
     return GDOCK_ERR_INVALID_ARG;
   auto* s = new gdock_server();
   s->cfg = c;
@@ -311,15 +306,16 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   const size_t cap = static_cast<size_t>(c.max_jobs) * c.max_population;
   const size_t cap_items = cap * kMaxPoseParts;  // a pose is at most kMaxPoseParts items
   Layout Ly;
-  const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_items = Ly.add<int2>(cap_items), o_n = Ly.add<int32_t>(4),
+  const size_t o_tab = Ly.add<DevComplex>(c.max_jobs), o_list = Ly.add<int2>(cap),
+               o_items = Ly.add<int2>(cap_items), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(cap * kPoseDoubles), o_out = Ly.add<double>(cap * kOutDoubles),
                o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap),
-               o_order = Ly.add<uint8_t>(order_bytes(cap_items));
+               o_order = Ly.add<uint8_t>(order_bytes(cap));
   hipError_t e = hipMalloc(&s->mem, Ly.size);
   if (e == hipSuccess) e = hipMemsetAsync(s->mem, 0, Ly.size, s->ctx->stream);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), cap * kOutDoubles * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items), cap_items * sizeof(int2));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_list), cap * sizeof(int2));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_table), c.max_jobs * sizeof(DevComplex));
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
   if (e == hipSuccess) e = hipEventCreate(&s->ev0);
@@ -328,18 +324,21 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   BatchLaunch& L = s->L;
   if (e == hipSuccess) {
     L.table = reinterpret_cast<const DevComplex*>(m + o_tab);
-    L.items = reinterpret_cast<const int2*>(m + o_items);
-    L.n_items = reinterpret_cast<const int32_t*>(m + o_n);
+    L.pose_list = reinterpret_cast<const int2*>(m + o_list);
+    L.n_pose_list = reinterpret_cast<const int32_t*>(m + o_n) + 3;
+    L.items = reinterpret_cast<int2*>(m + o_items);
+    L.n_items = reinterpret_cast<int32_t*>(m + o_n);
     L.counter = reinterpret_cast<int32_t*>(m + o_n) + 1;
     L.exit_count = reinterpret_cast<int32_t*>(m + o_n) + 2;
     L.poses = reinterpret_cast<const double*>(m + o_poses);
     L.out = reinterpret_cast<double*>(m + o_out);
     L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
     L.redo = reinterpret_cast<int32_t*>(m + o_redo);
-    bind_order(&L, m + o_order, cap_items);
+    bind_order(&L, m + o_order, cap);
     L.lds = batch_lds_bytes(c.max_receptor_atoms + 64);
-    L.capacity = static_cast<int>(cap_items);
-    L.grid = std::min(batch_grid(L.lds, device), L.capacity);
+    L.capacity = static_cast<int>(cap);
+    L.item_capacity = static_cast<int>(cap_items);
+    L.grid = std::min(batch_grid(L.lds, device), L.item_capacity);
     // One graph for every step: the kernels read the item count from device memory.
     hipStream_t cs = nullptr;
     const Weights w{c.weights.vdw, c.weights.elec, c.weights.desolv, c.weights.air};
@@ -382,7 +381,7 @@ void gdock_server_destroy(gdock_server* s) {
   if (s->mem) (void)hipFree(s->mem);
   if (s->h_poses) (void)hipHostFree(s->h_poses);
   if (s->h_out) (void)hipHostFree(s->h_out);
-  if (s->h_items) (void)hipHostFree(s->h_items);
+  if (s->h_list) (void)hipHostFree(s->h_list);
   if (s->h_table) (void)hipHostFree(s->h_table);
   if (s->ctx) gdock_ctx_destroy(s->ctx);
   delete s;

@@ -186,6 +186,13 @@ class Batch:
                                      C.byref(lds)), what="gdock_batch_plan")
         return {"lanes_k": k[:n].tolist(), "parts": parts[:n].tolist(), "grid": grid.value, "lds_bytes": lds.value}
 
+    def last_items(self) -> int:
+        """Work items of the last launch: one per pose, plus one per extra
+        part of each pose the launch split."""
+        n = C.c_int32()
+        check(lib().gdock_batch_last_items(self.handle, C.byref(n)), self.ctx.handle, "gdock_batch_last_items")
+        return n.value
+
     def score(self, genes: np.ndarray, weights: constants.EnergyWeights | None = None):
         g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
         assert len(g) == self.n_poses, (len(g), self.n_poses)

@@ -149,20 +149,24 @@ gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const d
  * The batch fixes the complexes and their pose counts; complex i owns pose
  * rows [off_i, off_i + poses_per_complex[i]) in complex order.  At creation
  * the library measures, per complex shape, how many ligand atoms each lane
- * holds (1-8) and orders the work longest first; every launch is one replay
- * of a captured HIP graph.  Complexes must outlive the batch.  Results equal
- * gdock_score_poses within the scoring tolerance. */
+ * holds (1-8); every launch is one replay of a captured HIP graph, which
+ * plans the launch's work from the poses themselves (an expensive pose is
+ * split by receptor range, the work runs most expensive first).  A pose's
+ * result depends only on the pose and its complex.  Complexes must outlive
+ * the batch.  Results equal gdock_score_poses within the scoring tolerance. */
 typedef struct gdock_batch gdock_batch;
 gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes, int32_t n,
                                 const int32_t* poses_per_complex, gdock_batch** out);
 void gdock_batch_destroy(gdock_ctx* ctx, gdock_batch* b);
 int32_t gdock_batch_n_poses(const gdock_batch* b);
-/* Chosen ligand atoms per lane (n entries), workgroups per pose (n entries:
- * a pose of a large complex is split by receptor range over up to 8
- * workgroups; a function of the complex's shape), persistent grid and LDS
- * bytes.  Any output may be null. */
+/* Chosen ligand atoms per lane (n entries), the most workgroups one pose may
+ * be split over (n entries, at most 8; a function of the complex's shape),
+ * persistent grid and LDS bytes.  Any output may be null. */
 gdock_status gdock_batch_plan(const gdock_batch* b, int32_t* lanes_k, int32_t* parts, int32_t* grid,
                               int32_t* lds_bytes);
+/* Work items of the last launch (one per pose plus one per extra part of
+ * each split pose); waits for the batch's work on its context stream. */
+gdock_status gdock_batch_last_items(const gdock_batch* b, int32_t* n_items);
 /* Synchronous: genes (n_poses x 6) in, out (n_poses x 6) and optional
  * n_satisfied out, as gdock_score_poses. */
 gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* genes, const gdock_weights* w,
@@ -283,7 +287,8 @@ gdock_status gdock_calc_metrics(gdock_ctx* ctx, gdock_evaluator* ev, const doubl
  * share its launches.  The server owns its own context on `device`. */
 typedef struct gdock_server_config {
   int32_t max_jobs;            /* active job slots (64) */
-  int32_t max_population;      /* largest population a job may use (150; at most 65536) */
+  int32_t max_population;      /* largest population a job may use (150; at most 65536; max_jobs x
+                                  max_population x 8 at most 2^30) */
   int32_t max_receptor_atoms;  /* sizes the kernel's LDS receptor table (8192) */
   int32_t reserved;
   int64_t memory_budget;       /* device bytes for resident jobs; 0: half the free memory */

@@ -99,10 +99,11 @@ def _lattice(mol):
 
 
 def test_split_poses_match_oracle():
-    """Poses of large complexes are scored by several workgroups over receptor
-    ranges; the parts' sums, the shared ligand burial counts and the overflow
-    rescoring of the whole pose agree with the oracle and the one-workgroup
-    path, and repeated replays (which reuse the cleared split state) agree."""
+    """Expensive poses of large complexes are scored by several workgroups
+    over receptor ranges (each launch estimates every pose's cost); the parts'
+    sums, the shared ligand burial counts and the overflow rescoring of the
+    whole pose agree with the oracle and the one-workgroup path, and repeated
+    replays (which reuse the cleared split state) agree."""
     specs = [(51, 6000, 3000, False), (52, 2600, 3300, False), (53, 5200, 2400, True)]
     cases = []
     for seed, nr, nl, lattice in specs:
@@ -123,6 +124,9 @@ def test_split_poses_match_oracle():
         genes.append(g)
     genes = np.concatenate(genes)
     out1, ns1 = b.score(genes)
+    items = b.last_items()
+    print("work items:", items, "poses:", len(genes))
+    assert items > len(genes)  # the near poses (identity genes) were split, the far ones need not be
     out2, ns2 = b.score(genes)
     assert np.array_equal(out1, out2) and np.array_equal(ns1, ns2)
     for i, (r, l, p) in enumerate(cases):
@@ -156,6 +160,7 @@ def test_split_pose_overflow_rescored():
     q = np.pi / 2
     genes = np.array([[0, 0, 0, 0, 0, 0], [q, 0, 0, 0, 0, 0], [0, q, 0, 0, 0, 0], [0, 0, q, 0, 0, 1.0]])
     out, ns = b.score(genes)
+    assert b.last_items() > len(genes)  # every pose here sits in the receptor: split
     ref, rns = orc.score_poses(rec.atoms, lig.atoms, [], genes, W, nthreads=8)
     assert_energies_close(out, ref, "split lattice storm")
     single, _ = cx.score_poses(genes)

@@ -50,8 +50,9 @@ def test_alone_equals_in_mix():
 
 
 def test_split_job_alone_equals_in_mix():
-    """A job large enough for split poses (5000 x 2000 atoms: each pose over
-    two workgroups) is bit-identical alone and among small jobs, and takes
+    """A job large enough for split poses (5000 x 2000 atoms: a pose near the
+    receptor over several workgroups) is bit-identical alone and among small
+    jobs (a pose's split depends only on the pose and its complex), and takes
     the same decisions as the plain one-launch-per-generation path (whose
     poses are unsplit: sums in another order, so near-ties may differ)."""
     from gdock_amd import ga
