@@ -459,12 +459,21 @@ __global__ __launch_bounds__(256) void overlap_kernel(const uint32_t* __restrict
 
 // ---------------------------------------------------------------- K8 greedy
 // clustering.rs:183-254 in one workgroup.  count[a] = unused neighbours of a
-// (self included).  Each step takes the unused element with the largest
-// count (lowest index on ties), forms {centre} + its unused neighbours (in
-// ascending order), marks them used and decrements the counts of every
-// element that has a new member as neighbour (rows of AT).  Cluster sizes are
-// non-increasing along the sequence, so the loop stops as soon as no element
-// can reach min_size (count + 1 < min_size).
+// (self included).  Each step of the reference takes the unused element with
+// the largest count (lowest index on ties), forms {centre} + its unused
+// neighbours (in ascending order), marks them used and decrements the counts
+// of every element that has a new member as neighbour (rows of AT).
+//
+// Counts only decrease, so the steps are taken in rounds: a round finds the
+// largest count m among unused elements and lists the unused elements of
+// count m in ascending order; walking that list, each element still unused
+// and still of count m is the reference's next centre (any unused element of
+// count m now was of count m at the start of the round, and every lower one
+// of the list was taken or fell below m), the others are passed over for a
+// later round.  This gives the reference's sequence of clusters with one
+// block-wide reduction per round instead of one per cluster.  Cluster sizes
+// are non-increasing along the sequence, so the loop stops as soon as no
+// element can reach min_size (count + 1 < min_size).
 // Neighbour count of every row of A (self included), one wave per row.
 __global__ __launch_bounds__(256) void row_count_kernel(const unsigned long long* __restrict__ A, int n, int nwords,
                                                         int32_t* __restrict__ counts) {
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(256) void row_count_kernel(const unsigned long long
 }
 
 constexpr int kGreedyThreads = 1024;
-constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count, 4 B member slot and 1 bit used flag per element
+constexpr int kGreedyMaxN = 15000;  // LDS: 4 B count, 2 B member slot, 2 B candidate slot, 1 bit used flag per element
 
 __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned long long* __restrict__ A,
                                                                 const unsigned long long* __restrict__ AT,
@@ -489,55 +498,24 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
                                                                 int64_t* __restrict__ out_offset,
                                                                 int64_t* __restrict__ out_members,
                                                                 int32_t* __restrict__ out_n) {
-  extern __shared__ unsigned long long s_used[];  // nwords, then count[n], then the current members[n]
+  // LDS: used bits (nwords), count[n], then the current members[n] and the
+  // round's candidates[n] as 16-bit indices
+  extern __shared__ unsigned long long s_used[];
   int32_t* s_count = reinterpret_cast<int32_t*>(s_used + nwords);
-  int32_t* s_members = s_count + n;
-  __shared__ unsigned long long s_key[kGreedyThreads / 64];
+  uint16_t* s_members = reinterpret_cast<uint16_t*>(s_count + n);
+  uint16_t* s_cand = s_members + n;
+  __shared__ int32_t s_red[kGreedyThreads / 64];
   __shared__ int32_t s_scan[kGreedyThreads / 64];
-  __shared__ unsigned long long s_best;
+  __shared__ int32_t s_ncand;
   __shared__ int64_t s_total;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int a = tid; a < n; a += kGreedyThreads) s_count[a] = counts[a];
   for (int w = tid; w < nwords; w += kGreedyThreads) s_used[w] = 0ull;
   __syncthreads();
-  int32_t nclu = 0;
-  int64_t written = 0;
-  for (;;) {
-    // argmax over unused of (count, -index)
-    unsigned long long key = 0ull;
-    for (int a = tid; a < n; a += kGreedyThreads) {
-      if ((s_used[a >> 6] >> (a & 63)) & 1ull) continue;
-      const unsigned long long k = (static_cast<unsigned long long>(static_cast<uint32_t>(s_count[a]) + 1u) << 32) |
-                                   static_cast<unsigned long long>(0xFFFFFFFFu - static_cast<uint32_t>(a));
-      key = k > key ? k : key;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(key, off);
-      key = o > key ? o : key;
-    }
-    if (lane == 0) s_key[wave] = key;
-    __syncthreads();
-    if (tid == 0) {
-      unsigned long long b = 0ull;
-      for (int i = 0; i < kGreedyThreads / 64; ++i) b = s_key[i] > b ? s_key[i] : b;
-      s_best = b;
-    }
-    __syncthreads();
-    const unsigned long long best = s_best;
-    if (best == 0ull) break;  // every element used
-    const long long best_count = static_cast<long long>(best >> 32) - 1;
-    if (min_size > 1 && best_count + 1 < min_size) break;
-    const int center = static_cast<int>(0xFFFFFFFFu - static_cast<uint32_t>(best & 0xFFFFFFFFull));
-    // members = centre + its unused neighbours, ascending (one word per thread)
-    const unsigned long long* row = A + static_cast<size_t>(center) * nwords;
-    unsigned long long mine = 0ull;
-    if (tid < nwords) {
-      mine = row[tid] & ~s_used[tid];
-      if (tid == (center >> 6)) mine |= 1ull << (center & 63);
-    }
-    const int local = __popcll(mine);
-    int incl = local;
+  auto used = [&](int a) { return ((s_used[a >> 6] >> (a & 63)) & 1ull) != 0ull; };
+  // block-wide exclusive prefix of v (all threads); *total gets the sum
+  auto prefix = [&](int v, int* total) {
+    int incl = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int o = __shfl_up(incl, off);
@@ -545,33 +523,76 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
     }
     if (lane == 63) s_scan[wave] = incl;
     __syncthreads();
-    int wbase = 0;
-    for (int i = 0; i < wave; ++i) wbase += s_scan[i];
-    if (tid == kGreedyThreads - 1) s_total = wbase + incl;
-    int pos = wbase + incl - local;
-    for (unsigned long long m = mine; m; m &= m - 1) {
-      const int e = tid * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-      s_members[pos] = e;
-      out_members[written + pos++] = e;
+    int base = 0, sum = 0;
+    for (int i = 0; i < kGreedyThreads / 64; ++i) {
+      const int x = s_scan[i];
+      base += i < wave ? x : 0;
+      sum += x;
     }
-    if (tid < nwords) s_used[tid] |= mine;
+    *total = sum;
+    return base + incl - v;
+  };
+  const int per = (n + kGreedyThreads - 1) / kGreedyThreads;  // thread t owns [t per, t per + per)
+  const int lo = min(tid * per, n), hi = min(lo + per, n);
+  int32_t nclu = 0;
+  int64_t written = 0;
+  for (;;) {
+    // m = the largest count among unused elements (-1: every element used)
+    int mx = -1;
+    for (int a = tid; a < n; a += kGreedyThreads)
+      if (!used(a)) mx = max(mx, s_count[a]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
+    if (lane == 0) s_red[wave] = mx;
     __syncthreads();
-    const int64_t size = s_total;
-    // decrement the counts of elements that have a new member as neighbour
-    for (int64_t q = tid; q < size * nwords; q += kGreedyThreads) {
-      const int64_t u = s_members[q / nwords];
-      const int w = static_cast<int>(q % nwords);
-      for (unsigned long long m = AT[static_cast<size_t>(u) * nwords + w]; m; m &= m - 1)
-        atomicSub(&s_count[w * 64 + (__ffsll(static_cast<long long>(m)) - 1)], 1);
-    }
-    if (tid == 0) {
-      out_center[nclu] = center;
-      out_size[nclu] = size;
-      out_offset[nclu] = written;
-    }
-    ++nclu;
-    written += size;
+    int m = -1;
+    for (int i = 0; i < kGreedyThreads / 64; ++i) m = max(m, s_red[i]);
+    if (m < 0) break;  // every element used
+    if (min_size > 1 && static_cast<long long>(m) + 1 < min_size) break;
+    // the round's candidates: unused elements of count m, ascending
+    int mine_n = 0;
+    for (int a = lo; a < hi; ++a) mine_n += (!used(a) && s_count[a] == m) ? 1 : 0;
+    int ncand = 0;
+    int pos = prefix(mine_n, &ncand);
+    for (int a = lo; a < hi; ++a)
+      if (!used(a) && s_count[a] == m) s_cand[pos++] = static_cast<uint16_t>(a);
     __syncthreads();
+    for (int t = 0; t < ncand; ++t) {
+      const int center = s_cand[t];
+      if (used(center) || s_count[center] != m) continue;  // taken, or below m now: a later round
+      // members = centre + its unused neighbours, ascending (one word per thread)
+      unsigned long long mine = 0ull;
+      if (tid < nwords) {
+        mine = A[static_cast<size_t>(center) * nwords + tid] & ~s_used[tid];
+        if (tid == (center >> 6)) mine |= 1ull << (center & 63);
+      }
+      const int local = __popcll(mine);
+      int size32 = 0;
+      int mpos = prefix(local, &size32);
+      for (unsigned long long mm = mine; mm; mm &= mm - 1) {
+        const int e = tid * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
+        s_members[mpos] = static_cast<uint16_t>(e);
+        out_members[written + mpos++] = e;
+      }
+      if (tid < nwords) s_used[tid] |= mine;
+      __syncthreads();
+      const int64_t size = size32;
+      // decrement the counts of elements that have a new member as neighbour
+      for (int64_t q = tid; q < size * nwords; q += kGreedyThreads) {
+        const int64_t u = s_members[q / nwords];
+        const int w = static_cast<int>(q % nwords);
+        for (unsigned long long mm = AT[static_cast<size_t>(u) * nwords + w]; mm; mm &= mm - 1)
+          atomicSub(&s_count[w * 64 + (__ffsll(static_cast<long long>(mm)) - 1)], 1);
+      }
+      if (tid == 0) {
+        out_center[nclu] = center;
+        out_size[nclu] = size;
+        out_offset[nclu] = written;
+      }
+      ++nclu;
+      written += size;
+      __syncthreads();
+    }
   }
   if (tid == 0) *out_n = nclu;
 }
