@@ -506,7 +506,6 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
   uint16_t* s_cand = s_members + n;
   __shared__ int32_t s_red[kGreedyThreads / 64];
   __shared__ int32_t s_scan[kGreedyThreads / 64];
-  __shared__ int64_t s_total;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int a = tid; a < n; a += kGreedyThreads) s_count[a] = counts[a];
   for (int w = tid; w < nwords; w += kGreedyThreads) s_used[w] = 0ull;
