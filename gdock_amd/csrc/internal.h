@@ -104,7 +104,8 @@ void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo,
 
 // Ligand atoms per lane for a complex, measured once per complex shape.
 int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err);
-// Workgroups per pose for a complex's shape (1 = unsplit, at most kMaxPoseParts).
+// Most workgroups one pose of a complex may be split over in batched launches
+// (1 = never split, at most kMaxPoseParts); each launch picks every pose's own.
 int pose_parts(const DevComplex& d, int lanes_k);
 // Device bytes of the split state of `poses` poses, and its binding into a
 // table entry (pose rows [pose_base, pose_base + poses), state cleared).
