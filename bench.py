@@ -363,6 +363,8 @@ def bench_serve(args):
         "generations": gens, "peak_active_jobs": st["peak_active"], "steps_run": st["steps"],
         "server_busy_s": st["busy_s"], "mean_step_ms": st["busy_s"] / max(st["steps"], 1) * 1e3,
         "mean_step_kernel_ms": st["kernel_s"] / max(st["steps"], 1) * 1e3,
+        "server_admit_s": st["admit_s"], "server_ga_s": st["ga_s"], "server_finish_s": st["finish_s"],
+        "span_s": span,
         "plain": {"jobs_per_min": len(jobs) / pspan * 60.0, "latency_p50_s": float(np.percentile(plat, 50)),
                   "latency_p99_s": float(np.percentile(plat, 99)), "mean_service_s": float(np.mean(service)),
                   "generations": pgens, "pose_evals_per_s": pgens * 150 / float(np.sum(service))},

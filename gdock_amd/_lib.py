@@ -70,7 +70,8 @@ class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("poses_scored", C.c_int64), ("jobs_finished", C.c_int64),
                 ("jobs_queued", C.c_int64), ("peak_active", C.c_int32), ("reserved", C.c_int32),
                 ("busy_s", C.c_double), ("bytes_in_use", C.c_int64), ("memory_budget", C.c_int64),
-                ("kernel_s", C.c_double)]
+                ("kernel_s", C.c_double), ("admit_s", C.c_double), ("ga_s", C.c_double),
+                ("finish_s", C.c_double)]
 
 
 class GdockError(RuntimeError):

@@ -253,6 +253,8 @@ void host_pool_for(int64_t n, F&& f) {
 
 }  // namespace
 
+void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& f) { host_pool_for(n, f); }
+
 }  // namespace gdock
 
 extern "C" {

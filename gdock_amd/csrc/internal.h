@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <string_view>
 #include <vector>
@@ -87,6 +88,11 @@ hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stre
 // must start zeroed).
 size_t order_bytes(size_t capacity);
 void bind_order(BatchLaunch* b, char* mem, size_t capacity);
+
+// Fork-join of f(lo, hi) over slices of [0, n) on the library's host threads
+// (the caller takes one slice; a call made while another runs, or in a child
+// process forked after the threads started, runs alone).
+void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& f);
 
 // Host-side bump layout of one device allocation (256-byte aligned pieces).
 struct Layout {

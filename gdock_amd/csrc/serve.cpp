@@ -87,7 +87,7 @@ struct gdock_server {
   // statistics
   int64_t n_steps = 0, n_poses = 0, n_finished = 0;
   int32_t peak_active = 0;
-  double busy_s = 0, kernel_s = 0;
+  double busy_s = 0, kernel_s = 0, admit_s = 0, ga_s = 0, finish_s = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   void run();
@@ -128,6 +128,7 @@ bool gdock_server::step() {
     admit_locked(&fresh);
   }
   // Materialise newly admitted jobs on the device (outside the lock).
+  const auto t_admit = std::chrono::steady_clock::now();
   for (Job* j : fresh) {
     j->t_admit = now_s(t0);
     gdock_status st = GDOCK_OK;
@@ -186,6 +187,7 @@ bool gdock_server::step() {
       done_cv.notify_all();
     }
   }
+  admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_admit).count();
   {
     std::lock_guard<std::mutex> g(mu);
     for (Job* s : slots)
@@ -196,17 +198,24 @@ bool gdock_server::step() {
   const auto t_step = std::chrono::steady_clock::now();
 
   // The poses of the step (rows slot * max_pop + p); the launch plans its
-  // work items from them.
-  int n_list = 0, max_row = 0, step_poses = 0;
-  for (Job* j : active) {
-    const GaJob& ga = j->result->job;
-    const int pop = ga.population();
-    const int row0 = j->slot * max_pop;
-    gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
-    for (int p = 0; p < pop; ++p) h_list[n_list++] = make_int2(j->slot, row0 + p);
-    max_row = std::max(max_row, row0 + pop);
-    step_poses += pop;
+  // work items from them.  Jobs are prepared on the host threads.
+  std::vector<int> list_at(active.size() + 1, 0);
+  int max_row = 0;
+  for (size_t i = 0; i < active.size(); ++i) {
+    const int pop = active[i]->result->job.population();
+    list_at[i + 1] = list_at[i] + pop;
+    max_row = std::max(max_row, active[i]->slot * max_pop + pop);
   }
+  const int n_list = list_at.back(), step_poses = n_list;
+  host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+      const Job* j = active[i];
+      const GaJob& ga = j->result->job;
+      const int pop = ga.population(), row0 = j->slot * max_pop;
+      gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
+      for (int p = 0; p < pop; ++p) h_list[list_at[i] + p] = make_int2(j->slot, row0 + p);
+    }
+  });
   int32_t head = n_list;
   hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
                                 static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
@@ -230,6 +239,16 @@ bool gdock_server::step() {
   ++n_steps;
   n_poses += step_poses;
 
+  // Each job's GA takes its generation's results (independent jobs, on the
+  // host threads); finished jobs are then clustered one after another.
+  if (e == hipSuccess) {
+    const auto t_ga = std::chrono::steady_clock::now();
+    host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i)
+        active[i]->result->job.absorb(h_out + static_cast<size_t>(active[i]->slot) * max_pop * kOutDoubles);
+    });
+    ga_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_ga).count();
+  }
   std::vector<Job*> finished;
   for (Job* j : active) {
     if (e != hipSuccess) {
@@ -240,11 +259,12 @@ bool gdock_server::step() {
       continue;
     }
     GaJob& ga = j->result->job;
-    ga.absorb(h_out + static_cast<size_t>(j->slot) * max_pop * kOutDoubles);
     if (!ga.done()) continue;
     if (j->params.cluster) {
+      const auto t_fcc = std::chrono::steady_clock::now();
       const gdock_status st = finish_job(ctx, j->cx, &ga, &j->result->clusters);
       if (st != GDOCK_OK) { j->state = kFailed; j->error = st; j->message = ctx->err; }
+      finish_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_fcc).count();
     }
     finished.push_back(j);
   }
@@ -450,6 +470,9 @@ gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out) {
   out->peak_active = s->peak_active;
   out->busy_s = s->busy_s;
   out->kernel_s = s->kernel_s;
+  out->admit_s = s->admit_s;
+  out->ga_s = s->ga_s;
+  out->finish_s = s->finish_s;
   out->bytes_in_use = static_cast<int64_t>(s->bytes_in_use);
   out->memory_budget = s->cfg.memory_budget;
   return GDOCK_OK;
