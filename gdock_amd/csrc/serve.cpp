@@ -84,6 +84,15 @@ struct gdock_server {
   int64_t next_id = 1;
   bool stop = false;
   std::thread worker;
+  // Jobs whose GA has ended are finished (FCC of the Hall of Fame, model
+  // selection, release of their complex) on a thread and stream of their own,
+  // while the worker goes on scoring the other jobs.
+  gdock_ctx* fctx = nullptr;
+  std::deque<Job*> to_finish;
+  int n_finishing = 0;             // handed to the finisher and not yet released
+  std::condition_variable fin_cv;  // finisher: a job to finish or stop
+  bool fin_stop = false;
+  std::thread finisher;
   // statistics
   int64_t n_steps = 0, n_poses = 0, n_finished = 0;
   int32_t peak_active = 0;
@@ -93,6 +102,10 @@ struct gdock_server {
   void run();
   void admit_locked(std::vector<Job*>* fresh);
   bool step();
+  void finish_loop();
+  // Frees the job's device state, then marks it ended (failed with `st` unless
+  // GDOCK_OK) and wakes its waiters; called holding no lock.
+  void release(Job* j, gdock_status st = GDOCK_OK, const std::string& message = std::string());
 };
 
 // Moves queued jobs into free slots while their device memory fits the
@@ -106,7 +119,8 @@ void gdock_server::admit_locked(std::vector<Job*>* fresh) {
     // (256 B per atom and 1 MiB for contact tables and launch state); once
     // the complex exists the larger of the estimate and its arena is charged.
     const size_t est = 256 * (j->rec.size() + j->lig.size()) + (size_t{1} << 20);
-    const bool nothing_active = std::all_of(slots.begin(), slots.end(), [](Job* s) { return s == nullptr; });
+    const bool nothing_active =
+        n_finishing == 0 && std::all_of(slots.begin(), slots.end(), [](Job* s) { return s == nullptr; });
     if (bytes_in_use + est > static_cast<size_t>(cfg.memory_budget) && !nothing_active) return;
     queue.pop_front();
     j->slot = static_cast<int>(free_slot - slots.begin());
@@ -194,7 +208,11 @@ bool gdock_server::step() {
       if (s && s->state == kActive) active.push_back(s);
     peak_active = std::max<int32_t>(peak_active, static_cast<int32_t>(active.size()));
   }
-  if (active.empty()) return true;
+  if (active.empty()) {  // queued jobs wait for memory held by finishing ones
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait_for(lk, std::chrono::milliseconds(2));
+    return true;
+  }
   const auto t_step = std::chrono::steady_clock::now();
 
   // The poses of the step (rows slot * max_pop + p); the launch plans its
@@ -249,38 +267,75 @@ bool gdock_server::step() {
     });
     ga_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_ga).count();
   }
-  std::vector<Job*> finished;
+  std::vector<Job*> failed, ended;
+  const gdock_status launch_st = e != hipSuccess ? hip_fail(ctx, e, "server: generation launch") : GDOCK_OK;
   for (Job* j : active) {
     if (e != hipSuccess) {
-      j->state = kFailed;
-      j->error = hip_fail(ctx, e, "server: generation launch");
-      j->message = ctx->err;
-      finished.push_back(j);
-      continue;
+      failed.push_back(j);
+    } else if (j->result->job.done()) {
+      ended.push_back(j);
     }
-    GaJob& ga = j->result->job;
-    if (!ga.done()) continue;
-    if (j->params.cluster) {
-      const auto t_fcc = std::chrono::steady_clock::now();
-      const gdock_status st = finish_job(ctx, j->cx, &ga, &j->result->clusters);
-      if (st != GDOCK_OK) { j->state = kFailed; j->error = st; j->message = ctx->err; }
-      finish_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_fcc).count();
-    }
-    finished.push_back(j);
   }
-  for (Job* j : finished) {
-    gdock_complex_destroy(ctx, j->cx);
-    j->cx = nullptr;
-    if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
+  {
+    // Ended jobs leave their slots (their rows are no longer scored) and go
+    // to the finisher, which holds their memory until it has finished them.
     std::lock_guard<std::mutex> g(mu);
-    if (j->state == kActive) j->state = kFinished;
+    for (Job* j : failed) slots[j->slot] = nullptr;
+    for (Job* j : ended) {
+      slots[j->slot] = nullptr;
+      to_finish.push_back(j);
+      ++n_finishing;
+    }
+  }
+  if (!ended.empty()) fin_cv.notify_one();
+  for (Job* j : failed) release(j, launch_st, ctx->err);
+  return true;
+}
+
+void gdock_server::release(Job* j, gdock_status st, const std::string& message) {
+  if (j->cx) gdock_complex_destroy(fctx ? fctx : ctx, j->cx);
+  j->cx = nullptr;
+  if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
+  {
+    std::lock_guard<std::mutex> g(mu);
+    if (st != GDOCK_OK) {
+      j->state = kFailed;
+      j->error = st;
+      j->message = message;
+    } else if (j->state == kActive) {
+      j->state = kFinished;
+    }
     j->t_finish = now_s(t0);
     bytes_in_use -= j->bytes;
-    slots[j->slot] = nullptr;
     ++n_finished;
   }
-  if (!finished.empty()) done_cv.notify_all();
-  return true;
+  done_cv.notify_all();
+  cv.notify_all();  // memory freed: the worker may admit queued jobs
+}
+
+void gdock_server::finish_loop() {
+  (void)hipSetDevice(fctx->device);
+  for (;;) {
+    Job* j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      fin_cv.wait(lk, [&] { return fin_stop || !to_finish.empty(); });
+      if (to_finish.empty()) return;  // stopping with nothing left
+      j = to_finish.front();
+      to_finish.pop_front();
+    }
+    gdock_status st = GDOCK_OK;
+    if (j->params.cluster) {
+      const auto t_fcc = std::chrono::steady_clock::now();
+      st = finish_job(fctx, j->cx, &j->result->job, &j->result->clusters);
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_fcc).count();
+      std::lock_guard<std::mutex> g(mu);
+      finish_s += dt;
+    }
+    release(j, st, st != GDOCK_OK ? std::string(fctx->err) : std::string());
+    std::lock_guard<std::mutex> g(mu);
+    --n_finishing;
+  }
 }
 
 void gdock_server::run() {
@@ -376,7 +431,12 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     gdock_server_destroy(s);
     return GDOCK_ERR_DEVICE;
   }
+  if (gdock_ctx_create(device, &s->fctx) != GDOCK_OK) {
+    gdock_server_destroy(s);
+    return GDOCK_ERR_DEVICE;
+  }
   s->worker = std::thread([s] { s->run(); });
+  s->finisher = std::thread([s] { s->finish_loop(); });
   *out = s;
   return GDOCK_OK;
 }
@@ -389,6 +449,12 @@ void gdock_server_destroy(gdock_server* s) {
   }
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {  // the finisher completes the jobs already handed to it, then stops
+    std::lock_guard<std::mutex> g(s->mu);
+    s->fin_stop = true;
+  }
+  s->fin_cv.notify_all();
+  if (s->finisher.joinable()) s->finisher.join();
   for (auto& kv : s->jobs) {
     if (kv.second->cx) gdock_complex_destroy(s->ctx, kv.second->cx);
     if (kv.second->split_mem) (void)hipFree(kv.second->split_mem);
@@ -404,6 +470,7 @@ void gdock_server_destroy(gdock_server* s) {
   if (s->h_list) (void)hipHostFree(s->h_list);
   if (s->h_table) (void)hipHostFree(s->h_table);
   if (s->ctx) gdock_ctx_destroy(s->ctx);
+  if (s->fctx) gdock_ctx_destroy(s->fctx);
   delete s;
 }
 
