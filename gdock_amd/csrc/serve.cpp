@@ -51,6 +51,9 @@ struct Job {
   int parts = 1;
   void* split_mem = nullptr;  // split-pose state (parts > 1)
   std::unique_ptr<gdock_ga_result> result;
+  DevComplex dev{};      // the job's launch-table entry, once prepared
+  bool ready = false;    // prepared: the worker scores it from its next step
+  bool in_table = false; // the worker has uploaded its table entry
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -84,6 +87,13 @@ struct gdock_server {
   int64_t next_id = 1;
   bool stop = false;
   std::thread worker;
+  // Admitted jobs are prepared (ligand positioning, complex tables, lanes,
+  // split state) on a thread and stream of their own while scoring goes on.
+  gdock_ctx* pctx = nullptr;
+  std::deque<Job*> to_prepare;
+  std::condition_variable prep_cv;  // preparer: a job to prepare or stop
+  bool prep_stop = false;
+  std::thread preparer;
   // Jobs whose GA has ended are finished (FCC of the Hall of Fame, model
   // selection, release of their complex) on a thread and stream of their own,
   // while the worker goes on scoring the other jobs.
@@ -103,6 +113,7 @@ struct gdock_server {
   void admit_locked(std::vector<Job*>* fresh);
   bool step();
   void finish_loop();
+  void prepare_loop();
   // Frees the job's device state, then marks it ended (failed with `st` unless
   // GDOCK_OK) and wakes its waiters; called holding no lock.
   void release(Job* j, gdock_status st = GDOCK_OK, const std::string& message = std::string());
@@ -126,6 +137,7 @@ void gdock_server::admit_locked(std::vector<Job*>* fresh) {
     j->slot = static_cast<int>(free_slot - slots.begin());
     *free_slot = j;
     j->state = kActive;
+    j->t_admit = now_s(t0);
     j->bytes = est;
     bytes_in_use += est;
     fresh->push_back(j);
@@ -141,73 +153,29 @@ bool gdock_server::step() {
     if (stop) return false;
     admit_locked(&fresh);
   }
-  // Materialise newly admitted jobs on the device (outside the lock).
-  const auto t_admit = std::chrono::steady_clock::now();
-  for (Job* j : fresh) {
-    j->t_admit = now_s(t0);
-    gdock_status st = GDOCK_OK;
-    std::vector<gdock_atom> lig = j->lig;
-    if (!lig.empty() && !j->rec.empty())
-      st = gdock_position_ligand(j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
-                                 static_cast<int32_t>(lig.size()));
-    if (st == GDOCK_OK)
-      st = gdock_complex_create(ctx, j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
-                                static_cast<int32_t>(lig.size()), j->pairs.data(),
-                                static_cast<int32_t>(j->pairs.size() / 2), &j->cx);
-    if (st == GDOCK_OK && j->cx->dev.n_rec > cfg.max_receptor_atoms + 64) {
-      set_error(ctx, "receptor of %d slots exceeds the server's %d", j->cx->dev.n_rec, cfg.max_receptor_atoms);
-      st = GDOCK_ERR_INVALID_ARG;
+  if (!fresh.empty()) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      to_prepare.insert(to_prepare.end(), fresh.begin(), fresh.end());
     }
-    hipError_t he = hipSuccess;
-    if (st == GDOCK_OK) {
-      j->lanes = tune_lanes(ctx, j->cx, &he);
-      if (he != hipSuccess) st = hip_fail(ctx, he, "server: autotune");
-    }
-    size_t split_bytes = 0;
-    if (st == GDOCK_OK) {
-      j->parts = pose_parts(j->cx->dev, j->lanes);
-      split_bytes = split_state_bytes(j->cx->dev, j->parts, max_pop);
-      if (split_bytes) {
-        he = hipMalloc(&j->split_mem, split_bytes);
-        if (he == hipSuccess) he = hipMemsetAsync(j->split_mem, 0, split_bytes, ctx->stream);
-        if (he != hipSuccess) st = hip_fail(ctx, he, "server: split state");
-      }
-    }
-    if (st == GDOCK_OK) {
-      j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
-      DevComplex d = j->cx->dev;
-      d.lanes_k = j->lanes;
-      bind_split_state(&d, j->parts, max_pop, j->slot * max_pop, static_cast<char*>(j->split_mem));
-      h_table[j->slot] = d;
-      he = hipMemcpyAsync(const_cast<DevComplex*>(L.table) + j->slot, &h_table[j->slot], sizeof(DevComplex),
-                          hipMemcpyHostToDevice, ctx->stream);
-      if (he != hipSuccess) st = hip_fail(ctx, he, "server: table upload");
-    }
-    std::lock_guard<std::mutex> g(mu);
-    if (st == GDOCK_OK) {  // charge the larger of the estimate and the arena
-      const size_t charged = std::max(j->bytes, j->cx->mem_bytes + split_bytes);
-      bytes_in_use += charged - j->bytes;
-      j->bytes = charged;
-    } else {
-      j->state = kFailed;
-      j->error = st;
-      j->message = ctx->err;
-      j->t_finish = now_s(t0);
-      if (j->cx) { gdock_complex_destroy(ctx, j->cx); j->cx = nullptr; }
-      if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
-      bytes_in_use -= j->bytes;
-      slots[j->slot] = nullptr;
-      ++n_finished;
-      done_cv.notify_all();
-    }
+    prep_cv.notify_one();
   }
-  admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_admit).count();
+  // The prepared jobs take part from this step: their table entries are
+  // uploaded on the worker's stream, ahead of the step's launch.
   {
     std::lock_guard<std::mutex> g(mu);
     for (Job* s : slots)
-      if (s && s->state == kActive) active.push_back(s);
+      if (s && s->state == kActive && s->ready) active.push_back(s);
     peak_active = std::max<int32_t>(peak_active, static_cast<int32_t>(active.size()));
   }
+  hipError_t te = hipSuccess;
+  for (Job* j : active)
+    if (!j->in_table && te == hipSuccess) {
+      h_table[j->slot] = j->dev;
+      te = hipMemcpyAsync(const_cast<DevComplex*>(L.table) + j->slot, &h_table[j->slot], sizeof(DevComplex),
+                          hipMemcpyHostToDevice, ctx->stream);
+      j->in_table = true;
+    }
   if (active.empty()) {  // queued jobs wait for memory held by finishing ones
     std::unique_lock<std::mutex> lk(mu);
     cv.wait_for(lk, std::chrono::milliseconds(2));
@@ -235,7 +203,9 @@ bool gdock_server::step() {
     }
   });
   int32_t head = n_list;
-  hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
+  hipError_t e = te;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
                                 static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream);
   if (e == hipSuccess)
@@ -311,6 +281,77 @@ void gdock_server::release(Job* j, gdock_status st, const std::string& message) 
   }
   done_cv.notify_all();
   cv.notify_all();  // memory freed: the worker may admit queued jobs
+}
+
+void gdock_server::prepare_loop() {
+  (void)hipSetDevice(pctx->device);
+  for (;;) {
+    Job* j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      prep_cv.wait(lk, [&] { return prep_stop || !to_prepare.empty(); });
+      if (prep_stop) return;
+      j = to_prepare.front();
+      to_prepare.pop_front();
+    }
+    const auto t_prep = std::chrono::steady_clock::now();
+    gdock_status st = GDOCK_OK;
+    std::vector<gdock_atom> lig = j->lig;
+    if (!lig.empty() && !j->rec.empty())
+      st = gdock_position_ligand(j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
+                                 static_cast<int32_t>(lig.size()));
+    if (st == GDOCK_OK)
+      st = gdock_complex_create(pctx, j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
+                                static_cast<int32_t>(lig.size()), j->pairs.data(),
+                                static_cast<int32_t>(j->pairs.size() / 2), &j->cx);
+    if (st == GDOCK_OK && j->cx->dev.n_rec > cfg.max_receptor_atoms + 64) {
+      set_error(pctx, "receptor of %d slots exceeds the server's %d", j->cx->dev.n_rec, cfg.max_receptor_atoms);
+      st = GDOCK_ERR_INVALID_ARG;
+    }
+    hipError_t he = hipSuccess;
+    if (st == GDOCK_OK) {
+      j->lanes = tune_lanes(pctx, j->cx, &he);
+      if (he != hipSuccess) st = hip_fail(pctx, he, "server: autotune");
+    }
+    size_t split_bytes = 0;
+    if (st == GDOCK_OK) {
+      j->parts = pose_parts(j->cx->dev, j->lanes);
+      split_bytes = split_state_bytes(j->cx->dev, j->parts, max_pop);
+      if (split_bytes) {
+        he = hipMalloc(&j->split_mem, split_bytes);
+        if (he == hipSuccess) he = hipMemsetAsync(j->split_mem, 0, split_bytes, pctx->stream);
+        if (he != hipSuccess) st = hip_fail(pctx, he, "server: split state");
+      }
+    }
+    if (st == GDOCK_OK) {  // everything on the preparer's stream is done before the worker uses it
+      he = hipStreamSynchronize(pctx->stream);
+      if (he != hipSuccess) st = hip_fail(pctx, he, "server: job preparation");
+    }
+    if (st == GDOCK_OK) {
+      j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
+      j->dev = j->cx->dev;
+      j->dev.lanes_k = j->lanes;
+      bind_split_state(&j->dev, j->parts, max_pop, j->slot * max_pop, static_cast<char*>(j->split_mem));
+    }
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_prep).count();
+    if (st == GDOCK_OK) {
+      std::lock_guard<std::mutex> g(mu);
+      const size_t charged = std::max(j->bytes, j->cx->mem_bytes + split_bytes);  // the larger of estimate and arena
+      bytes_in_use += charged - j->bytes;
+      j->bytes = charged;
+      j->ready = true;
+      admit_s += dt;
+    } else {
+      {
+        std::lock_guard<std::mutex> g(mu);
+        slots[j->slot] = nullptr;
+        admit_s += dt;
+      }
+      release(j, st, pctx->err);
+      continue;
+    }
+    cv.notify_all();  // the worker scores the job from its next step
+  }
 }
 
 void gdock_server::finish_loop() {
@@ -431,11 +472,12 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     gdock_server_destroy(s);
     return GDOCK_ERR_DEVICE;
   }
-  if (gdock_ctx_create(device, &s->fctx) != GDOCK_OK) {
+  if (gdock_ctx_create(device, &s->fctx) != GDOCK_OK || gdock_ctx_create(device, &s->pctx) != GDOCK_OK) {
     gdock_server_destroy(s);
     return GDOCK_ERR_DEVICE;
   }
   s->worker = std::thread([s] { s->run(); });
+  s->preparer = std::thread([s] { s->prepare_loop(); });
   s->finisher = std::thread([s] { s->finish_loop(); });
   *out = s;
   return GDOCK_OK;
@@ -449,6 +491,12 @@ void gdock_server_destroy(gdock_server* s) {
   }
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {  // jobs waiting for preparation are dropped with the rest
+    std::lock_guard<std::mutex> g(s->mu);
+    s->prep_stop = true;
+  }
+  s->prep_cv.notify_all();
+  if (s->preparer.joinable()) s->preparer.join();
   {  // the finisher completes the jobs already handed to it, then stops
     std::lock_guard<std::mutex> g(s->mu);
     s->fin_stop = true;
@@ -471,6 +519,7 @@ void gdock_server_destroy(gdock_server* s) {
   if (s->h_table) (void)hipHostFree(s->h_table);
   if (s->ctx) gdock_ctx_destroy(s->ctx);
   if (s->fctx) gdock_ctx_destroy(s->fctx);
+  if (s->pctx) gdock_ctx_destroy(s->pctx);
   delete s;
 }
 

@@ -304,9 +304,9 @@ typedef struct gdock_server_stats_t {
   double busy_s;               /* host time spent in scoring steps */
   int64_t bytes_in_use, memory_budget;
   double kernel_s;             /* device time of the scoring graph replays (HIP events) */
-  double admit_s;              /* host time materialising admitted jobs (positioning, complex, lanes) */
-  double ga_s;                 /* host time advancing the GAs after each step */
-  double finish_s;             /* time finishing jobs: FCC of the Hall of Fame, model selection */
+  double admit_s;              /* time preparing admitted jobs (positioning, complex, lanes), on a thread of its own */
+  double ga_s;                 /* host time advancing the GAs after each step (on the scoring thread) */
+  double finish_s;             /* time finishing jobs (FCC of the Hall of Fame, model selection), on a thread of its own */
 } gdock_server_stats_t;
 typedef struct gdock_server gdock_server;
 gdock_status gdock_server_default_config(gdock_server_config* c);
