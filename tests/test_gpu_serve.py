@@ -100,6 +100,8 @@ def test_memory_admission_limits_active_jobs():
             assert res.generations >= 1 and t.admit_s >= t.submit_s
         st = srv.stats()
         assert st["jobs_finished"] == 5 and st["peak_active"] <= 2 and st["bytes_in_use"] == 0
+        # the scoring, preparing and finishing threads each account their time
+        assert st["busy_s"] > 0 and st["admit_s"] > 0 and st["ga_s"] > 0 and st["finish_s"] >= 0
     finally:
         srv.close()
 
