@@ -23,7 +23,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-BASELINE_METRIC = "pose-evals/s and served docking jobs/min with p50/p99 latency vs plain GPU"
+METRIC = "pose-evals/s and served docking jobs/min with p50/p99 latency vs plain GPU"
 FLOP_PER_PAIR = 26          # FP32: 3 sub, 5 for d^2, 15 soft-core LJ, 3 Coulomb (division = 1)
 PEAK_FP32 = 157.3e12        # MI355X published FP32 peak
 PEAK_HBM = 8.0e12
@@ -163,7 +163,7 @@ def main():
     flop = FLOP_PER_PAIR * pairs
     achieved = flop / (kern_ms * 1e-3)
     result = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": value,
         "unit": "pose-evals/s",
         "n_gpus": 1,
@@ -255,7 +255,7 @@ def bench_uneven(args):
     flop = FLOP_PER_PAIR * pairs
     plan = b.plan()
     result = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": n / wall_b,
         "unit": "pose-evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -347,7 +347,7 @@ def bench_serve(args):
     plat = np.array(plat)
     pspan = free_at - arrivals[0]
     result = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": len(jobs) / span * 60.0,
         "unit": "jobs/min",
         "n_gpus": 1, "steps": len(jobs), "warmup": 1,
@@ -400,7 +400,7 @@ def bench_fcc(args):
     bits, sizes = cx.contacts(genes)
     t_contacts = time.perf_counter() - t0
     result = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": n / wall, "unit": "poses clustered/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -442,7 +442,7 @@ def bench_ga(args):
     wall = (time.perf_counter() - t0) / args.steps
     evals = res.generations * 150
     result = {
-        "metric": BASELINE_METRIC,
+        "metric": METRIC,
         "value": evals / wall, "unit": "pose-evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

@@ -10,7 +10,7 @@ import numpy as np
 GOLDEN = Path(__file__).resolve().parent / "golden"
 DATA = GOLDEN / "data"
 
-# north-star parity bar for every energy component: 1e-5 relative, 1e-3 kcal/mol absolute floor
+# parity bar for every energy component: 1e-5 relative, 1e-3 kcal/mol absolute floor
 REL_TOL = 1e-5
 ABS_TOL = 1e-3
 COMPONENTS = ["vdw", "elec", "desolv", "air", "restraint_penalty", "fitness"]
