@@ -92,7 +92,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (jobs would shard "
+        print("bench.py measures one GPU: the library has no multi-GPU path (jobs would shard "
               "across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "uneven_batch":
