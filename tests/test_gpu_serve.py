@@ -117,3 +117,24 @@ def test_submit_rejects_bad_parameters():
             srv.wait(12345)
     finally:
         srv.close()
+
+
+def test_job_failing_in_preparation_leaves_the_server_working():
+    """A receptor larger than the server's limit is accepted at submission and
+    fails while the job is prepared (on the preparing thread): its wait
+    reports the error, its memory is released, and the next job runs."""
+    big = _job(93, 60, 20)
+    small = _job(94, 20, 15)
+    srv = Server(max_jobs=2, max_receptor_atoms=200)
+    try:
+        bad = srv.submit(*big, seed=1, population_size=20, max_generations=3)
+        good = srv.submit(*small, seed=2, population_size=20, max_generations=3)
+        with pytest.raises(_lib.GdockError) as e:
+            srv.wait(bad, 120.0)
+        assert e.value.status == _lib.INVALID_ARG
+        res, _ = srv.wait(good, 120.0)
+        assert res.generations >= 1
+        st = srv.stats()
+        assert st["jobs_finished"] == 2 and st["bytes_in_use"] == 0
+    finally:
+        srv.close()
