@@ -257,6 +257,42 @@ void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& f
 
 }  // namespace gdock
 
+// Device scratch of gdock_score_poses_device_noalloc: one exact-rescore flag
+// per pose (256 KiB on every device the library runs on, held until the
+// process ends).  Host code never uses the name `scratch` as an address: its
+// device address is resolved once per device (scratch_resolve) and that
+// pointer is what kernels receive.
+namespace {
+constexpr int32_t kNoAllocMaxPoses = 1 << 16;
+static __device__ __align__(256) int32_t scratch[kNoAllocMaxPoses];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Resolves the scratch address on `device`, which must be current; once per
+// device, at context creation.
+void scratch_resolve(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+  g_scratch_err[device] = e;
+  g_scratch_done[device] = true;
+}
+
+int32_t* scratch_of(int device, hipError_t* err) {
+  if (device < 0 || device >= kMaxDevices) { *err = hipErrorInvalidDevice; return nullptr; }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  *err = g_scratch_done[device] ? g_scratch_err[device] : hipErrorNotInitialized;
+  return static_cast<int32_t*>(g_scratch_ptr[device]);
+}
+}  // namespace
+
 extern "C" {
 
 const char* gdock_version(void) { return "gdock-mi355x 0.1 (gfx950)"; }
@@ -273,6 +309,7 @@ gdock_status gdock_ctx_create(int device, gdock_ctx** out) {
   ctx->device = device;
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete ctx; return GDOCK_ERR_DEVICE; }
+  scratch_resolve(device);  // the no-allocation entry point's device scratch
   *out = ctx;
   return GDOCK_OK;
 }
@@ -551,6 +588,30 @@ gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const d
   e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat,
                    static_cast<int32_t*>(ctx->redo.ptr), s);
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device: launch");
+}
+
+// The same kernels as gdock_score_poses_device, with the per-pose rescore
+// flags in the device's static scratch instead of the context's arena: no
+// allocation of any kind.  The fast kernel writes every pose's flag before
+// the exact kernel reads it, so the scratch needs no clearing; two calls that
+// overlap on one device share the scratch and race.
+gdock_status gdock_score_poses_device_noalloc(gdock_ctx* ctx, gdock_complex* cx, const double* d_poses, int32_t n,
+                                              const gdock_weights* w, double* d_out, int32_t* d_nsat, void* stream) {
+  if (!ctx || !cx || !w || n < 0 || (n && (!d_poses || !d_out))) {
+    if (ctx) set_error(ctx, "score_poses_device_noalloc: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (n > kNoAllocMaxPoses) {
+    set_error(ctx, "score_poses_device_noalloc: %d poses exceed the limit of %d per call", n, kNoAllocMaxPoses);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSuccess;
+  int32_t* flags = scratch_of(ctx->device, &e);
+  if (!flags) return hip_fail(ctx, e, "score_poses_device_noalloc: device scratch");
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat, flags, s);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device_noalloc: launch");
 }
 
 gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,

@@ -141,6 +141,16 @@ gdock_status gdock_prepare_poses(const double* genes, int32_t n, double* poses);
 gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const double* d_poses,
                                       int32_t n, const gdock_weights* w, double* d_out,
                                       int32_t* d_nsat, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gdock_score_poses_device with the same kernels and
+ * results, but it allocates nothing.  Its per-pose scratch is a static device
+ * array of 256 KiB per GPU, resolved when a context is created on that GPU, so
+ * n is at most 65536 (GDOCK_ERR_INVALID_ARG beyond).  Calls that overlap on one
+ * GPU share that scratch and race: serialise them (one stream, or events). */
+gdock_status gdock_score_poses_device_noalloc(gdock_ctx* ctx, gdock_complex* cx, const double* d_poses,
+                                              int32_t n, const gdock_weights* w, double* d_out,
+                                              int32_t* d_nsat, void* stream);
 
 /* ---- batched scoring (many complexes, one persistent launch) ------------- */
 /* Population::eval_fitness (src/population.rs:48-61) for several complexes at
