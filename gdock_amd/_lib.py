@@ -137,6 +137,8 @@ def lib() -> C.CDLL:
         "gdock_batch_score": (C.c_int, [P, P, D, C.POINTER(Weights), D, I32]),
         "gdock_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "gdock_batch_launch": (C.c_int, [P, P, C.POINTER(Weights), P]),
+        "gdock_choose_lanes": (C.c_int32, [P]),
+        "gdock_time_lanes": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
         "gdock_ga_default_params": (C.c_int, [C.POINTER(GaParams)]),
         "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_free": (None, [P]),

@@ -145,6 +145,70 @@ void spread_blocks(std::vector<int32_t>& idx, size_t block) {
   idx.swap(out);
 }
 
+// Fixed-point value of an FP64 coordinate: (x - o) in units of 2^-qexp A,
+// clamped to +-kFixedClamp, rounded to nearest even.  The scoring kernel
+// applies the same operations to the posed ligand atoms.
+int32_t fixed_of(double x, double o, double scale) {
+  double v = (x - o) * scale;
+  v = std::min(std::max(v, -static_cast<double>(kFixedClamp)), static_cast<double>(kFixedClamp));
+  return static_cast<int32_t>(std::nearbyint(v));
+}
+
+// Finest fixed-point unit that holds the receptor within kFixedClamp units
+// minus 16 A (so a clamped ligand atom is beyond every cutoff); -1 if none.
+int fixed_exponent(double rec_extent) {
+  for (int e = kFixedMaxExp; e >= kFixedMinExp; --e)
+    if (rec_extent + 16.0 <= static_cast<double>(kFixedClamp) * std::ldexp(1.0, -e)) return e;
+  return -1;
+}
+
+// Guard bands of the FP32 cutoff decisions on s, the FP32 squared distance in
+// units^2 (u = 2^-qexp A).  For a pair with reference FP64 coordinates r, p:
+//   n_i = round(r_i 2^e) - round(p_i 2^e) (after subtracting the origin in
+//   FP64) is within a = 1 + 1e-6 units of the exact difference D_i;
+//   f_i = fl32(n_i) is within u32 |n_i| of n_i (u32 = 2^-24), so
+//   |f_i - D_i| <= eps = a + u32 (|D| + a);
+//   |sum f_i^2 - |D|^2| <= 2 sqrt(3) eps |D| + 3 eps^2;
+//   the three FP32 roundings of s = fma(z, z, fma(y, y, x x)) add at most
+//   3.0001 u32 s, and the reference's own FP64 rounding of d^2 6e-16 d^2.
+// Near a cutoff T (|D|^2 ~ T 2^2e) the band is twice the sum; a pair whose s
+// lies outside it gets the reference's decision from the FP32 comparison, a
+// pair inside it is decided again in FP64.  The four cutoffs share the
+// widest band width (in ulps of s).
+void fixed_bands(int qexp, uint32_t* c, uint32_t* w) {
+  const double T[4] = {1.0, 64.0, 144.0, 225.0};
+  const double u32 = 5.9604644775390625e-08, a = 1.000001;
+  uint32_t wmax = 0;
+  for (int i = 0; i < 4; ++i) {
+    const double Tu = std::ldexp(T[i], 2 * qexp), Du = std::sqrt(Tu) * 1.001;
+    const double eps = a + u32 * (Du + a);
+    const double b = 2.0 * (2.0 * 1.7320508075688772 * eps * Du + 3.0 * eps * eps + 3.0001 * u32 * Tu * 1.001 +
+                            6e-16 * Tu) + 1.0;
+    float lo = static_cast<float>(Tu - b), hi = static_cast<float>(Tu + b);
+    if (static_cast<double>(lo) > Tu - b) lo = std::nextafter(lo, -INFINITY);
+    if (static_cast<double>(hi) < Tu + b) hi = std::nextafter(hi, INFINITY);
+    uint32_t lb, hb;
+    std::memcpy(&lb, &lo, 4);
+    std::memcpy(&hb, &hi, 4);
+    c[i] = lb + 1u;
+    wmax = std::max(wmax, hb - c[i]);
+  }
+  *w = wmax;
+}
+
+// Charge unit of the receptor's FP32 charges: 1e4 when every charge is an
+// exact multiple of 1e-4 e (as every OPLS charge of the toppar table is), so
+// the FP32 values q * 1e4 are exact integers; otherwise 1 (charges rounded
+// to FP32).
+double charge_scale(const gdock_atom* a, int32_t n) {
+  constexpr double kScale = 1e4;
+  for (int32_t i = 0; i < n; ++i) {
+    const double m = std::nearbyint(a[i].charge * kScale);
+    if (std::fabs(m) >= 16777216.0 || m / kScale != a[i].charge) return 1.0;
+  }
+  return kScale;
+}
+
 namespace {
 
 // Pose records [lo, hi) from genes (genes already checked finite).
@@ -363,8 +427,8 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     set_error(ctx, "complex_create: bad sizes or null arrays");
     return GDOCK_ERR_INVALID_ARG;
   }
-  if (n_rec + 4 * 64 > kMaxLdsReceptor) {
-    set_error(ctx, "complex_create: %d receptor atoms exceeds the supported %d", n_rec, kMaxLdsReceptor);
+  if (n_rec > kMaxReceptorAtoms) {
+    set_error(ctx, "complex_create: %d receptor atoms exceeds the supported %d", n_rec, kMaxReceptorAtoms);
     return GDOCK_ERR_INVALID_ARG;
   }
   if (!finite_atoms(rec, n_rec) || !finite_atoms(lig, n_lig)) {
@@ -412,24 +476,38 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;  // serial -> device indices
   for (int32_t q = 0; q < n_lig; ++q) lig_by_serial[cx->lig[lig_perm[q]].serial].push_back(q);
 
-  std::vector<float4> xyzq(n_pad);
+  // Fixed-point frame: the receptor COM as origin, the finest unit that holds
+  // the receptor (with room to clamp far ligand atoms).
+  double rec_ext = 0.0;
+  for (int32_t i = 0; i < n_rec; ++i)
+    rec_ext = std::max({rec_ext, std::fabs(rec[i].x - origin[0]), std::fabs(rec[i].y - origin[1]),
+                        std::fabs(rec[i].z - origin[2])});
+  const int qexp = fixed_exponent(rec_ext);
+  if (qexp < 0) {
+    delete cx;
+    set_error(ctx, "complex_create: receptor extent %.1f A is too large for the fixed-point frame", rec_ext);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  const double qscale = std::ldexp(1.0, qexp);
+  const double charge_unit = charge_scale(rec, n_rec);
+  std::vector<int4> fixed(n_pad);
   std::vector<double> rx(n_pad), ry(n_pad), rz(n_pad), rasp(n_pad, 0.0), rrad(n_pad, 0.0);
   std::vector<int2> meta(n_pad, make_int2(0, 0));
-  double rec_ext = 0.0;
   std::vector<int4> coll;  // fitness.rs:92 pairs: same serial in receptor and ligand
   int cls = 0;
   for (int32_t j = 0; j < n_pad; ++j) {
     while (class_start[cls + 1] <= j) ++cls;
-    if (perm[j] < 0) {
-      xyzq[j] = make_float4(kPadRecCoord, kPadRecCoord, kPadRecCoord, 0.0f);
-      rx[j] = ry[j] = rz[j] = static_cast<double>(kPadRecCoord);
+    if (perm[j] < 0) {  // padding: far from every (clamped) ligand atom, never evaluated
+      fixed[j] = make_int4(kFixedPad, kFixedPad, kFixedPad, 0);
+      rx[j] = ry[j] = rz[j] = -1.0e9;
       continue;
     }
     const gdock_atom& a = cx->rec[perm[j]];
-    const double px = a.x - origin[0], py = a.y - origin[1], pz = a.z - origin[2];
-    xyzq[j] = make_float4(static_cast<float>(px), static_cast<float>(py), static_cast<float>(pz),
-                          static_cast<float>(a.charge));
-    rec_ext = std::max({rec_ext, std::fabs(px), std::fabs(py), std::fabs(pz)});
+    float q = static_cast<float>(a.charge * charge_unit);  // an exact integer when charge_unit > 1
+    int32_t qbits;
+    std::memcpy(&qbits, &q, 4);
+    fixed[j] = make_int4(fixed_of(a.x, origin[0], qscale), fixed_of(a.y, origin[1], qscale),
+                         fixed_of(a.z, origin[2], qscale), qbits);
     rx[j] = a.x; ry[j] = a.y; rz[j] = a.z;
     rasp[j] = asp_of(a);
     rrad[j] = a.vdw_radius;
@@ -440,12 +518,18 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);
+  // padded with zeros to whole chunks of the largest lane count: the kernel
+  // reads a lane's factor without a bounds test
+  const size_t n_lig_pad = (static_cast<size_t>(n_lig) + kScoreThreads * kMaxAtomsPerLane - 1) /
+                           (kScoreThreads * kMaxAtomsPerLane) * (kScoreThreads * kMaxAtomsPerLane);
+  std::vector<double> lkq(std::max<size_t>(n_lig_pad, 1), 0.0);
   double lig_r = 0.0;
   for (int32_t i = 0; i < n_lig; ++i) {  // device index i holds input atom lig_perm[i]
     const gdock_atom& a = cx->lig[lig_perm[i]];
     lx[i] = a.x; ly[i] = a.y; lz[i] = a.z;
     lrm[i] = a.rmin2; leps[i] = a.epsilon; lq[i] = a.charge;
     lasp[i] = asp_of(a); lrad[i] = a.vdw_radius; lser[i] = a.serial;
+    lkq[i] = kCoulomb * a.charge / charge_unit * std::ldexp(1.0, 2 * qexp);  // the sums hold q_r / s, s = d^2 / q^2
     const double dx = a.x - lcom[0], dy = a.y - lcom[1], dz = a.z - lcom[2];
     lig_r = std::max(lig_r, std::sqrt(dx * dx + dy * dy + dz * dz));
   }
@@ -468,26 +552,32 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   cx->n_restr = static_cast<int32_t>(rl_idx.size());
 
   Layout L;
-  const size_t o_xyzq = L.add<float4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
+  const size_t o_fixed = L.add<int4>(n_pad), o_rx = L.add<double>(n_pad), o_ry = L.add<double>(n_pad),
                o_rz = L.add<double>(n_pad), o_rasp = L.add<double>(n_pad), o_rrad = L.add<double>(n_pad),
                o_meta = L.add<int2>(n_pad), o_coll = L.add<int4>(coll.size()),
                o_lorig = L.add<int32_t>(n_lig), o_cs = L.add<int32_t>(nc + 1), o_crm = L.add<double>(nc),
                o_ceps = L.add<double>(nc), o_lx = L.add<double>(n_lig), o_ly = L.add<double>(n_lig),
                o_lz = L.add<double>(n_lig), o_lrm = L.add<double>(n_lig), o_leps = L.add<double>(n_lig),
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
-               o_lser = L.add<int32_t>(n_lig), o_rrx = L.add<double>(rr_xyz.size()),
+               o_lser = L.add<int32_t>(n_lig), o_lkq = L.add<double>(lkq.size()), o_rrx = L.add<double>(rr_xyz.size()),
                o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1),
                o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig);
   // Lennard-Jones factors of every (receptor class, ligand atom) pair, with
-  // the FP64 operations of fitness.rs:58-63 (R = rmin2_r + rmin2_l, R^6 as
-  // R^2 (R^2 R^2), 1 / R^6; sqrt(eps_r eps_l)) rounded once to FP32.
+  // the FP64 operations of fitness.rs:58-63 (R = rmin2_r + rmin2_l;
+  // sqrt(eps_r eps_l)), as q^2 / R^2 2^-16 in the kernel's squared units
+  // (the 2^-16 keeps the pair step's reciprocal at the scale of A), rounded
+  // once to FP32.  1 / R^2 is capped at 5e4 (R >= 4.5e-3 A): only
+  // unparameterized pairs (R = 0, eps = 0, a zero term) and nearly
+  // coincident atoms notice the cap, and with it every product of the pair
+  // step stays finite.
   std::vector<float2> lcls(static_cast<size_t>(nc) * n_lig);
+  const double q2 = std::ldexp(1.0, -2 * qexp - 16);
   for (int c = 0; c < nc; ++c)
     for (int32_t i = 0; i < n_lig; ++i) {
       const double R = class_rmin[c] + lrm[i];
-      const double R2 = R * R, R6 = R2 * (R2 * R2);
+      const double inv = R * R > 2e-5 ? 1.0 / (R * R) : 5e4;
       float2& f = lcls[static_cast<size_t>(c) * n_lig + i];
-      f.x = R6 > 1e-10 ? static_cast<float>(1.0 / R6) : 1.0e10f;
+      f.x = static_cast<float>(std::min(inv, 5e4) * q2);
       f.y = static_cast<float>(std::sqrt(class_eps[c] * leps[i]));
     }
   cx->contacts_layout_begin = L.size;
@@ -495,7 +585,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   build_contact_tables(cx, &L, &host);  // FCC residue tables appended to the same arena
   std::vector<uint8_t> blob(L.size, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(blob.data() + off, src, bytes); };
-  put(o_xyzq, xyzq.data(), xyzq.size() * sizeof(float4));
+  put(o_fixed, fixed.data(), fixed.size() * sizeof(int4));
   put(o_rx, rx.data(), n_pad * 8); put(o_ry, ry.data(), n_pad * 8); put(o_rz, rz.data(), n_pad * 8);
   put(o_rasp, rasp.data(), n_pad * 8); put(o_rrad, rrad.data(), n_pad * 8);
   put(o_meta, meta.data(), n_pad * sizeof(int2));
@@ -505,6 +595,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_lx, lx.data(), n_lig * 8); put(o_ly, ly.data(), n_lig * 8); put(o_lz, lz.data(), n_lig * 8);
   put(o_lrm, lrm.data(), n_lig * 8); put(o_leps, leps.data(), n_lig * 8); put(o_lq, lq.data(), n_lig * 8);
   put(o_lasp, lasp.data(), n_lig * 8); put(o_lrad, lrad.data(), n_lig * 8); put(o_lser, lser.data(), n_lig * 4);
+  put(o_lkq, lkq.data(), lkq.size() * 8);
   put(o_rrx, rr_xyz.data(), rr_xyz.size() * 8); put(o_rli, rl_idx.data(), rl_idx.size() * 4);
   put(o_lcls, lcls.data(), lcls.size() * sizeof(float2));
   if (!host.empty()) std::memcpy(blob.data() + cx->contacts_layout_begin, host.data(), host.size());
@@ -515,7 +606,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   cx->mem_bytes = std::max<size_t>(L.size, 256);
   char* m = static_cast<char*>(cx->mem);
   DevComplex& d = cx->dev;
-  d.rec_xyzq = reinterpret_cast<const float4*>(m + o_xyzq);
+  d.rec_fixed = reinterpret_cast<const int4*>(m + o_fixed);
   d.rec_x = reinterpret_cast<const double*>(m + o_rx);
   d.rec_y = reinterpret_cast<const double*>(m + o_ry);
   d.rec_z = reinterpret_cast<const double*>(m + o_rz);
@@ -536,6 +627,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.lig_asp = reinterpret_cast<const double*>(m + o_lasp);
   d.lig_radius = reinterpret_cast<const double*>(m + o_lrad);
   d.lig_serial = reinterpret_cast<const int32_t*>(m + o_lser);
+  d.lig_kq = reinterpret_cast<const double*>(m + o_lkq);
   d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
   d.lig_class = reinterpret_cast<const float2*>(m + o_lcls);
@@ -543,6 +635,11 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.n_coll = static_cast<int32_t>(coll.size());
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }
   d.rec_extent = rec_ext;
+  d.qexp = qexp;
+  fixed_bands(qexp, d.band_c, &d.band_w);
+  d.q_scale = charge_unit;
+  d.parts = 1;
+  d.min_parts = 1;
   d.lig_extent = lig_r + com_off;
   bind_contact_tables(cx);
   cx->d_dev = reinterpret_cast<const DevComplex*>(m + o_dev);
@@ -584,9 +681,8 @@ gdock_status gdock_score_poses_device(gdock_ctx* ctx, gdock_complex* cx, const d
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
   hipError_t e = hipSuccess;
-  if (!ctx->redo.reserve(static_cast<size_t>(std::max(n, 1)) * 4, &e)) return hip_fail(ctx, e, "score_poses_device: scratch");
-  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat,
-                   static_cast<int32_t*>(ctx->redo.ptr), s);
+  if (!ctx->redo.reserve(plain_scratch_bytes(cx->dev, n), &e)) return hip_fail(ctx, e, "score_poses_device: scratch");
+  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, n, ww, d_out, d_nsat, ctx->redo.ptr, s);
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device: launch");
 }
 
@@ -605,12 +701,17 @@ gdock_status gdock_score_poses_device_noalloc(gdock_ctx* ctx, gdock_complex* cx,
     set_error(ctx, "score_poses_device_noalloc: %d poses exceed the limit of %d per call", n, kNoAllocMaxPoses);
     return GDOCK_ERR_INVALID_ARG;
   }
+  if (plain_parts(cx->dev) > 1) {
+    set_error(ctx, "score_poses_device_noalloc: a receptor of %d slots needs split-pose state; use "
+              "gdock_score_poses_device", cx->dev.n_rec);
+    return GDOCK_ERR_INVALID_ARG;
+  }
   hipError_t e = hipSuccess;
   int32_t* flags = scratch_of(ctx->device, &e);
   if (!flags) return hip_fail(ctx, e, "score_poses_device_noalloc: device scratch");
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, nullptr, n, ww, d_out, d_nsat, flags, s);
+  e = launch_score(cx->dev, cx->d_dev, d_poses, nullptr, n, ww, d_out, d_nsat, flags, s);
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses_device_noalloc: launch");
 }
 
@@ -631,14 +732,14 @@ gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* 
   const size_t pb = poses.size() * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8, nb = static_cast<size_t>(n) * 4;
   const size_t o_out = (pb + 255) & ~static_cast<size_t>(255), o_ns = (o_out + ob + 255) & ~static_cast<size_t>(255),
                o_rd = (o_ns + nb + 255) & ~static_cast<size_t>(255);
-  if (!ctx->staging.reserve(o_rd + nb, &e)) return hip_fail(ctx, e, "score_poses: staging");
+  if (!ctx->staging.reserve(o_rd + plain_scratch_bytes(cx->dev, n), &e)) return hip_fail(ctx, e, "score_poses: staging");
   char* base = static_cast<char*>(ctx->staging.ptr);
   e = hipMemcpyAsync(base, poses.data(), pb, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: upload");
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
-  e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(base), nullptr, nullptr, n, ww,
-                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns),
-                   reinterpret_cast<int32_t*>(base + o_rd), ctx->stream);
+  e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(base), nullptr, n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), base + o_rd,
+                   ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_poses: launch");
   e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);
@@ -652,29 +753,21 @@ gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double*
   if (n == 0) return GDOCK_OK;
   const int32_t nl = cx->dev.n_lig;
   const size_t ncoord = static_cast<size_t>(n) * nl * 3;
-  std::vector<double> extent(n, 0.0);
-  for (int32_t p = 0; p < n; ++p)
-    for (int32_t i = 0; i < nl; ++i)
-      for (int k = 0; k < 3; ++k) {
-        const double v = lig_xyz[(static_cast<size_t>(p) * nl + i) * 3 + k];
-        if (!std::isfinite(v)) { set_error(ctx, "score_coords: non-finite coordinate"); return GDOCK_ERR_INVALID_ARG; }
-        extent[p] = std::max(extent[p], std::fabs(v - cx->dev.origin[k]));
-      }
+  for (size_t i = 0; i < ncoord; ++i)
+    if (!std::isfinite(lig_xyz[i])) { set_error(ctx, "score_coords: non-finite coordinate"); return GDOCK_ERR_INVALID_ARG; }
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: set device");
-  const size_t cb = ncoord * 8, eb = static_cast<size_t>(n) * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8,
-               nb = static_cast<size_t>(n) * 4;
+  const size_t cb = ncoord * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8, nb = static_cast<size_t>(n) * 4;
   auto al = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
-  const size_t o_ext = al(cb), o_out = al(o_ext + eb), o_ns = al(o_out + ob), o_rd = al(o_ns + nb);
-  if (!ctx->staging.reserve(o_rd + nb, &e)) return hip_fail(ctx, e, "score_coords: staging");
+  const size_t o_out = al(cb), o_ns = al(o_out + ob), o_rd = al(o_ns + nb);
+  if (!ctx->staging.reserve(o_rd + plain_scratch_bytes(cx->dev, n), &e)) return hip_fail(ctx, e, "score_coords: staging");
   char* base = static_cast<char*>(ctx->staging.ptr);
   e = hipMemcpyAsync(base, lig_xyz, cb, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(base + o_ext, extent.data(), eb, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: upload");
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
-  e = launch_score(cx->dev, cx->d_dev, nullptr, reinterpret_cast<double*>(base), reinterpret_cast<double*>(base + o_ext), n, ww,
-                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns),
-                   reinterpret_cast<int32_t*>(base + o_rd), ctx->stream);
+  e = launch_score(cx->dev, cx->d_dev, nullptr, reinterpret_cast<double*>(base), n, ww,
+                   reinterpret_cast<double*>(base + o_out), reinterpret_cast<int32_t*>(base + o_ns), base + o_rd,
+                   ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "score_coords: launch");
   e = hipMemcpyAsync(out, base + o_out, ob, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && n_satisfied) e = hipMemcpyAsync(n_satisfied, base + o_ns, nb, hipMemcpyDeviceToHost, ctx->stream);

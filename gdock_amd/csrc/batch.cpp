@@ -47,44 +47,24 @@ struct gdock_batch {
 
 namespace gdock {
 
-// Ligand atoms per lane, by measurement.  A ligand of n atoms is scored in
-// ceil(n / (256 K)) chunks of 256 K lanes-slots; each chunk walks the whole
-// receptor.  Candidates are the K of each chunk count (the smallest K that
-// covers the ligand in that many chunks); the fastest on a probe launch wins.
-// Decisions are cached per (ligand atoms, receptor size class) for the
-// process, so every complex of the same shape gets the same K (and so the
-// same summation order) whatever batch it is in.
-namespace {
-std::mutex g_tune_mu;
-std::map<std::pair<int, int>, int> g_tuned;
-
-std::vector<int> lane_candidates(int n_lig) {
-  std::vector<int> ks;
-  for (int chunks = 1; chunks <= 16; ++chunks) {
-    const int k = (n_lig + kScoreThreads * chunks - 1) / (kScoreThreads * chunks);
-    if (k < 1) break;
-    if (k > kMaxAtomsPerLane) continue;
-    if (std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
-    if (ks.size() >= 3) break;
-  }
-  if (ks.empty()) ks.push_back(1);
-  return ks;
-}
-
-int rec_size_class(int n_rec) { return static_cast<int>(std::lround(2.0 * std::log2(std::max(n_rec, 1)))); }
-}  // namespace
-
 // Largest number of workgroups one pose of a complex may be split over in
 // batched launches: enough parts of kSplitCost for a pose with every
 // receptor slot within reach, at least 512 receptor slots each, at most
-// kMaxPoseParts.  Each launch picks every pose's own count up to this
-// (pose_plan_kernel).  A function of the complex's shape only.
+// kMaxPoseParts, and never fewer than min_batch_parts.  Each launch picks
+// every pose's own count between the two (pose_plan_kernel).  A function of
+// the complex's shape only.
+int min_batch_parts(const DevComplex& d) {
+  const int tiles = (d.n_rec + 63) / 64, per = kPartSlots / 64;
+  return std::max(1, (tiles + per - 1) / per);
+}
+
 int pose_parts(const DevComplex& d, int lanes_k) {
   const int64_t per_chunk = static_cast<int64_t>(kScoreThreads) * std::max(lanes_k, 1);
   const double slots = static_cast<double>((d.n_lig + per_chunk - 1) / per_chunk * per_chunk);
   const double worst = 1.0625 * d.n_rec * slots;
   const int64_t p = std::min<int64_t>(static_cast<int64_t>(std::ceil(worst / kSplitCost)), d.n_rec / 512);
-  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(p, kMaxPoseParts)));
+  const int64_t q = std::max<int64_t>(p, min_batch_parts(d));
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(q, kMaxPoseParts)));
 }
 
 size_t split_state_bytes(const DevComplex& d, int parts, int poses) {
@@ -107,62 +87,9 @@ void bind_split_state(DevComplex* d, int parts, int poses, int pose_base, char* 
   d->split_sync = reinterpret_cast<int32_t*>(mem + a + b);
 }
 
-int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err) {
-  *err = hipSuccess;
-  const int n_lig = cx->dev.n_lig;
-  const std::pair<int, int> key{n_lig, rec_size_class(cx->dev.n_rec_atoms)};
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tuned.find(key);
-    if (it != g_tuned.end()) return it->second;
-  }
-  const std::vector<int> ks = lane_candidates(n_lig);
-  int best = ks.front();
-  if (ks.size() > 1) {
-    // Probe poses: fixed rotations, the ligand kept on the receptor.
-    constexpr int kProbe = 128;
-    std::vector<double> genes(kProbe * 6), poses(kProbe * kPoseDoubles);
-    uint64_t s = 0x9e3779b97f4a7c15ull;
-    auto u01 = [&s]() {
-      s = s * 6364136223846793005ull + 1442695040888963407ull;
-      return static_cast<double>(s >> 11) * (1.0 / 9007199254740992.0);
-    };
-    for (int i = 0; i < kProbe; ++i) {
-      for (int a = 0; a < 3; ++a) genes[6 * i + a] = 6.283185307179586 * u01();
-      for (int a = 3; a < 6; ++a) genes[6 * i + a] = 10.0 * (u01() - 0.5);
-    }
-    gdock_prepare_poses(genes.data(), kProbe, poses.data());
-    double* d = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipMalloc(&d, kProbe * (kPoseDoubles + kOutDoubles + 2) * sizeof(double));
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d, poses.data(), poses.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-    double* d_out = d + kProbe * kPoseDoubles;
-    int32_t* d_ns = reinterpret_cast<int32_t*>(d_out + kProbe * kOutDoubles);
-    int32_t* d_redo = d_ns + kProbe;
-    const Weights w{0.4, 0.05, 3.4, 1.0};
-    float best_ms = 0.0f;
-    for (size_t c = 0; c < ks.size() && e == hipSuccess; ++c) {
-      e = launch_score(cx->dev, cx->d_dev, d, nullptr, nullptr, kProbe, w, d_out, d_ns, d_redo, ctx->stream, ks[c]);
-      if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
-      for (int r = 0; r < 2 && e == hipSuccess; ++r)
-        e = launch_score(cx->dev, cx->d_dev, d, nullptr, nullptr, kProbe, w, d_out, d_ns, d_redo, ctx->stream, ks[c]);
-      if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.0f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e == hipSuccess && (c == 0 || ms < best_ms)) { best_ms = ms; best = ks[c]; }
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (d) (void)hipFree(d);
-    if (e != hipSuccess) { *err = e; return 0; }
-  }
-  std::lock_guard<std::mutex> g(g_tune_mu);
-  return g_tuned.emplace(key, best).first->second;
-}
+// Most receptor slots one item of a complex covers in batched launches (the
+// LDS burial table of the batch kernel must hold them).
+int batch_part_slots(const DevComplex& d) { return part_slots(d.n_rec, min_batch_parts(d)); }
 
 }  // namespace gdock
 
@@ -182,13 +109,8 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
       set_error(ctx, "batch_create: complex %d: null complex or negative pose count", i);
       return GDOCK_ERR_INVALID_ARG;
     }
-    if (complexes[i]->dev.n_rec > kMaxLdsReceptor) {
-      set_error(ctx, "batch_create: complex %d: %d receptor slots exceed the LDS table (%d)", i,
-                complexes[i]->dev.n_rec, kMaxLdsReceptor);
-      return GDOCK_ERR_INVALID_ARG;
-    }
     total += poses_per_complex[i];
-    max_rec = std::max(max_rec, complexes[i]->dev.n_rec);
+    max_rec = std::max(max_rec, batch_part_slots(complexes[i]->dev));
   }
   if (total > (1 << 30)) {
     set_error(ctx, "batch_create: %lld poses is too many", static_cast<long long>(total));
@@ -205,10 +127,7 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   b->n_poses = static_cast<int32_t>(total);
   b->max_n_rec = max_rec;
   b->lanes_k.resize(n);
-  for (int32_t i = 0; i < n; ++i) {
-    b->lanes_k[i] = tune_lanes(ctx, b->cx[i], &e);
-    if (e != hipSuccess) { delete b; return hip_fail(ctx, e, "batch_create: autotune"); }
-  }
+  for (int32_t i = 0; i < n; ++i) b->lanes_k[i] = choose_lanes(b->cx[i]->dev);
 
   b->parts.resize(n);
   for (int32_t i = 0; i < n; ++i) {
@@ -229,6 +148,7 @@ gdock_status gdock_batch_create(gdock_ctx* ctx, gdock_complex* const* complexes,
   for (int32_t i = 0; i < n; ++i) {
     table[i] = b->cx[i]->dev;
     table[i].lanes_k = b->lanes_k[i];
+    table[i].min_parts = min_batch_parts(table[i]);
   }
   const size_t np = std::max<int64_t>(total, 1);
   const size_t ni = std::max<int64_t>(b->item_capacity, 1);

@@ -1,6 +1,7 @@
 // Shared host/device definitions of the gfx950 scoring and FCC library.
 #pragma once
 #include <cstdint>
+#include <climits>
 
 namespace gdock {
 
@@ -28,20 +29,38 @@ constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, 
 
 constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup
 constexpr int kMaxAtomsPerLane = 8;
-constexpr int kMaxPoseParts = 8;  // workgroups one pose can be split over
-constexpr int kMaxLdsReceptor = 16000;  // receptor burial counters kept in LDS
-constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes sit far away ...
-constexpr float kPadRecCoord = -1.0e4f;  // ... on the other side from padding receptor atoms
+constexpr int kMaxPoseParts = 15;  // workgroups one pose can be split over (a 4-bit field of a work item)
+// Receptor slots one workgroup covers: the whole receptor in the plain
+// one-workgroup-per-pose path up to kMaxLdsReceptor slots (larger receptors
+// are split over receptor ranges), at most kPartSlots slots per workgroup in
+// batched launches (the burial-count table is 4 B per slot of LDS: 32 KiB
+// leaves room for 4 workgroups per CU).
+constexpr int kMaxLdsReceptor = 16384;
+constexpr int kPartSlots = 8192;
+constexpr int kMaxReceptorAtoms = kMaxPoseParts * kPartSlots - 4 * 64;
+constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes of the contact kernel sit far away
 constexpr int kRecGroup = 4;             // receptor atoms per scalar load
 constexpr int kLigBlock = 64;            // ligand atoms per spatial block (one wave's slot)
 // A receptor group and a wave's ligand atoms farther apart than this have no
 // pair within any cutoff (15 A) or its guard band.
 constexpr float kCullReach = 15.01f;
 
+// Fixed-point coordinates of the scoring kernels: integers in units of
+// 2^-qexp A in the receptor-COM frame, so a pair's difference is exact and
+// only its FP32 conversion rounds (relative to the distance, not the frame).
+// Ligand coordinates are clamped to +-kFixedClamp units; receptor atoms lie
+// within kFixedClamp units - 16 A, so a clamped atom is beyond every cutoff of
+// every receptor atom.  Receptor padding slots hold kFixedPad (far from the
+// clamped cube; never evaluated) and ligand padding lanes kFixedClamp.
+constexpr int32_t kFixedClamp = (1 << 30) - 1;
+constexpr int32_t kFixedPad = INT32_MIN;
+constexpr int kFixedMaxExp = 24;
+constexpr int kFixedMinExp = 8;
+
 // Device view of one complex (kernel argument, plain pointers).
 struct DevComplex {
   // receptor, permuted so that atoms of one (rmin2, epsilon) class are contiguous
-  const float4* rec_xyzq;      // x, y, z relative to origin (FP32), charge
+  const int4* rec_fixed;       // x, y, z fixed point (units of 2^-qexp A, origin frame); w: FP32 bits of the charge in units of 1 / q_scale
   const double* rec_x;         // FP64 original coordinates (permuted)
   const double* rec_y;
   const double* rec_z;
@@ -52,7 +71,7 @@ struct DevComplex {
   const int32_t* class_start;  // n_class + 1
   const double* class_rmin2;
   const double* class_eps;
-  const float2* lig_class;     // n_class x n_lig: {1/R^6, sqrt(eps_r eps_l)} per receptor class and ligand atom
+  const float2* lig_class;     // n_class x n_lig: {q^2 / R^2 (q: the fixed-point unit), sqrt(eps_r eps_l)} per receptor class and ligand atom
   // ligand (positioned), in spatial block order (lig_orig: index in the input)
   const double* lig_x;
   const double* lig_y;
@@ -64,6 +83,7 @@ struct DevComplex {
   const double* lig_radius;
   const int32_t* lig_serial;
   const int32_t* lig_orig;
+  const double* lig_kq;        // kCoulomb * charge / q_scale / q^2 per ligand atom (the elec fold factor; sums hold q_r / s), zero-padded to whole chunks
   // restraints: receptor CA coordinates (FP64) and ligand CA index
   const double* restr_rec_xyz;  // n_restr x 3
   const int32_t* restr_lig;     // n_restr
@@ -71,18 +91,26 @@ struct DevComplex {
   int32_t n_rec_atoms;  // real receptor atoms
   int32_t n_lig, n_class, n_restr;
   int32_t n_coll;       // pairs in coll_pairs
+  int32_t qexp;         // fixed-point exponent: coordinates in units of 2^-qexp A
+  // FP32 decisions on the squared distance in units^2, as integers of its bit
+  // pattern: bits(s) - band_c[T] borrows exactly when s is below the guard band
+  // of cutoff T (1, 8, 12, 15 A), and lies within band_w of it inside the band.
+  uint32_t band_c[4];
+  uint32_t band_w;
+  double q_scale;       // receptor charges are stored as charge * q_scale (exact integers when q_scale > 1)
   int32_t lanes_k;      // ligand atoms per lane chosen for batched launches (0: ceil(n_lig / 256), at most 8)
   // batched launches: a pose may be split across up to `parts` workgroups by
   // receptor range (1: never split; each launch picks every pose's count),
   // with the split state of the complex's poses, indexed by global pose row -
   // split_pose_base
   int32_t parts;
+  int32_t min_parts;    // parts every pose takes at least (receptor slots over the per-workgroup LDS table)
   int32_t split_pose_base;
   int32_t* split_lig;   // n_lig ints per pose
   double* split_sums;   // 3 doubles per part (up to `parts`) per pose
   int32_t* split_sync;  // 2 ints per pose
   double com[3];     // ligand centre of mass (rotation centre, structure.rs:82)
-  double origin[3];  // FP32 frame origin
+  double origin[3];  // fixed-point frame origin (receptor COM)
   double rec_extent;   // max |rec - origin| component
   double lig_extent;   // max |lig - com| (2-norm) + max |com - origin| component
 };

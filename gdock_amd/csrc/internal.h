@@ -73,14 +73,22 @@ std::string_view field(const char* f, size_t n);
 void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::vector<int32_t>* offsets);
 
 // Scores n_poses poses: the fast kernel, then the exact kernel for any pose
-// whose settle buffer overflowed.  d_redo: n_poses int32 of device scratch.
-// lanes_k: ligand atoms per lane (1-8); 0 picks ceil(n_lig / 256) capped at 8.
+// whose settle buffer overflowed.  scratch: plain_scratch_bytes(cx, n_poses)
+// of device memory.  lanes_k: ligand atoms per lane (1-8); 0 picks
+// ceil(n_lig / 256) capped at 8.
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
-                        const double* d_extent, int n_poses, const Weights& w, double* d_out,
-                        int32_t* d_nsat, int32_t* d_redo, hipStream_t stream, int lanes_k = 0);
+                        int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
+                        hipStream_t stream, int lanes_k = 0);
 int score_atoms_per_lane(int n_lig);
+// Receptor-range parts of the plain path (1 unless the receptor exceeds one
+// workgroup's LDS table) and its scratch bytes (flags, split state).
+int plain_parts(const DevComplex& cx);
+size_t plain_scratch_bytes(const DevComplex& cx, int n_poses);
+// Most receptor slots one of `parts` equal tile ranges covers.
+int part_slots(int n_rec, int parts);
+size_t score_lds_bytes(int slots);
 
-size_t batch_lds_bytes(int max_n_rec);
+size_t batch_lds_bytes(int max_slots);
 int batch_grid(size_t lds, int device);
 hipError_t launch_batch(const BatchLaunch& b, const Weights& w, hipStream_t stream);
 // Device bytes of the per-launch item order for `capacity` items, and the
@@ -108,11 +116,16 @@ struct Layout {
 // spatially compact (recursive bisection across the widest extent).
 void bisect_order(const gdock_atom* atoms, std::vector<int32_t>& idx, size_t lo, size_t hi, size_t block);
 
-// Ligand atoms per lane for a complex, measured once per complex shape.
-int tune_lanes(gdock_ctx* ctx, gdock_complex* cx, hipError_t* err);
-// Most workgroups one pose of a complex may be split over in batched launches
-// (1 = never split, at most kMaxPoseParts); each launch picks every pose's own.
+// Ligand atoms per lane of a complex shape (a pure function of the shape),
+// and the autotuner's candidates for a ligand size.
+int choose_lanes(const DevComplex& d);
+std::vector<int> lane_candidates(int n_lig);
+// Parts of a pose in batched launches: at least min_batch_parts (receptor
+// ranges of at most kPartSlots slots), at most pose_parts (also 1 = never
+// split, at most kMaxPoseParts); each launch picks every pose's count.
+int min_batch_parts(const DevComplex& d);
 int pose_parts(const DevComplex& d, int lanes_k);
+int batch_part_slots(const DevComplex& d);
 // Device bytes of the split state of `poses` poses, and its binding into a
 // table entry (pose rows [pose_base, pose_base + poses), state cleared).
 size_t split_state_bytes(const DevComplex& d, int parts, int poses);

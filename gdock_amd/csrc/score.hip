@@ -1,32 +1,39 @@
 // Population scoring kernel for gfx950 (CDNA4, wave64).
 //
-// One workgroup (4 x wave64) scores one pose: it rigidly transforms the
-// ligand in FP64 (structure.rs:80-131 operation order), keeps K ligand atoms
-// per lane in registers as FP32 coordinates, and walks the receptor.  Receptor
-// atoms are wave-uniform, so they are read with scalar loads into SGPRs and
-// every pair costs VALU work only; the receptor (16 B/atom) stays resident in
-// the scalar cache / L2 across the whole launch.
+// One workgroup (4 x wave64) scores one pose (or one receptor range of a
+// pose): it rigidly transforms the ligand in FP64 (structure.rs:80-131
+// operation order), keeps K ligand atoms per lane in registers as fixed-point
+// coordinates, and walks the receptor.  Receptor atoms are wave-uniform, so
+// they are read with scalar loads into SGPRs and every pair costs VALU work
+// only; the receptor (16 B/atom) stays resident in the scalar cache / L2
+// across the whole launch.
 //
 // Per pair (fitness.rs:57-105,122-184,284-302): soft-core LJ, Coulomb with
-// eps = r, and the two desolvation burial counts.  Pair math is FP32.  Every
-// cutoff decision (d > 1, d < 8, d < 12, d < 15) is taken in FP32 against a
-// guard band derived from the coordinate magnitudes.  A pair inside a band is
-// rare; the wave records it as an event in LDS and, once per ligand chunk,
-// decides it again in FP64 from the original coordinates with the reference's
-// operation order and applies the difference, so every decision matches the
-// FP64 reference.  (A wave whose event buffer overflows flags its pose, and
-// the exact variant of the kernel, launched right after, rescores that pose
-// deciding such pairs inline.)  Energies are accumulated per lane in FP32 over
-// 64-atom receptor tiles, then in FP64; the cross-lane and cross-wave
-// reductions are FP64.  Desolvation burial counts are integers; the per-atom
-// desolvation and AIR restraint terms and the weighted fitness are computed
-// in FP64 in the reference's order.
+// eps = r, and the two desolvation burial counts.  Coordinates are integers in
+// units of 2^-qexp A (qexp up to 24), so a pair's coordinate difference is
+// exact and only its FP32 conversion rounds: the squared distance carries an
+// error relative to the distance itself, not to the size of the frame.  Pair
+// math is FP32.  Every cutoff decision (d > 1, d < 8, d < 12, d < 15) is taken
+// in FP32 against a guard band; a pair inside a band is rare, the wave records
+// it as an event in LDS and, once per ligand chunk, decides it again in FP64
+// from the original coordinates with the reference's operation order and
+// applies the difference, so every decision matches the FP64 reference.  (A
+// wave whose event buffer overflows flags its pose, and the exact variant of
+// the kernel, launched right after, rescores that pose deciding such pairs
+// inline.)  Receptor charges are exact integers in units of 1e-4 e (OPLS
+// charges), 1/d^2 is a Newton-refined reciprocal, Coulomb terms are summed
+// per lane in FP32 over a group of 4 receptor atoms and then in FP64,
+// soft-core LJ terms in FP32 per 64-slot tile and class and then in FP64;
+// the cross-lane and cross-wave reductions are FP64.  Desolvation burial
+// counts are integers; the per-atom desolvation and AIR restraint terms and
+// the weighted fitness are computed in FP64 in the reference's order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
 
 #include "common.h"
+#include "internal.h"
 
 // The fast scoring kernels run at 4 waves per SIMD (at most 128 VGPRs): the
 // pair loop is VALU-bound and needs the fourth wave to hide its latencies.
@@ -39,10 +46,12 @@ namespace gdock {
 #define GDOCK_AS4 __attribute__((address_space(4)))
 using CxRef = const DevComplex GDOCK_AS4&;
 
-struct RecAtom { float x, y, z, w; };  // receptor x, y, z (FP32, origin frame), charge
+// Receptor atom: fixed-point x, y, z (origin frame) and its charge in units
+// of 1 / q_scale.
+struct RecAtom { int32_t x, y, z; float w; };
 
-__device__ __forceinline__ RecAtom rec_atom(const float GDOCK_AS4* rf, int j) {
-  return RecAtom{rf[4 * j], rf[4 * j + 1], rf[4 * j + 2], rf[4 * j + 3]};
+__device__ __forceinline__ RecAtom rec_atom(const int32_t GDOCK_AS4* rf, int j) {
+  return RecAtom{rf[4 * j], rf[4 * j + 1], rf[4 * j + 2], __int_as_float(rf[4 * j + 3])};
 }
 
 struct PoseCoef {
@@ -86,56 +95,44 @@ __device__ __forceinline__ float uniform(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// Guard band around a squared cutoff T for FP32 d2 computed from coordinates
-// rounded to FP32 in a frame where every component is at most X in magnitude.
-// Bound: |d2_f32 - d2| <= 2 sqrt(3) d e + 3 e^2 + 3.01 u T with e = 2uX + u d,
-// u = 2^-24; the band is twice that.  Returned as FP32 bounds rounded outward.
-__device__ __forceinline__ void guard_band(double T, double X, float& lo, float& hi) {
-  const double u = 5.9604644775390625e-08;
-  const double s = sqrt(T) * 1.01 + 0.01;
-  const double e = 2.0 * u * X + u * s;
-  const double b = 2.0 * (3.4641016151377544 * s * e * (1.0 + 2.0 * u) + 3.0 * e * e + 3.01 * u * T * 1.01) + 1e-9;
-  float l = static_cast<float>(T - b), h = static_cast<float>(T + b);
-  if (static_cast<double>(l) > T - b) l = nextafterf(l, -INFINITY);
-  if (static_cast<double>(h) < T + b) h = nextafterf(h, INFINITY);
-  lo = uniform(l);
-  hi = uniform(h);
+// Fixed-point value of a posed FP64 coordinate, the same operations as the
+// host applies to the receptor: (p - origin) 2^qexp, clamped, rounded to
+// nearest even.
+__device__ __forceinline__ int32_t to_fixed(double p, double o, double scale) {
+  double v = (p - o) * scale;
+  v = fmin(fmax(v, -static_cast<double>(kFixedClamp)), static_cast<double>(kFixedClamp));
+  return static_cast<int32_t>(rint(v));
 }
 
-// One receptor atom r (FP32, wave-uniform) against the K ligand atoms of the
-// lane.  The fast path decides every cutoff in FP32 (d2 <= lo_T: below the
-// guard band of T; d2 > hi_1: beyond the elec lower cutoff).  A pair whose d2
-// lies inside a band, or at d <= ~1 A, flags the receptor atom; settle_pairs()
-// then re-decides those pairs in FP64 exactly as the reference does and adds
-// the difference to what the fast path accumulated.
-//
-// Bands are kept as integers: for a non-negative float, its bit pattern
-// orders like its value.  c_T = bits(lo_T) + 1, so bits(d2) - c_T borrows
-// exactly when d2 <= lo_T (the FP32 decision), and the same difference, as
-// an unsigned number below a width in units in the last place, puts d2
-// inside the band of T.  The 1 A band has a width of its own (w1): near
-// d2 = 1 the coordinate rounding error spans several hundred ulps, ten times
-// more than at the 8, 12 and 15 A cutoffs, which share the widest of their
-// three widths (w); a narrower band widened to w only sends a few more pairs
-// to the FP64 decision.
+// FP32 difference of two fixed-point coordinates (exact integer difference,
+// one rounding).  Real pairs never overflow (|values| <= kFixedClamp); the
+// unsigned form keeps the padding sentinel's wrap-around defined.
+__device__ __forceinline__ float fixed_diff(int32_t a, int32_t b) {
+  return static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(a) - static_cast<uint32_t>(b)));
+}
+
+// Squared distance in units^2.
+__device__ __forceinline__ float pair_s(const RecAtom& r, int32_t x, int32_t y, int32_t z) {
+  const float dx = fixed_diff(r.x, x), dy = fixed_diff(r.y, y), dz = fixed_diff(r.z, z);
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// The FP32 cutoff decisions on s (units^2) against the complex's guard bands,
+// kept as integers: for a non-negative float, its bit pattern orders like its
+// value.  c_T = bits(lo_T) + 1, so bits(s) - c_T borrows exactly when s <=
+// lo_T (the FP32 decision), and the same difference, as an unsigned number
+// below w, puts s inside the band of T.  The four bands share one width.
 struct Bands {
   float lo1, lo64, lo144, lo225;
-  uint32_t c1, c64, c144, c225, w1, w;
+  uint32_t c1, c64, c144, c225, w;
 };
 
-__device__ __forceinline__ Bands make_bands(double X) {
+__device__ __forceinline__ Bands make_bands(CxRef cx) {
   Bands B;
-  float h1, h64, h144, h225;
-  guard_band(1.0, X, B.lo1, h1);
-  guard_band(64.0, X, B.lo64, h64);
-  guard_band(144.0, X, B.lo144, h144);
-  guard_band(225.0, X, B.lo225, h225);
-  B.c1 = __float_as_uint(B.lo1) + 1u;
-  B.c64 = __float_as_uint(B.lo64) + 1u;
-  B.c144 = __float_as_uint(B.lo144) + 1u;
-  B.c225 = __float_as_uint(B.lo225) + 1u;
-  B.w1 = __float_as_uint(h1) - B.c1;
-  B.w = max(__float_as_uint(h64) - B.c64, max(__float_as_uint(h144) - B.c144, __float_as_uint(h225) - B.c225));
+  B.c1 = cx.band_c[0]; B.c64 = cx.band_c[1]; B.c144 = cx.band_c[2]; B.c225 = cx.band_c[3];
+  B.w = cx.band_w;
+  B.lo1 = __uint_as_float(B.c1 - 1u); B.lo64 = __uint_as_float(B.c64 - 1u);
+  B.lo144 = __uint_as_float(B.c144 - 1u); B.lo225 = __uint_as_float(B.c225 - 1u);
   return B;
 }
 
@@ -153,31 +150,30 @@ __device__ __forceinline__ int lig_index(int base, int k) {
          static_cast<int>(threadIdx.x & 63);
 }
 
-// Axis-aligned boxes of a wave's K ligand blocks (FP32, origin frame; empty
+// Axis-aligned boxes of a wave's K ligand blocks (FP32 A, origin frame; empty
 // when a block holds no atom), kept in LDS as 8 floats per block, and the
-// test that a receptor group's box is farther from a block's box than every
-// cutoff.
+// test that a receptor atom is farther from a block's box than every cutoff.
 constexpr int kBoxFloats = 8;
 
-__device__ __forceinline__ float wave_min(float v) {
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-  return uniform(v);
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
-__device__ __forceinline__ float wave_max(float v) {
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-  return uniform(v);
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // The distance from a point to any point of a box is at least the
-// point-to-box distance; FP32 rounding of it is far below the 0.01 A margin
-// of kCullReach over the 15 A cutoff.
-__device__ __forceinline__ bool beyond_reach(const float4 r, const float* __restrict__ b) {
-  const float dx = fmaxf(fmaxf(b[0] - r.x, r.x - b[3]), 0.0f);
-  const float dy = fmaxf(fmaxf(b[1] - r.y, r.y - b[4]), 0.0f);
-  const float dz = fmaxf(fmaxf(b[2] - r.z, r.z - b[5]), 0.0f);
+// point-to-box distance; FP32 rounding of it (and of the fixed-point to A
+// conversions) is far below the 0.01 A margin of kCullReach over 15 A.
+__device__ __forceinline__ bool beyond_reach(float rx, float ry, float rz, const float* __restrict__ b) {
+  const float dx = fmaxf(fmaxf(b[0] - rx, rx - b[3]), 0.0f);
+  const float dy = fmaxf(fmaxf(b[1] - ry, ry - b[4]), 0.0f);
+  const float dz = fmaxf(fmaxf(b[2] - rz, rz - b[5]), 0.0f);
   return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
 }
 
@@ -185,60 +181,75 @@ __device__ __forceinline__ bool beyond_reach(const float4 r, const float* __rest
 // t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
 // slot against every block; padding slots sit far from every block).
 template <int K>
-__device__ __forceinline__ void atoms_in_reach(CxRef cx, int g0, int g1, const float* __restrict__ boxes,
+__device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int g1, const float* __restrict__ boxes,
                                                uint64_t (&keep)[K]) {
   const int lane = threadIdx.x & 63;
   const bool valid = g0 + lane < g1;
-  const float4 r = cx.rec_xyzq[valid ? g0 + lane : g0];
+  const int4 r = cx.rec_fixed[valid ? g0 + lane : g0];
+  const float rx = static_cast<float>(r.x) * unit, ry = static_cast<float>(r.y) * unit,
+              rz = static_cast<float>(r.z) * unit;
 #pragma unroll
-  for (int k = 0; k < K; ++k) keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(r, boxes + kBoxFloats * k));
+  for (int k = 0; k < K; ++k)
+    keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(rx, ry, rz, boxes + kBoxFloats * k));
 }
 
 template <int K>
 struct LaneAtoms {
-  // acv: sum of rho (rho - 2) over the current receptor class within the
-  // tile (times eps when folded into FP64); ace: sum of q_r / d^2.
-  float x[K], y[K], z[K], eps[K], ir6[K], acv[K], ace[K];
+  // x, y, z: fixed point.  acv: sum of rho (rho - 2) over the current
+  // receptor class within the tile (times eps when folded into FP64); ace:
+  // sum of q_r / d^2 over the current 16 receptor slots.
+  int32_t x[K], y[K], z[K];
+  float eps[K], ir2[K], acv[K], ace[K];
   int32_t cnt[K];
 };
 
-__device__ __forceinline__ float pair_d2(float rx, float ry, float rz, float lx, float ly, float lz) {
-  const float dx = rx - lx, dy = ry - ly, dz = rz - lz;
-  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+// Soft-core LJ (fitness.rs:57-70) and the Coulomb factor from s, the squared
+// distance in units^2 (s = d^2 2^48 at the finest unit).  The pair's vdw term
+// is eps (rho^2 - 2 rho), rho = R^6 / (d^6 + R^6/2) = 1 / (t^3 + 1/2), t =
+// d^2/R^2.  rho and 1/s come from one reciprocal: with t' = t 2^-16 = s ir2
+// (ir2: the class factor q^2 / R^2 2^-16 of the lane's ligand atom) and den'
+// = t'^3 + 2^-49 = (t^3 + 1/2) 2^-48, rp = 1 / (den' s) = 1 / ((t^3 + 1/2) d^2)
+// stays a normal float for every pair within the cutoffs (a product at the
+// unit's scale, about 2^48 larger, would reach FP32's denormals for tiny R);
+// then rho' = rp s = rho 2^48 and y = rp den' = 1/s.  The vdw sums carry
+// rho' (u = rho' 2^-48 - 2, the term eps u rho' 2^-48 when folded into FP64).
+// 1/s is refined by one Newton step (e = 1 - s y, y + y e, two FMAs):
+// v_rcp_f32 is within 1 ulp but biased (-0.07 ulp on average), and over
+// ~10^6 Coulomb terms of one sign a bias adds up where rounding errors
+// cancel; after the step the reciprocal is correctly rounded but for rare
+// half-ulp cases.  s = 0 gives rho' = 0, a zero vdw term (the reference's
+// term is 0 there too) and a finite 1/s the elec mask never selects.
+constexpr float kRhoScale = 0x1p-48f;  // rho' to rho
+__device__ __forceinline__ void pair_terms(float s, float ir2, float& rho, float& u, float& is) {
+  const float t = s * ir2;
+  const float den = __builtin_fmaf(t * t, t, 0x1p-49f);
+  const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, s, 1.0e-30f));
+  rho = rp * s;
+  const float y = rp * den;
+  is = __builtin_fmaf(y, __builtin_fmaf(-s, y, 1.0f), y);
+  u = __builtin_fmaf(rho, kRhoScale, -2.0f);
 }
 
-// Soft-core LJ (fitness.rs:57-70): the pair's vdw term is eps (rho^2 - 2 rho)
-// = eps u rho with u = rho - 2, rho = R^6 / (d^6 + R^6/2); rho and 1/d^2
-// come from one reciprocal, 1 / (d^2 (d^6/R^6 + 1/2)).  The pair loop sums
-// u rho over the receptor atoms of one (rmin2, eps) class and multiplies by
-// eps once per class and tile.
-__device__ __forceinline__ void pair_terms(float d2, float ir6, float& rho, float& u, float& id2) {
-  const float d6 = (d2 * d2) * d2;
-  const float den = __builtin_fmaf(d6, ir6, 0.5f);
-  const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, d2, 1.0e-30f));
-  rho = rp * d2;
-  id2 = rp * den;
-  u = rho - 2.0f;
-}
-
-// The vdw term of a pair as the pair loop counts it: eps (u rho), in FP64
-// from the FP32 factors.
+// The vdw term of a pair as the pair loop counts it: eps (u rho') 2^-48, in
+// FP64 from the FP32 factors.
 __device__ __forceinline__ double vdw_term(float eps, float u, float rho) {
-  return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho));
+  return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho)) *
+         static_cast<double>(kRhoScale);
 }
 
-// d2 inside one of the four guard bands (the same test as the pair loop's).
-__device__ __forceinline__ bool in_band(float d2, const Bands& B) {
-  const uint32_t b = __float_as_uint(d2);
-  return b - B.c1 < B.w1 || min(b - B.c64, min(b - B.c144, b - B.c225)) < B.w;
+// s (units^2) inside one of the four guard bands (the pair loop's test).
+__device__ __forceinline__ bool in_band(float s, const Bands& B) {
+  const uint32_t b = __float_as_uint(s);
+  return min(min(b - B.c1, b - B.c64), min(b - B.c144, b - B.c225)) < B.w;
 }
 
 // Per-lane Lennard-Jones factors of ligand atom l against receptor class c:
-// 1/R^6 and sqrt(eps_r eps_l) from FP64 (fitness.rs:58-63), rounded once,
-// tabulated per complex (one coalesced 8-byte load per lane).
-__device__ __forceinline__ void class_factors(CxRef cx, int c, int l, float& ir6, float& eps) {
+// q^2 / R^2 2^-16 (q: the fixed-point unit, R = rmin2_r + rmin2_l) and
+// sqrt(eps_r eps_l) from FP64 (fitness.rs:58-63), rounded once, tabulated
+// per complex (one coalesced 8-byte load per lane).
+__device__ __forceinline__ void class_factors(CxRef cx, int c, int l, float& ir2, float& eps) {
   const float2 f = cx.lig_class[static_cast<size_t>(c) * cx.n_lig + l];
-  ir6 = f.x;
+  ir2 = f.x;
   eps = f.y;
 }
 
@@ -261,7 +272,7 @@ __device__ __forceinline__ void exact_flags(CxRef cx, const PoseCoef& pc, const 
 }
 
 struct SettleEvent { int32_t j, pad; uint64_t mask; };  // first receptor slot of a group, lanes
-constexpr int kEventsPerWave = 128;
+constexpr int kEventsPerWave = 64;
 
 // Fast path of one receptor atom against the lane's ligand atom in slot k;
 // returns this wave's burial count of the receptor atom from that slot and
@@ -269,11 +280,11 @@ constexpr int kEventsPerWave = 128;
 template <int K>
 __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int k,
                                              uint64_t& ambm) {
-  const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
-  // FP32 decisions below each band, as the borrows of bits(d2) - c_T (pairs
+  const float s = pair_s(r, a.x[k], a.y[k], a.z[k]);
+  // FP32 decisions below each band, as the borrows of bits(s) - c_T (pairs
   // of equal serial are included in the vdw sum and taken out afterwards, see
   // collision_correction), and the band test on the same differences.
-  const uint32_t bd = __float_as_uint(d2);
+  const uint32_t bd = __float_as_uint(s);
   uint32_t q1, q64, q144, q225;
   const bool le1 = __builtin_sub_overflow(bd, B.c1, &q1);
   // d <= 8 A as a lane mask (the borrow): it is the carry-in that counts the
@@ -289,13 +300,12 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
       : "v"(bd), "s"(B.c64));
   const bool mv = __builtin_sub_overflow(bd, B.c144, &q144);
   const bool me0 = __builtin_sub_overflow(bd, B.c225, &q225);
-  // The two band tests as lane masks straight from the compares (predicate
-  // 36: unsigned less-than), joined by a scalar OR; a ballot of the combined
-  // bool would first widen it to a 0/1 value.
-  ambm |= __builtin_amdgcn_uicmp(q1, B.w1, 36) | __builtin_amdgcn_uicmp(min(q64, min(q144, q225)), B.w, 36);
+  // The band test as a lane mask straight from the compare (predicate 36:
+  // unsigned less-than); a ballot of a bool would first widen it to 0/1.
+  ambm |= __builtin_amdgcn_uicmp(min(min(q1, q64), min(q144, q225)), B.w, 36);
   const bool me = me0 && !le1;
   float rho, u, id2;
-  pair_terms(d2, a.ir6[k], rho, u, id2);
+  pair_terms(s, a.ir2[k], rho, u, id2);
   a.acv[k] = __builtin_fmaf(mv ? u : 0.0f, rho, a.acv[k]);
   a.ace[k] = __builtin_fmaf(me ? id2 : 0.0f, r.w, a.ace[k]);
   const int32_t cj = __popcll(md);
@@ -306,21 +316,29 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
   return cj;
 }
 
+// The fast path's FP32 decisions of a pair from its s: d <= 8, within the
+// vdw cutoff (d <= 12), within the elec cutoffs (1 < d <= 15).
+__device__ __forceinline__ void fast_flags(float s, const Bands& B, bool& f64, bool& f144, bool& f225) {
+  f64 = s <= B.lo64;
+  f144 = s <= B.lo144;
+  f225 = s <= B.lo225 && !(s <= B.lo1);
+}
+
 // Band pairs of one receptor atom: recorded as events (fast kernel) or
 // settled on the spot in FP64 (exact kernel).
 template <int K, bool EXACT>
 __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const double* __restrict__ coords,
                                            int base, int j, const RecAtom r, const Bands& B,
-                                           LaneAtoms<K>& a, SettleEvent* __restrict__ ev, int& nev, int& overflow,
-                                           int32_t* __restrict__ s_rcnt) {
+                                           LaneAtoms<K>& a, double& sum_e, SettleEvent* __restrict__ ev, int& nev,
+                                           int& overflow, int32_t* __restrict__ s_rcnt, int r0) {
   const int lane = threadIdx.x & 63;
   const int2 m = cx.rec_meta[j];
   const int rser = m.y ? m.x : INT_MIN;  // INT_MIN: no ligand atom shares this receptor atom's serial
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const float d2 = pair_d2(r.x, r.y, r.z, a.x[k], a.y[k], a.z[k]);
+    const float s = pair_s(r, a.x[k], a.y[k], a.z[k]);
     const int l = lig_index<K>(base, k);
-    const uint64_t am = __builtin_amdgcn_ballot_w64(in_band(d2, B) && l < cx.n_lig);
+    const uint64_t am = __builtin_amdgcn_ballot_w64(in_band(s, B) && l < cx.n_lig);
     if (am == 0ull) continue;
     if (!EXACT) {  // settled once per ligand chunk
       if (nev < kEventsPerWave) {
@@ -332,18 +350,19 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
       continue;
     }
     if ((am >> lane) & 1ull) {
-      bool x64, x144, x225;
+      bool x64, x144, x225, f64, f144, f225;
       exact_flags(cx, pc, coords, l, j, rser, x64, x144, x225);
-      const bool f144 = d2 <= B.lo144;
-      const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
+      fast_flags(s, B, f64, f144, f225);
       float rho, u, id2;
-      pair_terms(d2, a.ir6[k], rho, u, id2);
+      pair_terms(s, a.ir2[k], rho, u, id2);
       if (x144 != f144) a.acv[k] = __builtin_fmaf(x144 ? u : -u, rho, a.acv[k]);
-      if (x225 != f225) a.ace[k] = __builtin_fmaf(x225 ? id2 : -id2, r.w, a.ace[k]);
+      // the elec difference goes straight to FP64: the lane's FP32 partial
+      // may already have been folded
+      if (x225 != f225) sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * cx.lig_kq[l];
       const int dc = static_cast<int>(x64) - static_cast<int>(f64);
       if (dc) {
         a.cnt[k] += dc;
-        atomicAdd(&s_rcnt[j], dc);
+        atomicAdd(&s_rcnt[j - r0], dc);
       }
     }
   }
@@ -353,19 +372,21 @@ __device__ __forceinline__ void band_pairs(CxRef cx, const PoseCoef& pc, const d
 // receptor atoms and the lanes that saw a band pair in it; each band pair is
 // decided in FP64 and the difference to the FP32 decisions is added to the
 // FP64 sums, the lane's burial count and the receptor atom's count.  The
-// ligand slot is a loop variable here, so its FP32 coordinates are rebuilt
-// from the FP64 transform (the same operations that filled the lane's
-// registers) and its burial-count changes are carried as 16-bit fields of
-// two packed integers: the lane's register arrays are only ever indexed by
-// constants and stay in registers.
+// ligand slot is a loop variable here, so its fixed-point coordinates are
+// rebuilt from the FP64 transform (the same operations that filled the
+// lane's registers) and its burial-count changes are carried as 16-bit fields
+// of two packed integers: the lane's register arrays are only ever indexed by
+// constants and stay in registers.  s_rcnt is indexed from the part's first
+// receptor slot r0.
 template <int K>
 __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict__ pose_rec,
-                                              const double* __restrict__ coords, int base, const Bands& B,
+                                              const double* __restrict__ coords, int base, int r0, const Bands& B,
                                               const SettleEvent* __restrict__ ev, int nev, LaneAtoms<K>& a,
                                               double& sum_v, double& sum_e, int32_t* __restrict__ s_rcnt) {
   const int lane = threadIdx.x & 63;
   const int32_t GDOCK_AS4* cstart = (const int32_t GDOCK_AS4*)cx.class_start;
-  const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
+  const int32_t GDOCK_AS4* rf = (const int32_t GDOCK_AS4*)cx.rec_fixed;
+  const double scale = ldexp(1.0, cx.qexp);
   PoseCoef pc{};
   if (!coords) pc = load_pose(pose_rec);
   int64_t dpack0 = 0, dpack1 = 0;  // slot k's burial-count change in bits [16 (k % 4), +16)
@@ -380,35 +401,33 @@ __device__ __forceinline__ void settle_events(CxRef cx, const double* __restrict
       if (l >= cx.n_lig) break;
       double px, py, pz;
       ligand_atom(cx, pc, coords, l, px, py, pz);
-      const float lx = static_cast<float>(px - cx.origin[0]);
-      const float ly = static_cast<float>(py - cx.origin[1]);
-      const float lz = static_cast<float>(pz - cx.origin[2]);
+      const int32_t lx = to_fixed(px, cx.origin[0], scale), ly = to_fixed(py, cx.origin[1], scale),
+                    lz = to_fixed(pz, cx.origin[2], scale);
       bool have_factors = false;
-      float ir6 = 0.0f, eps = 0.0f;
+      float ir2 = 0.0f, eps = 0.0f;
       for (int j = E.j; j < E.j + kRecGroup; ++j) {
-        const RecAtom r = rec_atom(rq, j);
-        const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
-        if (!in_band(d2, B)) continue;
+        const RecAtom r = rec_atom(rf, j);
+        if (r.x == kFixedPad) continue;  // padding slot: never evaluated
+        const float s = pair_s(r, lx, ly, lz);
+        if (!in_band(s, B)) continue;
         const int2 m = cx.rec_meta[j];
         const int rser = m.y ? m.x : INT_MIN;
         if (!have_factors) {
-          class_factors(cx, c, l, ir6, eps);
+          class_factors(cx, c, l, ir2, eps);
           have_factors = true;
         }
-        bool x64, x144, x225;
+        bool x64, x144, x225, f64, f144, f225;
         exact_flags_at(cx, px, py, pz, l, j, rser, x64, x144, x225);
-        const bool f144 = d2 <= B.lo144;
-        const bool f225 = d2 <= B.lo225 && !(d2 <= B.lo1), f64 = d2 <= B.lo64;
+        fast_flags(s, B, f64, f144, f225);
         float rho, u, id2;
-        pair_terms(d2, ir6, rho, u, id2);
+        pair_terms(s, ir2, rho, u, id2);
         if (x144 != f144) sum_v += x144 ? vdw_term(eps, u, rho) : -vdw_term(eps, u, rho);
-        if (x225 != f225)
-          sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * (kCoulomb * cx.lig_q[l]);
+        if (x225 != f225) sum_e += static_cast<double>(x225 ? id2 : -id2) * static_cast<double>(r.w) * cx.lig_kq[l];
         const int dc = static_cast<int>(x64) - static_cast<int>(f64);
         if (dc) {
           const int64_t d = static_cast<int64_t>(dc) << (16 * (k & 3));
           if (k < 4) dpack0 += d; else dpack1 += d;
-          atomicAdd(&s_rcnt[j], dc);
+          atomicAdd(&s_rcnt[j - r0], dc);
         }
       }
     }
@@ -476,36 +495,34 @@ __device__ __forceinline__ double ligand_desolv(CxRef cx, int l, int32_t cnt) {
   return asp * burial * rad * rad * 4.0 * kPI;
 }
 
-// Scores one pose with the whole workgroup.  pose_rec: the pose record
-// (kPoseDoubles doubles) or null; coords: this pose's ligand coordinates (n_lig x 3) when
-// scoring coordinates as given, with `extent` = their largest |x - origin|.
-// o: 6 outputs; *redo is set when the fast variant could not settle every
-// band pair (the exact variant then rescores the pose).
+// Scores one pose (or the receptor range pp of one pose) with the whole
+// workgroup.  pose_rec: the pose record (kPoseDoubles doubles) or null;
+// coords: this pose's ligand coordinates (n_lig x 3) when scoring
+// coordinates as given.  o: 6 outputs; *redo is set when the fast variant
+// could not settle every band pair (the exact variant then rescores the
+// pose).  The LDS receptor count table covers the part's slots [r0, r1).
 template <int K, bool EXACT>
 __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ pose_rec,
-                                           const double* __restrict__ coords, double extent, const Weights& w,
+                                           const double* __restrict__ coords, const Weights& w,
                                            double* __restrict__ o, int32_t* __restrict__ nsat_out,
                                            int32_t* __restrict__ redo, int32_t* __restrict__ s_rcnt,
                                            int32_t* s_overflow, const PosePart& pp) {
-  double* s_red = reinterpret_cast<double*>(s_rcnt + ((cx.n_rec + 1) & ~1));
+  const int range = pp.r1 - pp.r0;
+  double* s_red = reinterpret_cast<double*>(s_rcnt + ((range + 1) & ~1));
   SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
   float* s_box = reinterpret_cast<float*>(s_ev + (kScoreThreads / 64) * kEventsPerWave);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SettleEvent* my_ev = s_ev + wave * kEventsPerWave;
-  const float GDOCK_AS4* rq = (const float GDOCK_AS4*)cx.rec_xyzq;
+  const int32_t GDOCK_AS4* rf = (const int32_t GDOCK_AS4*)cx.rec_fixed;
   const int32_t GDOCK_AS4* cstart = (const int32_t GDOCK_AS4*)cx.class_start;
 
   PoseCoef pc{};
-  double X;
-  if (coords) {
-    X = fmax(cx.rec_extent, extent);
-  } else {
-    pc = load_pose(pose_rec);
-    X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
-  }
-  const Bands B = make_bands(X);
+  if (!coords) pc = load_pose(pose_rec);
+  const Bands B = make_bands(cx);
+  const double scale = ldexp(1.0, cx.qexp);
+  const float unit = ldexpf(1.0f, -cx.qexp);
 
-  for (int i = pp.r0 + tid; i < pp.r1; i += kScoreThreads) s_rcnt[i] = 0;
+  for (int i = tid; i < range; i += kScoreThreads) s_rcnt[i] = 0;
   if (tid == 0) *s_overflow = 0;
   __syncthreads();
 
@@ -519,25 +536,31 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       if (l < cx.n_lig) {
         double px, py, pz;
         ligand_atom(cx, pc, coords, l, px, py, pz);
-        a.x[k] = static_cast<float>(px - cx.origin[0]);
-        a.y[k] = static_cast<float>(py - cx.origin[1]);
-        a.z[k] = static_cast<float>(pz - cx.origin[2]);
+        a.x[k] = to_fixed(px, cx.origin[0], scale);
+        a.y[k] = to_fixed(py, cx.origin[1], scale);
+        a.z[k] = to_fixed(pz, cx.origin[2], scale);
       } else {
-        a.x[k] = a.y[k] = a.z[k] = kPadCoord;
+        a.x[k] = a.y[k] = a.z[k] = kFixedClamp;
       }
-      a.eps[k] = 0.0f; a.ir6[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
+      a.eps[k] = 0.0f; a.ir2[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
     }
     float* my_box = s_box + wave * (kMaxAtomsPerLane * kBoxFloats);
     __syncthreads();  // the previous chunk's boxes are no longer read
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const bool in = lig_index<K>(base, k) < cx.n_lig;
-      const float x0 = wave_min(in ? a.x[k] : INFINITY), y0 = wave_min(in ? a.y[k] : INFINITY),
-                  z0 = wave_min(in ? a.z[k] : INFINITY), x1 = wave_max(in ? a.x[k] : -INFINITY),
-                  y1 = wave_max(in ? a.y[k] : -INFINITY), z1 = wave_max(in ? a.z[k] : -INFINITY);
+      const int32_t x0 = wave_min(in ? a.x[k] : INT_MAX), y0 = wave_min(in ? a.y[k] : INT_MAX),
+                    z0 = wave_min(in ? a.z[k] : INT_MAX), x1 = wave_max(in ? a.x[k] : INT_MIN),
+                    y1 = wave_max(in ? a.y[k] : INT_MIN), z1 = wave_max(in ? a.z[k] : INT_MIN);
       if (lane == 0) {
         float* b = my_box + kBoxFloats * k;
-        b[0] = x0; b[1] = y0; b[2] = z0; b[3] = x1; b[4] = y1; b[5] = z1;
+        const bool empty = x0 > x1;
+        b[0] = empty ? INFINITY : static_cast<float>(x0) * unit;
+        b[1] = empty ? INFINITY : static_cast<float>(y0) * unit;
+        b[2] = empty ? INFINITY : static_cast<float>(z0) * unit;
+        b[3] = empty ? -INFINITY : static_cast<float>(x1) * unit;
+        b[4] = empty ? -INFINITY : static_cast<float>(y1) * unit;
+        b[5] = empty ? -INFINITY : static_cast<float>(z1) * unit;
       }
     }
     __syncthreads();
@@ -549,7 +572,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       // adds nothing and holds no band pair with it: that block skips the
       // atom (a group when no block is near, the tile when nothing is).
       uint64_t keep[K];
-      atoms_in_reach<K>(cx, g0, g1, my_box, keep);
+      atoms_in_reach<K>(cx, unit, g0, g1, my_box, keep);
       uint64_t any_keep = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) any_keep |= keep[k];
@@ -562,24 +585,32 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
+            sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]) * static_cast<double>(kRhoScale);
             a.acv[k] = 0.0f;
             const int l = lig_index<K>(base, k);
-            if (l < cx.n_lig) class_factors(cx, cls, l, a.ir6[k], a.eps[k]);
+            if (l < cx.n_lig) class_factors(cx, cls, l, a.ir2[k], a.eps[k]);
           }
         }
-        const RecAtom r0 = rec_atom(rq, j), r1 = rec_atom(rq, j + 1), r2 = rec_atom(rq, j + 2), r3 = rec_atom(rq, j + 3);
+        const RecAtom r0 = rec_atom(rf, j), r1 = rec_atom(rf, j + 1), r2 = rec_atom(rf, j + 2),
+                      r3 = rec_atom(rf, j + 3);
         uint64_t ambm = 0;  // lanes that saw a band pair in this group
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {  // wave-uniform branches
           const uint32_t mk = static_cast<uint32_t>(keep[k] >> sh);
+          if ((mk & 15u) == 0u) continue;
           if (mk & 1u) c0 += pair_fast<K>(r0, B, a, k, ambm);
           if (mk & 2u) c1 += pair_fast<K>(r1, B, a, k, ambm);
           if (mk & 4u) c2 += pair_fast<K>(r2, B, a, k, ambm);
           if (mk & 8u) c3 += pair_fast<K>(r3, B, a, k, ambm);
+          // Coulomb: the group's FP32 partial (at most 4 terms) goes to FP64
+          // at once; a longer FP32 partial carries a close pair's large term
+          // and rounds every later term at its scale.
+          const int l = lig_index<K>(base, k);
+          sum_e += static_cast<double>(a.ace[k]) * cx.lig_kq[l];  // zero-padded past n_lig
+          a.ace[k] = 0.0f;
         }
-        const int t = lane - (j - g0);
+        const int t = lane - sh;
         rc = t == 0 ? c0 : rc;
         rc = t == 1 ? c1 : rc;
         rc = t == 2 ? c2 : rc;
@@ -587,10 +618,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         const uint64_t amask = ambm;
         if (amask != 0ull) {  // rare: this group holds band pairs
           if (EXACT) {
-            band_pairs<K, EXACT>(cx, pc, coords, base, j, r0, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, EXACT>(cx, pc, coords, base, j + 1, r1, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, EXACT>(cx, pc, coords, base, j + 2, r2, B, a, my_ev, nev, overflow, s_rcnt);
-            band_pairs<K, EXACT>(cx, pc, coords, base, j + 3, r3, B, a, my_ev, nev, overflow, s_rcnt);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j, r0, B, a, sum_e, my_ev, nev, overflow, s_rcnt, pp.r0);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 1, r1, B, a, sum_e, my_ev, nev, overflow, s_rcnt, pp.r0);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 2, r2, B, a, sum_e, my_ev, nev, overflow, s_rcnt, pp.r0);
+            band_pairs<K, EXACT>(cx, pc, coords, base, j + 3, r3, B, a, sum_e, my_ev, nev, overflow, s_rcnt, pp.r0);
           } else if (nev < kEventsPerWave) {  // settled once per ligand chunk
             if (lane == 0) my_ev[nev] = SettleEvent{j, 0, amask};
             ++nev;
@@ -599,19 +630,16 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           }
         }
       }
-      if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane], rc);
+      if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane - pp.r0], rc);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {  // fold the tile's FP32 partial sums into FP64
-        const int l = lig_index<K>(base, k);
-        if (l < cx.n_lig) {
-          sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
-          sum_e += static_cast<double>(a.ace[k]) * (kCoulomb * cx.lig_q[l]);
-        }
-        a.acv[k] = 0.0f; a.ace[k] = 0.0f;
+      for (int k = 0; k < K; ++k) {  // fold the tile's FP32 LJ sums into FP64
+        sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]) * static_cast<double>(kRhoScale);
+        a.acv[k] = 0.0f;
       }
     }
     if (!EXACT && nev > 0)
-      settle_events<K>(cx, pose_rec, coords, base, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e, s_rcnt);
+      settle_events<K>(cx, pose_rec, coords, base, pp.r0, B, my_ev, min(nev, kEventsPerWave), a, sum_v, sum_e,
+                       s_rcnt);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int l = lig_index<K>(base, k);
@@ -644,20 +672,18 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     if (P.x < pp.r0 || P.x >= pp.r1) continue;  // another part's receptor slot
     double px, py, pz;
     ligand_atom(cx, pc, coords, P.y, px, py, pz);
-    const float lx = static_cast<float>(px - cx.origin[0]);
-    const float ly = static_cast<float>(py - cx.origin[1]);
-    const float lz = static_cast<float>(pz - cx.origin[2]);
-    const float4 r = cx.rec_xyzq[P.x];
-    const float d2 = pair_d2(r.x, r.y, r.z, lx, ly, lz);
-    if (d2 <= B.lo144 && !in_band(d2, B)) {
-      float ir6, eps, rho, u, id2;
-      class_factors(cx, P.z, P.y, ir6, eps);
-      pair_terms(d2, ir6, rho, u, id2);
+    const RecAtom r = rec_atom(rf, P.x);
+    const float s = pair_s(r, to_fixed(px, cx.origin[0], scale), to_fixed(py, cx.origin[1], scale),
+                           to_fixed(pz, cx.origin[2], scale));
+    if (s <= B.lo144 && !in_band(s, B)) {
+      float ir2, eps, rho, u, id2;
+      class_factors(cx, P.z, P.y, ir2, eps);
+      pair_terms(s, ir2, rho, u, id2);
       sum_v -= vdw_term(eps, u, rho);
     }
   }
   for (int j = pp.r0 + tid; j < pp.r1; j += kScoreThreads) {  // fitness.rs:141-162
-    const int32_t cnt = s_rcnt[j];
+    const int32_t cnt = s_rcnt[j - pp.r0];
     if (cnt == 0) continue;  // burial 0: the term is exactly zero
     const double asp = cx.rec_asp[j];
     if (fabs(asp) >= 0.001) {
@@ -723,7 +749,8 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           if (pp.redo_count) atomicAdd(pp.redo_count, 1);
         } else {
           finish_pose(cx, pc, coords, w, v, e, d + lig_d, o, nsat_out);
-          *redo = 0;
+          // the exact variant leaves the flag set: its other parts read it
+          if (!EXACT) *redo = 0;
         }
       }
     }
@@ -731,21 +758,39 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
   __syncthreads();  // LDS is reused by the workgroup's next pose
 }
 
+// Receptor slots [r0, r1) of part `part` of a pose split into `parts`: an
+// equal share of the 64-slot tiles.
+__device__ __forceinline__ int2 part_range(int n_rec, int part, int parts) {
+  const int tiles = (n_rec + 63) / 64;
+  const int t0 = part * tiles / parts, t1 = (part + 1) * tiles / parts;
+  return make_int2(64 * t0, min(64 * t1, n_rec));
+}
+
+// Plain path: pose blockIdx.x / parts, receptor part blockIdx.x % parts (one
+// part unless the receptor exceeds one workgroup's LDS table).  A split pose
+// keeps its shared state at split_* (cleared before the launch; the last part
+// clears it again).
 template <int K, bool EXACT>
 __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4, EXACT ? 8 : 4))) void score_poses_kernel(
     const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
-    const double* __restrict__ pose_extent, const Weights w, double* __restrict__ out,
-    int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo) {
-  const int pose = blockIdx.x;
+    const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo,
+    int parts, int32_t* __restrict__ split_lig, double* __restrict__ split_sums, int32_t* __restrict__ split_sync) {
+  const int pose = blockIdx.x / parts, part = blockIdx.x % parts;
   if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
   extern __shared__ int32_t s_lds[];
   __shared__ int32_t s_flags[2];
   CxRef cx = *(const DevComplex GDOCK_AS4*)d_cx;
-  const PosePart whole{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
+  const int2 rr = part_range(cx.n_rec, part, parts);
+  PosePart pp{rr.x, rr.y, part, parts, nullptr, nullptr, nullptr, nullptr};
+  if (parts > 1) {
+    pp.lig_cnt = split_lig + static_cast<size_t>(pose) * cx.n_lig;
+    pp.sums = split_sums + static_cast<size_t>(pose) * 3 * parts;
+    pp.sync = split_sync + static_cast<size_t>(pose) * 2;
+  }
   score_pose<K, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
-                             coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr,
-                             coords ? pose_extent[pose] : 0.0, w, out + static_cast<size_t>(kOutDoubles) * pose,
-                             nsat_out ? nsat_out + pose : nullptr, redo + pose, s_lds, s_flags, whole);
+                       coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr, w,
+                       out + static_cast<size_t>(kOutDoubles) * pose, nsat_out ? nsat_out + pose : nullptr,
+                       redo + pose, s_lds, s_flags, pp);
 }
 
 int score_atoms_per_lane(int n_lig) {
@@ -764,7 +809,7 @@ __device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ 
   switch (cx.lanes_k) {
 #define GDOCK_ITEM(KK)                                                                                   \
   case KK:                                                                                               \
-    score_pose<KK, EXACT>(cx, pose_rec, nullptr, 0.0, w, o, nsat, redo, s_lds, s_flags, pp);            \
+    score_pose<KK, EXACT>(cx, pose_rec, nullptr, w, o, nsat, redo, s_lds, s_flags, pp);                 \
     break;
     GDOCK_ITEM(1) GDOCK_ITEM(2) GDOCK_ITEM(3) GDOCK_ITEM(4)
     GDOCK_ITEM(5) GDOCK_ITEM(6) GDOCK_ITEM(7) GDOCK_ITEM(8)
@@ -776,12 +821,11 @@ __device__ __forceinline__ void score_item(CxRef cx, const double* __restrict__ 
 // Part `part` of a pose split into `parts` (1: the whole pose): an equal share
 // of the receptor tiles; the split state has room for cx.parts parts.
 __device__ __forceinline__ PosePart part_of(CxRef cx, int part, int parts, int pose_row) {
-  if (parts <= 1) return PosePart{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
-  const int tiles = (cx.n_rec + 63) / 64;
-  const int t0 = part * tiles / parts, t1 = (part + 1) * tiles / parts;
+  if (parts <= 1) return PosePart{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr, nullptr};
+  const int2 rr = part_range(cx.n_rec, part, parts);
   const size_t q = static_cast<size_t>(pose_row - cx.split_pose_base);
-  return PosePart{64 * t0, min(64 * t1, cx.n_rec), part, parts, cx.split_lig + q * cx.n_lig,
-                  cx.split_sums + q * 3 * cx.parts, cx.split_sync + q * 2};
+  return PosePart{rr.x, rr.y, part, parts, cx.split_lig + q * cx.n_lig, cx.split_sums + q * 3 * cx.parts,
+                  cx.split_sync + q * 2, nullptr};
 }
 
 __device__ __forceinline__ int item_slot(int2 I) { return I.x & kItemSlotMask; }
@@ -825,9 +869,9 @@ __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(4
   }
 }
 
-// Exact rescoring of the batch's poses whose settle buffer overflowed: the
-// whole pose in one workgroup (items of part 0; the other parts' items and
-// every clean pose return at once).
+// Exact rescoring of the batch's poses whose settle buffer overflowed: every
+// item (part) of a flagged pose runs again, deciding band pairs inline; the
+// items of clean poses return at once.
 __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
     const DevComplex* __restrict__ table, const int2* __restrict__ items, const int32_t* __restrict__ n_items_p,
     const double* __restrict__ poses, const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out,
@@ -838,41 +882,77 @@ __global__ __launch_bounds__(kScoreThreads) void score_batch_exact_kernel(
   const int n_items = *n_items_p;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int2 I = items[it];
-    if (item_part(I) != 0 || redo[I.y] == 0) continue;  // uniform per workgroup
+    if (redo[I.y] == 0) continue;  // uniform per workgroup
     CxRef cx = ((const DevComplex GDOCK_AS4*)table)[item_slot(I)];
-    const PosePart whole{0, cx.n_rec, 0, 1, nullptr, nullptr, nullptr};
+    const PosePart pp = part_of(cx, item_part(I), item_parts(I), I.y);
     score_item<true>(cx, poses + static_cast<size_t>(kPoseDoubles) * I.y, w,
-                     out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, whole);
+                     out + static_cast<size_t>(kOutDoubles) * I.y, nsat_out + I.y, redo + I.y, s_lds, s_flags, pp);
   }
 }
 
-size_t score_lds_bytes(const DevComplex& cx) {
-  return static_cast<size_t>((cx.n_rec + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
+// Dynamic LDS of a scoring workgroup covering at most `slots` receptor slots:
+// the burial-count table, the cross-wave reduction, the settle events and the
+// ligand block boxes.
+size_t score_lds_bytes(int slots) {
+  return static_cast<size_t>((slots + 1) & ~1) * sizeof(int32_t) + 4 * (kScoreThreads / 64) * sizeof(double) +
          (kScoreThreads / 64) * kEventsPerWave * sizeof(SettleEvent) +
          (kScoreThreads / 64) * kMaxAtomsPerLane * kBoxFloats * sizeof(float);
 }
 
+// Most receptor slots one part covers when a receptor of n_rec slots is
+// split into `parts` (whole tiles).
+int part_slots(int n_rec, int parts) {
+  const int tiles = (n_rec + 63) / 64;
+  return std::min(n_rec, 64 * ((tiles + parts - 1) / parts));
+}
+
+// Parts of the plain path: one workgroup per pose while the receptor fits one
+// workgroup's LDS table, else the fewest equal receptor ranges that fit.
+int plain_parts(const DevComplex& cx) {
+  const int tiles = (cx.n_rec + 63) / 64, per = kMaxLdsReceptor / 64;
+  return std::max(1, (tiles + per - 1) / per);
+}
+
+size_t plain_scratch_bytes(const DevComplex& cx, int n_poses) {
+  const size_t flags = (static_cast<size_t>(std::max(n_poses, 1)) * sizeof(int32_t) + 255) / 256 * 256;
+  return flags + split_state_bytes(cx, plain_parts(cx), n_poses);
+}
+
 template <int K>
-static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, const double* d_extent,
-                        int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, size_t lds,
-                        hipStream_t stream) {
-  hipLaunchKernelGGL((score_poses_kernel<K, false>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
-                     d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
-  hipLaunchKernelGGL((score_poses_kernel<K, true>), dim3(n_poses), dim3(kScoreThreads), lds, stream, d_cx,
-                     d_poses, d_coords, d_extent, w, d_out, d_nsat, d_redo);
+static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, int n_poses,
+                        const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, int parts,
+                        const DevComplex& split, size_t lds, hipStream_t stream) {
+  const dim3 grid(static_cast<unsigned>(n_poses) * parts);
+  hipLaunchKernelGGL((score_poses_kernel<K, false>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync);
+  hipLaunchKernelGGL((score_poses_kernel<K, true>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync);
 }
 
 // Fast kernel over all poses, then the exact kernel (a no-op for every pose
-// that did not overflow its settle buffer).  d_redo: n_poses int32 scratch.
+// that did not overflow its settle buffer).  scratch: plain_scratch_bytes
+// of device memory (per-pose flags, then the split state of a receptor
+// larger than one workgroup's LDS table, cleared here).
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
-                        const double* d_extent, int n_poses, const Weights& w, double* d_out, int32_t* d_nsat,
-                        int32_t* d_redo, hipStream_t stream, int lanes_k) {
+                        int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
+                        hipStream_t stream, int lanes_k) {
   if (n_poses <= 0) return hipSuccess;
-  const size_t lds = score_lds_bytes(cx);
+  const int parts = plain_parts(cx);
+  const size_t lds = score_lds_bytes(part_slots(cx.n_rec, parts));
+  char* m = static_cast<char*>(scratch);
+  int32_t* d_redo = reinterpret_cast<int32_t*>(m);
+  DevComplex split{};
+  split.n_lig = cx.n_lig;
+  if (parts > 1) {
+    const size_t flags = (static_cast<size_t>(n_poses) * sizeof(int32_t) + 255) / 256 * 256;
+    bind_split_state(&split, parts, n_poses, 0, m + flags);
+    const hipError_t e = hipMemsetAsync(m + flags, 0, split_state_bytes(cx, parts, n_poses), stream);
+    if (e != hipSuccess) return e;
+  }
   switch (lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig)) {
-#define GDOCK_CASE(KK)                                                                                            \
-  case KK:                                                                                                        \
-    launch_pair<KK>(d_cx, d_poses, d_coords, d_extent, n_poses, w, d_out, d_nsat, d_redo, lds, stream);         \
+#define GDOCK_CASE(KK)                                                                                  \
+  case KK:                                                                                              \
+    launch_pair<KK>(d_cx, d_poses, d_coords, n_poses, w, d_out, d_nsat, d_redo, parts, split, lds, stream); \
     break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)
@@ -882,11 +962,7 @@ hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const doub
   return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int max_n_rec) {
-  DevComplex cx{};
-  cx.n_rec = max_n_rec;
-  return score_lds_bytes(cx);
-}
+size_t batch_lds_bytes(int max_slots) { return score_lds_bytes(max_slots); }
 
 int batch_grid(size_t lds, int device) {
   int per_cu = 0, cus = 0;
@@ -931,11 +1007,13 @@ __global__ __launch_bounds__(256) void pose_plan_kernel(const DevComplex* __rest
     const float px = static_cast<float>(ox + p[3]), py = static_cast<float>(oy + p[4]),
                 pz = static_cast<float>(oz + p[5]);
     const float reach = static_cast<float>(lig_r) + kCullReach;
+    const float unit = ldexpf(1.0f, -cx.qexp);
     int32_t near = 0;
 #pragma unroll 4
     for (int j = lane; j < cx.n_rec; j += 256) {  // every fourth slot: an estimate
-      const float4 r = cx.rec_xyzq[j];
-      const float dx = r.x - px, dy = r.y - py, dz = r.z - pz;
+      const int4 r = cx.rec_fixed[j];
+      const float dx = static_cast<float>(r.x) * unit - px, dy = static_cast<float>(r.y) * unit - py,
+                  dz = static_cast<float>(r.z) * unit - pz;
       near += dx * dx + dy * dy + dz * dz <= reach * reach ? 4 : 0;
     }
 #pragma unroll
@@ -944,7 +1022,7 @@ __global__ __launch_bounds__(256) void pose_plan_kernel(const DevComplex* __rest
     const int per_chunk = kScoreThreads * k;
     const double slots = static_cast<double>((cx.n_lig + per_chunk - 1) / per_chunk) * per_chunk;
     const double cost = (near + 0.0625 * cx.n_rec) * slots;
-    const int parts = min(max(cx.parts, 1), max(1, static_cast<int>(ceil(cost / kSplitCost))));
+    const int parts = min(max(cx.parts, 1), max(cx.min_parts, static_cast<int>(ceil(cost / kSplitCost))));
     const double share = cost / parts;
     const int b = share < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(share)));
     if (lane == 0) {

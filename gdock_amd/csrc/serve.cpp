@@ -76,6 +76,7 @@ struct gdock_server {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   int max_pop = 0;
+  int lds_slots = 0;  // receptor slots the launch's LDS table holds per item
   // jobs
   std::mutex mu;
   std::condition_variable cv;      // worker: new work or stop
@@ -304,15 +305,13 @@ void gdock_server::prepare_loop() {
       st = gdock_complex_create(pctx, j->rec.data(), static_cast<int32_t>(j->rec.size()), lig.data(),
                                 static_cast<int32_t>(lig.size()), j->pairs.data(),
                                 static_cast<int32_t>(j->pairs.size() / 2), &j->cx);
-    if (st == GDOCK_OK && j->cx->dev.n_rec > cfg.max_receptor_atoms + 64) {
-      set_error(pctx, "receptor of %d slots exceeds the server's %d", j->cx->dev.n_rec, cfg.max_receptor_atoms);
+    if (st == GDOCK_OK && (j->cx->dev.n_rec_atoms > cfg.max_receptor_atoms || batch_part_slots(j->cx->dev) > lds_slots)) {
+      set_error(pctx, "receptor of %d atoms (%d slots) exceeds the server's %d", j->cx->dev.n_rec_atoms,
+                j->cx->dev.n_rec, cfg.max_receptor_atoms);
       st = GDOCK_ERR_INVALID_ARG;
     }
     hipError_t he = hipSuccess;
-    if (st == GDOCK_OK) {
-      j->lanes = tune_lanes(pctx, j->cx, &he);
-      if (he != hipSuccess) st = hip_fail(pctx, he, "server: autotune");
-    }
+    if (st == GDOCK_OK) j->lanes = choose_lanes(j->cx->dev);
     size_t split_bytes = 0;
     if (st == GDOCK_OK) {
       j->parts = pose_parts(j->cx->dev, j->lanes);
@@ -331,6 +330,7 @@ void gdock_server::prepare_loop() {
       j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
       j->dev = j->cx->dev;
       j->dev.lanes_k = j->lanes;
+      j->dev.min_parts = min_batch_parts(j->dev);
       bind_split_state(&j->dev, j->parts, max_pop, j->slot * max_pop, static_cast<char*>(j->split_mem));
     }
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_prep).count();
@@ -404,7 +404,7 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   gdock_server_config c;
   if (config) c = *config; else gdock_server_default_config(&c);
   if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_population > 65536 || c.max_receptor_atoms < 1 ||
-      c.max_receptor_atoms > kMaxLdsReceptor - 64 || c.memory_budget < 0 ||
+      c.max_receptor_atoms > kMaxReceptorAtoms || c.memory_budget < 0 ||
       static_cast<int64_t>(c.max_jobs) * c.max_population * kMaxPoseParts > (int64_t{1} << 30))
     return GDOCK_ERR_INVALID_ARG;
   auto* s = new gdock_server();
@@ -451,7 +451,10 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
     L.redo = reinterpret_cast<int32_t*>(m + o_redo);
     bind_order(&L, m + o_order, cap);
-    L.lds = batch_lds_bytes(c.max_receptor_atoms + 64);
+    // a job's items cover at most min(its receptor slots, kPartSlots) slots;
+    // class padding adds at most 3 slots per (rmin2, eps) class
+    s->lds_slots = std::min(kPartSlots, (c.max_receptor_atoms + 256 + 63) / 64 * 64);
+    L.lds = batch_lds_bytes(s->lds_slots);
     L.capacity = static_cast<int>(cap);
     L.item_capacity = static_cast<int>(cap_items);
     L.grid = std::min(batch_grid(L.lds, device), L.item_capacity);

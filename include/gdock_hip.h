@@ -186,6 +186,15 @@ gdock_status gdock_batch_score(gdock_ctx* ctx, gdock_batch* b, const double* gen
  * counts) and an asynchronous graph replay on `stream`. */
 gdock_status gdock_batch_device_buffers(gdock_batch* b, double** d_poses, double** d_out, int32_t** d_nsat);
 gdock_status gdock_batch_launch(gdock_ctx* ctx, gdock_batch* b, const gdock_weights* w, void* stream);
+/* Ligand atoms per lane the library uses for a complex (a pure function of
+ * its ligand and receptor sizes, read from the library's committed table of
+ * measured choices), and the autotuner's measurement: the mean time (ms) of
+ * one plain scoring launch of n_poses fixed probe poses with lanes_k ligand
+ * atoms per lane (1-8), over reps launches after one warm-up.  No
+ * counterpart in the reference (its CPU loop has no such parameter). */
+int32_t gdock_choose_lanes(const gdock_complex* cx);
+gdock_status gdock_time_lanes(gdock_ctx* ctx, gdock_complex* cx, int32_t lanes_k, int32_t n_poses, int32_t reps,
+                              float* ms);
 
 /* ---- GA docking job (host driver, device scoring) ------------------------ */
 /* commands::run (src/commands/run.rs:227-733) from the positioned complex on:

@@ -229,7 +229,11 @@ def test_component_errors_against_fp64_oracle():
     complex (receptor 2000 x ligand 1500 atoms, seed 1, poses as the benchmark
     draws them) and on a large uneven-batch complex with GA-initial poses:
     the largest absolute and relative errors are printed (the numbers the
-    design notes quote) and every one is inside the tolerance."""
+    design notes quote) and every one is inside the tolerance.  Coulomb sums
+    are held to 5e-4 kcal/mol absolute, well inside the 1e-3 floor, whatever
+    the size of the net energy: fixed-point coordinate differences, exact
+    receptor charges and FP64 folds every 16 receptor slots keep the error at
+    the FP32 rounding of the individual terms."""
     cases = []
     cfg = synth.CONFIGS["batched_scoring"]
     c = synth.make_complex(cfg["seed"], rec_atoms=cfg["rec_atoms"], lig_atoms=cfg["lig_atoms"],
@@ -252,3 +256,4 @@ def test_component_errors_against_fp64_oracle():
             nz = np.abs(ref[:, j]) > 1e-3
             parts.append(f"{name} max abs {err[:, j].max():.2e} max rel {rel[nz, j].max() if nz.any() else 0.0:.2e}")
         print(f"{what} ({len(rec)} x {len(lig)} atoms, {len(genes)} poses): " + "; ".join(parts))
+        assert err[:, 1].max() <= 5e-4, (what, err[:, 1].max())
