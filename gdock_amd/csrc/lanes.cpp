@@ -22,10 +22,14 @@ namespace gdock {
 
 namespace {
 
-// One row per measured shape: ligand atoms, receptor size class (half
-// octaves of receptor atoms), chosen chunks.
+// One row per measured shape: ligand atoms (the midpoint of a 128-atom
+// interval, within which the candidate K values do not change: they change
+// at multiples of 256, 512 and 768), receptor size class (half octaves of
+// receptor atoms), chosen chunks.
 struct LanesRow { int32_t n_lig, rec_class, chunks; };
 #include "lanes_table.inc"
+
+constexpr int kLigInterval = 128;
 
 int rec_size_class(int n_rec_atoms) { return static_cast<int>(std::lround(2.0 * std::log2(std::max(n_rec_atoms, 1)))); }
 
@@ -52,16 +56,18 @@ std::vector<int> lane_candidates(int n_lig) {
   return ks;
 }
 
-// The chunk count of the nearest tabulated shape of the same receptor size
-// class (ties: the smaller ligand), applied to this ligand; without a row
-// for the class, the fewest chunks with K <= 6 (K = 7 and 8 hold their ligand
-// atoms in more registers than the 4-waves-per-SIMD budget).
+// The chunk count of the tabulated shape in the same 128-atom ligand
+// interval with the nearest receptor size class (ties: the smaller class);
+// without a row for the interval, the fewest chunks with K <= 6 (K = 7 and 8
+// hold their ligand atoms in more registers than the 4-waves-per-SIMD
+// budget leaves).
 int choose_lanes(const DevComplex& d) {
   const int n_lig = std::max(d.n_lig, 1), cls = rec_size_class(d.n_rec_atoms);
+  const int interval = (n_lig - 1) / kLigInterval;
   const LanesRow* best = nullptr;
   for (const LanesRow& r : kLanesTable) {
-    if (r.rec_class != cls) continue;
-    if (!best || std::abs(r.n_lig - n_lig) < std::abs(best->n_lig - n_lig)) best = &r;
+    if (r.rec_class < 0 || (r.n_lig - 1) / kLigInterval != interval) continue;
+    if (!best || std::abs(r.rec_class - cls) < std::abs(best->rec_class - cls)) best = &r;
   }
   if (best) return lanes_for_chunks(n_lig, best->chunks);
   const int chunks = (n_lig + kScoreThreads * 6 - 1) / (kScoreThreads * 6);
