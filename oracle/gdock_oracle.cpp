@@ -576,9 +576,28 @@ std::vector<ClusterResult> cluster_structures(const std::vector<Molecule>& struc
 }
 
 // ---------------------------------------------------------------- GA
-// Project-defined random stream: SplitMix64 seeding, xoshiro256**.
+// Test builds of the oracle define ORACLE_MUTANT to plant one plausible
+// mistake in the GA decisions (see the switches below); the known-answer
+// tests must fail on every such build.  0: the restatement itself.
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
+// Project-defined random stream: SplitMix64 seeding, xoshiro256**.  Test
+// hook: with a script, every draw replays the next scripted value instead
+// (unit and unit_incl: the value itself; below(n): the index; normal: the
+// standard deviate z, returned as mean + sd z) and is logged as (kind, n).
 struct Rng {
   uint64_t s[4];
+  const double* script = nullptr;
+  int script_len = 0, script_pos = 0;
+  bool script_error = false;  // ran out, or a scripted index out of range
+  std::vector<std::pair<int, double>>* log = nullptr;
+  double scripted(int kind, double arg) {
+    if (log) log->emplace_back(kind, arg);
+    if (script_pos >= script_len) { script_error = true; return 0.0; }
+    return script[script_pos++];
+  }
   explicit Rng(uint64_t seed) {
     uint64_t x = seed;
     for (auto& v : s) {
@@ -595,10 +614,21 @@ struct Rng {
     s[2] ^= t; s[3] = rotl(s[3], 45);
     return r;
   }
-  double unit() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }          // [0,1)
-  double unit_incl() { return static_cast<double>(next() >> 11) / 9007199254740991.0; }  // [0,1]
+  double unit() {  // [0,1)
+    if (script) return scripted(0, 0.0);
+    return static_cast<double>(next() >> 11) * 0x1.0p-53;
+  }
+  double unit_incl() {  // [0,1]
+    if (script) return scripted(1, 0.0);
+    return static_cast<double>(next() >> 11) / 9007199254740991.0;
+  }
   double range_incl(double lo, double hi) { double v = lo + (hi - lo) * unit_incl(); return v > hi ? hi : v; }
   uint64_t below(uint64_t n) {  // Lemire
+    if (script) {
+      const double v = scripted(2, static_cast<double>(n));
+      if (!(v >= 0.0 && v < static_cast<double>(n))) { script_error = true; return 0; }
+      return static_cast<uint64_t>(v);
+    }
     __uint128_t m = static_cast<__uint128_t>(next()) * n;
     uint64_t l = static_cast<uint64_t>(m);
     if (l < n) {
@@ -608,6 +638,7 @@ struct Rng {
     return static_cast<uint64_t>(m >> 64);
   }
   double normal(double mean, double sd) {  // Box-Muller, one draw per call
+    if (script) return mean + sd * scripted(3, 0.0);
     const double u1 = 1.0 - unit(), u2 = unit();
     return mean + sd * (std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPI * u2));
   }
@@ -634,8 +665,16 @@ Chromo new_chromosome(Rng& rng) {
 // chromosome.rs:56-77
 void mutate(Chromo& c, Rng& rng, double rate) {
   for (int i = 0; i < 6; ++i)
+#if ORACLE_MUTANT == 4  // planted: mutation at u < rate
+    if (rng.unit() < rate) {
+#else
     if (rng.unit() <= rate) {
+#endif
+#if ORACLE_MUTANT == 8  // planted: angles wrapped with fmod alone (negative angles stay negative)
+      if (i < 3) c.genes[i] = std::fmod(c.genes[i] + rng.normal(0.0, 0.174), 2.0 * kPI);
+#else
       if (i < 3) c.genes[i] = rem_euclid(c.genes[i] + rng.normal(0.0, 0.174), 2.0 * kPI);
+#endif
       else c.genes[i] = std::min(std::max(c.genes[i] + rng.normal(0.0, 1.0), -kMaxDisplacement), kMaxDisplacement);
     }
 }
@@ -644,7 +683,11 @@ void mutate(Chromo& c, Rng& rng, double rate) {
 void crossover(Rng& rng, const Chromo& p1, const Chromo& p2, Chromo* o1, Chromo* o2) {
   *o1 = p1; *o2 = p2;
   for (int i = 0; i < 6; ++i)
+#if ORACLE_MUTANT == 3  // planted: crossover at u < rate
+    if (rng.unit() < kCrossoverRate) { o1->genes[i] = p2.genes[i]; o2->genes[i] = p1.genes[i]; }
+#else
     if (rng.unit() <= kCrossoverRate) { o1->genes[i] = p2.genes[i]; o2->genes[i] = p1.genes[i]; }
+#endif
 }
 
 // population.rs:124-158
@@ -654,7 +697,11 @@ std::vector<Chromo> tournament(const std::vector<Chromo>& pop, Rng& rng) {
     const Chromo* best = nullptr;
     for (int t = 0; t < kTournamentSize; ++t) {
       const Chromo* c = &pop[rng.below(pop.size())];
+#if ORACLE_MUTANT == 5  // planted: the last minimum wins
+      if (!best || c->fitness <= best->fitness) best = c;
+#else
       if (!best || c->fitness < best->fitness) best = c;  // min_by keeps the first minimum
+#endif
     }
     out.push_back(*best);
   }
@@ -664,10 +711,17 @@ std::vector<Chromo> tournament(const std::vector<Chromo>& pop, Rng& rng) {
 // population.rs:74-121
 std::vector<Chromo> evolve(const std::vector<Chromo>& pop, Rng& rng) {
   std::vector<Chromo> elite = pop;
+#if ORACLE_MUTANT == 2  // planted: an unstable sort (equal fitness in reverse order)
+  std::reverse(elite.begin(), elite.end());
+#endif
   std::stable_sort(elite.begin(), elite.end(), [](const Chromo& a, const Chromo& b) { return a.fitness < b.fitness; });
   elite.resize(std::min<size_t>(kElitism, elite.size()));
   std::vector<Chromo> np = tournament(pop, rng);
+#if ORACLE_MUTANT == 10  // planted: a forward shuffle (swap i with below(n - i) + i)
+  for (size_t i = 0; i + 1 < np.size(); ++i) std::swap(np[i], np[i + rng.below(np.size() - i)]);
+#else
   for (size_t i = np.size(); i > 1; --i) std::swap(np[i - 1], np[rng.below(i)]);  // shuffle
+#endif
   std::vector<Chromo> kids;
   for (size_t i = 0; i + 1 < np.size(); i += 2) {
     Chromo a, b;
@@ -675,7 +729,11 @@ std::vector<Chromo> evolve(const std::vector<Chromo>& pop, Rng& rng) {
     kids.push_back(a); kids.push_back(b);
   }
   for (auto& c : kids) mutate(c, rng, kMutationRate);
+#if ORACLE_MUTANT == 1  // planted: the elites overwrite the best children by stale fitness
+  std::stable_sort(kids.begin(), kids.end(), [](const Chromo& a, const Chromo& b) { return a.fitness < b.fitness; });
+#else
   std::stable_sort(kids.begin(), kids.end(), [](const Chromo& a, const Chromo& b) { return b.fitness < a.fitness; });
+#endif
   for (size_t i = 0; i < elite.size() && i < kids.size(); ++i) kids[i] = elite[i];
   return kids;
 }
@@ -712,6 +770,34 @@ struct HallOfFame {
     }
   }
 };
+
+// run.rs:694-733: cluster centres stable-sorted by fitness, the first five
+// taken; fewer than five are filled from the best-scored Hall of Fame
+// entries not chosen yet (stable by fitness), each with cluster size 1.
+std::vector<std::pair<int64_t, int64_t>> select_models(const std::vector<HofEntry>& hof,
+                                                      const std::vector<ClusterResult>& clusters) {
+  std::vector<std::pair<int64_t, int64_t>> selected;
+  std::vector<std::tuple<int64_t, double, int64_t>> centres;
+  for (auto& c : clusters)
+    centres.emplace_back(c.center, hof[c.center].fitness, static_cast<int64_t>(c.members.size()));
+  std::stable_sort(centres.begin(), centres.end(), [](auto& a, auto& b) { return std::get<1>(a) < std::get<1>(b); });
+  for (size_t i = 0; i < centres.size() && i < static_cast<size_t>(kNumOutputModels); ++i)
+    selected.emplace_back(std::get<0>(centres[i]), std::get<2>(centres[i]));
+  if (selected.size() < static_cast<size_t>(kNumOutputModels)) {
+    std::vector<std::pair<int64_t, double>> all;
+    for (size_t i = 0; i < hof.size(); ++i) all.emplace_back(static_cast<int64_t>(i), hof[i].fitness);
+#if ORACLE_MUTANT != 7  // planted (7): filled in Hall of Fame order instead of by fitness
+    std::stable_sort(all.begin(), all.end(), [](auto& a, auto& b) { return a.second < b.second; });
+#endif
+    std::vector<int64_t> chosen;
+    for (auto& s : selected) chosen.push_back(s.first);
+    for (auto& [idx, f] : all) {
+      if (selected.size() >= static_cast<size_t>(kNumOutputModels)) break;
+      if (std::find(chosen.begin(), chosen.end(), idx) == chosen.end()) selected.emplace_back(idx, 1);
+    }
+  }
+  return selected;
+}
 
 // Per-job state the parity test reads back.
 struct GaTrace {
@@ -764,8 +850,16 @@ void run_ga(const Molecule& rec, const Molecule& lig_pos, const std::vector<Rest
     tr->best_idx.push_back(static_cast<int32_t>(bi));
     const double best = pop[bi].fitness;
     if (gen > 0) {
+#if ORACLE_MUTANT == 9  // planted: the improvement divided by the signed previous best
+      const double imp = (last - best) / last;
+#else
       const double imp = (last - best) / std::fabs(last);
+#endif
+#if ORACLE_MUTANT == 6  // planted: a stall counted at imp <= threshold
+      stall = imp <= kConvergenceThreshold ? stall + 1 : 0;
+#else
       stall = imp < kConvergenceThreshold ? stall + 1 : 0;
+#endif
     }
     last = best;
     tr->generations = gen + 1;
@@ -784,23 +878,7 @@ void run_ga(const Molecule& rec, const Molecule& lig_pos, const std::vector<Rest
   }
   ClusteringConfig cfg{5.0, 0.60, 0.75, 4};
   tr->clusters = cluster_structures(structs, cfg, nullptr);
-  std::vector<std::tuple<int64_t, double, int64_t>> centres;
-  for (auto& c : tr->clusters)
-    centres.emplace_back(c.center, tr->hof.entries[c.center].fitness, static_cast<int64_t>(c.members.size()));
-  std::stable_sort(centres.begin(), centres.end(), [](auto& a, auto& b) { return std::get<1>(a) < std::get<1>(b); });
-  for (size_t i = 0; i < centres.size() && i < static_cast<size_t>(kNumOutputModels); ++i)
-    tr->selected.emplace_back(std::get<0>(centres[i]), std::get<2>(centres[i]));
-  if (tr->selected.size() < static_cast<size_t>(kNumOutputModels)) {
-    std::vector<std::pair<int64_t, double>> all;
-    for (size_t i = 0; i < tr->hof.entries.size(); ++i) all.emplace_back(static_cast<int64_t>(i), tr->hof.entries[i].fitness);
-    std::stable_sort(all.begin(), all.end(), [](auto& a, auto& b) { return a.second < b.second; });
-    std::vector<int64_t> chosen;
-    for (auto& s : tr->selected) chosen.push_back(s.first);
-    for (auto& [idx, f] : all) {
-      if (tr->selected.size() >= static_cast<size_t>(kNumOutputModels)) break;
-      if (std::find(chosen.begin(), chosen.end(), idx) == chosen.end()) tr->selected.emplace_back(idx, 1);
-    }
-  }
+  tr->selected = select_models(tr->hof.entries, tr->clusters);
 }
 
 Molecule from_records(const RecordAtom* r, int n) {
@@ -1162,6 +1240,70 @@ void or_evolve(const double* genes, const double* fitness, int n, uint64_t seed,
   Rng rng(seed);
   auto np = evolve(pop, rng);
   for (size_t i = 0; i < np.size(); ++i) { std::memcpy(out_genes + 6 * i, np[i].genes, 6 * sizeof(double)); out_fitness[i] = np[i].fitness; }
+}
+
+// One evolve step on a scripted random stream (Rng's test hook): returns the
+// number of draws taken, or -1 when the script ran out or a scripted index
+// was out of range.  log_kind / log_arg (capacity log_cap) receive each
+// draw's kind (0 unit, 1 unit_incl, 2 below, 3 normal) and below's n.
+int or_evolve_scripted(const double* genes, const double* fitness, int n, const double* script, int script_len,
+                       double* out_genes, double* out_fitness, int32_t* log_kind, double* log_arg, int log_cap) {
+  std::vector<Chromo> pop(n);
+  for (int i = 0; i < n; ++i) { std::memcpy(pop[i].genes, genes + 6 * i, 6 * sizeof(double)); pop[i].fitness = fitness[i]; }
+  Rng rng(0);
+  std::vector<std::pair<int, double>> log;
+  rng.script = script; rng.script_len = script_len; rng.log = &log;
+  auto np = evolve(pop, rng);
+  for (size_t i = 0; i < np.size(); ++i) { std::memcpy(out_genes + 6 * i, np[i].genes, 6 * sizeof(double)); out_fitness[i] = np[i].fitness; }
+  for (size_t i = 0; i < log.size() && static_cast<int>(i) < log_cap; ++i) { log_kind[i] = log[i].first; log_arg[i] = log[i].second; }
+  if (rng.script_error) return -1;
+  return static_cast<int>(log.size());
+}
+
+// The generation loop of run.rs:361-531 with every individual of generation
+// g given fitness fseq[min(g, nseq - 1)] (the random stream of `seed` breeds
+// the populations, which do not matter here): returns the number of
+// evaluated generations; *converged_at receives the early-stop generation or
+// -1.
+int or_ga_fitness_sequence(int pop_size, int max_gen, const double* fseq, int nseq, uint64_t seed, int32_t* converged_at) {
+  GaTrace tr;
+  const Molecule none;
+  const std::vector<Restraint> rs;
+  int g = 0;
+  auto score = [&](std::vector<Chromo>& pop) {
+    const double f = fseq[std::min(g, nseq - 1)];
+    for (auto& c : pop) c.fitness = f;
+    ++g;
+  };
+  run_ga(none, none, rs, nullptr, seed, pop_size, max_gen, false, 1, &tr, score);
+  if (converged_at) *converged_at = tr.converged_at;
+  return tr.generations;
+}
+
+// Model selection (run.rs:694-733) from Hall of Fame fitness values and a
+// cluster list (centre, size) in cluster_structures' order; returns the
+// number of models.
+int or_select_models(const double* hof_fitness, int n_hof, const int64_t* centers, const int64_t* sizes,
+                     int n_clusters, int64_t* out_idx, int64_t* out_size) {
+  std::vector<HofEntry> hof(n_hof);
+  for (int i = 0; i < n_hof; ++i) { std::memset(hof[i].genes, 0, sizeof hof[i].genes); hof[i].fitness = hof_fitness[i]; hof[i].generation = 0; }
+  std::vector<ClusterResult> cl(n_clusters);
+  for (int i = 0; i < n_clusters; ++i) { cl[i].center = centers[i]; cl[i].members.assign(static_cast<size_t>(sizes[i]), 0); }
+  auto sel = select_models(hof, cl);
+  for (size_t i = 0; i < sel.size(); ++i) { out_idx[i] = sel[i].first; out_size[i] = sel[i].second; }
+  return static_cast<int>(sel.size());
+}
+
+// HallOfFame::add_from_population (run.rs:194-216) on a given population.
+void or_hof_add_population(void* h, const double* genes, const double* fitness, int n, int gen) {
+  std::vector<Chromo> pop(n);
+  for (int i = 0; i < n; ++i) { std::memcpy(pop[i].genes, genes + 6 * i, 6 * sizeof(double)); pop[i].fitness = fitness[i]; }
+  static_cast<HallOfFame*>(h)->add_from_population(pop, static_cast<uint32_t>(gen));
+}
+int or_hof_genes(void* h, double* genes) {
+  auto* t = static_cast<HallOfFame*>(h);
+  for (size_t i = 0; i < t->entries.size(); ++i) std::memcpy(genes + 6 * i, t->entries[i].genes, 6 * sizeof(double));
+  return static_cast<int>(t->entries.size());
 }
 
 // Evaluator (evaluator.rs): receptor and reference ligand as given.

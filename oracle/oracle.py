@@ -25,8 +25,11 @@ P, D, I32, I64 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POIN
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", str(HERE)], check=True)
     return LIB
+
+
+MUTANTS = HERE / "mutants"
 
 
 def lib() -> C.CDLL:
@@ -34,7 +37,21 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB.exists():
             build()
-        L = C.CDLL(str(LIB))
+        _lib = load(LIB)
+    return _lib
+
+
+_loaded: dict = {}
+
+
+def load(path) -> C.CDLL:
+    """The oracle library at `path` (the restatement or one of its mutant
+    builds, oracle/mutants/libgdock_oracle_m<k>.so) with its signatures set."""
+    path = str(path)
+    if path in _loaded:
+        return _loaded[path]
+    if True:
+        L = C.CDLL(path)
         sig = {
             "or_pdb_count_models": (C.c_int, [C.c_char_p]),
             "or_pdb_model": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int]),
@@ -75,12 +92,17 @@ def lib() -> C.CDLL:
             "or_evaluator_info": (None, [P, I32, I32, I32, C.POINTER(C.c_int16), C.POINTER(C.c_int16)]),
             "or_calc_metrics": (None, [P, P, C.c_int, C.c_int, D]),
             "or_eval_contacts": (C.c_int, [P, C.c_int, P, C.c_int]),
+            "or_evolve_scripted": (C.c_int, [D, D, C.c_int, D, C.c_int, D, D, I32, D, C.c_int]),
+            "or_ga_fitness_sequence": (C.c_int, [C.c_int, C.c_int, D, C.c_int, C.c_uint64, I32]),
+            "or_select_models": (C.c_int, [D, C.c_int, I64, I64, C.c_int, I64, I64]),
+            "or_hof_add_population": (None, [P, D, D, C.c_int, C.c_int]),
+            "or_hof_genes": (C.c_int, [P, D]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
             f.restype, f.argtypes = r, a
-        _lib = L
-    return _lib
+        _loaded[path] = L
+    return L
 
 
 def _p(a, t=C.c_double):
@@ -289,6 +311,18 @@ class Hof:
         lib().or_hof_entries(self.h, _p(o))
         return o
 
+    def genes(self):
+        n = lib().or_hof_entries(self.h, None)
+        o = np.zeros((max(n, 1), 6))
+        lib().or_hof_genes(self.h, _p(o))
+        return o[:n]
+
+    def add_population(self, genes, fitness, gen):
+        """HallOfFame::add_from_population (run.rs:194-216)."""
+        g = np.ascontiguousarray(genes, np.float64).reshape(-1, 6)
+        f = np.ascontiguousarray(fitness, np.float64)
+        lib().or_hof_add_population(self.h, _p(g), _p(f), len(f), gen)
+
     def __del__(self):
         try:
             lib().or_hof_free(self.h)
@@ -332,3 +366,50 @@ class Evaluator:
 def eval_contacts(a, b) -> int:
     """evaluator.rs calculate_contacts: number of atom pairs within 5 A."""
     return lib().or_eval_contacts(_v(a), len(a), _v(b), len(b))
+
+
+# ---- GA decisions on scripted streams (the oracle's Rng test hook)
+DRAW_KINDS = {0: "unit", 1: "unit_incl", 2: "below", 3: "normal"}
+
+
+def evolve_scripted(genes, fitness, script, L=None):
+    """One Population::evolve step (population.rs:74-121) whose random draws
+    replay `script` in order (unit / unit_incl: the value; below(n): the
+    index; normal: the standard deviate).  Returns (genes, fitness, draws,
+    ok), draws being the (kind, n) of every draw taken."""
+    L = L or lib()
+    genes = np.ascontiguousarray(genes, np.float64).reshape(-1, 6)
+    fitness = np.ascontiguousarray(fitness, np.float64)
+    script = np.ascontiguousarray(script, np.float64)
+    n = len(fitness)
+    og, of = np.zeros((n, 6)), np.zeros(n)
+    cap = 4096
+    kinds, args = np.zeros(cap, np.int32), np.zeros(cap)
+    m = L.or_evolve_scripted(_p(genes), _p(fitness), n, _p(script), len(script), _p(og), _p(of), _p(kinds, C.c_int32),
+                             _p(args), cap)
+    taken = m if m >= 0 else cap
+    draws = [(DRAW_KINDS[int(kinds[i])], int(args[i])) for i in range(min(taken, cap)) if kinds[i] in DRAW_KINDS]
+    return og, of, draws, m >= 0 and m == len(script)
+
+
+def ga_fitness_sequence(fseq, pop_size=6, max_gen=40, seed=1, L=None):
+    """The generation loop with every individual of generation g scored
+    fseq[min(g, len - 1)]: (generations evaluated, early-stop generation or -1)."""
+    L = L or lib()
+    f = np.ascontiguousarray(fseq, np.float64)
+    conv = C.c_int32()
+    g = L.or_ga_fitness_sequence(pop_size, max_gen, _p(f), len(f), seed, C.byref(conv))
+    return g, conv.value
+
+
+def select_models(hof_fitness, clusters, L=None):
+    """Model selection (run.rs:694-733) from Hall of Fame fitness values and
+    clusters [(centre, size), ...] in cluster_structures' order."""
+    L = L or lib()
+    hf = np.ascontiguousarray(hof_fitness, np.float64)
+    cen = np.ascontiguousarray([c for c, _ in clusters] or [0], np.int64)
+    siz = np.ascontiguousarray([s for _, s in clusters] or [0], np.int64)
+    oi, os_ = np.zeros(16, np.int64), np.zeros(16, np.int64)
+    k = L.or_select_models(_p(hf), len(hf), _p(cen, C.c_int64), _p(siz, C.c_int64), len(clusters),
+                           _p(oi, C.c_int64), _p(os_, C.c_int64))
+    return [(int(oi[i]), int(os_[i])) for i in range(k)]
