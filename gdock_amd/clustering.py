@@ -28,7 +28,7 @@ class ClusterResult:
 def calculate_contacts(cx: Complex, genes, contact_distance: float = 5.0) -> list[set[str]]:
     """Contact strings of receptor + (ligand moved by each gene row), clustering.rs:62-115."""
     bits, _ = cx.contacts(np.asarray(genes).reshape(-1, 6), contact_distance)
-    return [cx.contact_set(b) for b in bits]
+    return [cx.contact_set(b, contact_distance) for b in bits]
 
 
 def cluster_structures(cx: Complex, genes, config: ClusteringConfig | None = None) -> list[ClusterResult]:

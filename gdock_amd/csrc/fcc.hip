@@ -5,8 +5,14 @@
 // then FP32) sit in lanes, receptor heavy atoms are wave-uniform scalar loads
 // walked residue by residue.  The 5 A test is FP32 against a guard band with
 // an FP64 re-decision inside the band, so every contact bit equals the FP64
-// reference's decision.  Bit b = receptor residue * n_lig_res + ligand
-// residue; the residue-pair string of a bit is reported by gdock_contact_pair.
+// reference's decision.  Residues are (chain, resseq) keys over both
+// molecules; receptor x ligand key pairs of different chains take the first
+// bits (bit = receptor group * n_lig_res + ligand group for one receptor
+// chain against one other ligand chain), and key pairs that only receptor x
+// receptor or ligand x ligand atoms can bring into contact follow, decided
+// once per cutoff on the host (a ligand x ligand atom pair within 1e-6 A^2 of
+// the threshold is re-decided per pose in FP64).  The residue-pair string of
+// a bit is reported by gdock_contact_pair.
 //
 // Overlap: bitsets are compacted to the 32-bit words that are set in any
 // pose and stored word-major, so a wave reads 16 row words with one scalar
@@ -25,6 +31,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gdock_hip.h"
@@ -36,27 +43,81 @@ namespace gdock {
 size_t contacts_lds_bytes(const DevContacts& c);
 
 // ---------------------------------------------------------------- tables
+// Residue keys (clustering.rs:66-76): (chain, resseq) over the heavy atoms of
+// receptor and ligand; a key's receptor atoms form its receptor group, its
+// ligand atoms its ligand group.  Receptor x ligand group pairs of different
+// chains get the bits [0, n_dyn), in row-major order of (receptor group,
+// ligand group); a key pair reached from both sides (both keys hold receptor
+// and ligand atoms) shares one bit.  With one receptor chain against one
+// other ligand chain the map is the identity and is not stored.
 void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* host) {
   cx->con = DevContacts{};
   cx->con_offsets.clear();
-  cx->rec_res_key.clear();
-  cx->lig_res_key.clear();
+  cx->res_key.clear();
+  cx->rec_group_key.clear();
+  cx->lig_group_key.clear();
+  cx->pair_bit.clear();
+  cx->dyn_keys.clear();
+  cx->rec_heavy_xyz.clear();
+  cx->lig_heavy_xyz.clear();
+  cx->rec_heavy_key.clear();
+  cx->lig_heavy_key.clear();
+  cx->lig_heavy_slot.clear();
   cx->con_unsupported.clear();
+  cx->cut = ContactCut{};
   auto heavy = [](const gdock_atom& a) { return field(a.element, sizeof a.element) != "H"; };
-  std::map<std::pair<char, int16_t>, std::vector<int32_t>> rres, lres;
+  std::map<std::pair<char, int16_t>, int32_t> key_of;
+  for (auto& a : cx->rec) if (heavy(a)) key_of[{a.chainid, a.resseq}] = 0;
+  for (auto& a : cx->lig) if (heavy(a)) key_of[{a.chainid, a.resseq}] = 0;
+  for (auto& kv : key_of) { kv.second = static_cast<int32_t>(cx->res_key.size()); cx->res_key.push_back(kv.first); }
+  const int32_t nkeys = static_cast<int32_t>(cx->res_key.size());
+  std::vector<std::vector<int32_t>> rres(nkeys), lres(nkeys);
   std::set<char> rch, lch;
   for (int32_t i = 0; i < static_cast<int32_t>(cx->rec.size()); ++i)
-    if (heavy(cx->rec[i])) { rres[{cx->rec[i].chainid, cx->rec[i].resseq}].push_back(i); rch.insert(cx->rec[i].chainid); }
+    if (heavy(cx->rec[i])) { rres[key_of[{cx->rec[i].chainid, cx->rec[i].resseq}]].push_back(i); rch.insert(cx->rec[i].chainid); }
   for (int32_t i = 0; i < static_cast<int32_t>(cx->lig.size()); ++i)
-    if (heavy(cx->lig[i])) { lres[{cx->lig[i].chainid, cx->lig[i].resseq}].push_back(i); lch.insert(cx->lig[i].chainid); }
-  // The device path covers one receptor chain against one other ligand chain
-  // (the layout every docking run uses).  Same-chain keys would merge
-  // receptor and ligand residues and extra chains add receptor-receptor or
-  // ligand-ligand candidates; those topologies are rejected by gdock_contacts.
-  if (rch.size() > 1 || lch.size() > 1 || (!rch.empty() && !lch.empty() && *rch.begin() == *lch.begin())) {
-    cx->con_unsupported = "contacts: device path needs one receptor chain and one different ligand chain";
-    return;
+    if (heavy(cx->lig[i])) { lres[key_of[{cx->lig[i].chainid, cx->lig[i].resseq}]].push_back(i); lch.insert(cx->lig[i].chainid); }
+  cx->rec_chains = static_cast<int32_t>(rch.size());
+  cx->lig_chains = static_cast<int32_t>(lch.size());
+  cx->rg_of_key.assign(nkeys, -1);
+  cx->lg_of_key.assign(nkeys, -1);
+  for (int32_t k = 0; k < nkeys; ++k) {
+    if (!rres[k].empty()) { cx->rg_of_key[k] = static_cast<int32_t>(cx->rec_group_key.size()); cx->rec_group_key.push_back(k); }
+    if (!lres[k].empty()) { cx->lg_of_key[k] = static_cast<int32_t>(cx->lig_group_key.size()); cx->lig_group_key.push_back(k); }
   }
+  const int32_t n_rg = static_cast<int32_t>(cx->rec_group_key.size()), n_lg = static_cast<int32_t>(cx->lig_group_key.size());
+  // bits of the receptor x ligand group pairs
+  std::vector<int32_t> pb(static_cast<size_t>(n_rg) * n_lg, -1);
+  bool identity = true;
+  for (int32_t rg = 0; rg < n_rg; ++rg)
+    for (int32_t lg = 0; lg < n_lg; ++lg) {
+      const int32_t k1 = cx->rec_group_key[rg], k2 = cx->lig_group_key[lg];
+      const size_t at = static_cast<size_t>(rg) * n_lg + lg;
+      if (cx->res_key[k1].first == cx->res_key[k2].first) { identity = false; continue; }  // same chain (or key)
+      int32_t bit = -1;
+      if (cx->rg_of_key[k2] >= 0 && cx->lg_of_key[k1] >= 0)  // the mirror pair, if already numbered
+        bit = pb[static_cast<size_t>(cx->rg_of_key[k2]) * n_lg + cx->lg_of_key[k1]];
+      if (bit < 0) {
+        bit = static_cast<int32_t>(cx->dyn_keys.size());
+        cx->dyn_keys.push_back({std::min(k1, k2), std::max(k1, k2)});
+      }
+      pb[at] = bit;
+      identity = identity && bit == static_cast<int32_t>(at);
+    }
+  if (!identity) cx->pair_bit = pb;
+  // FP64 heavy atoms of both sides for the static key pairs (contact_cut)
+  for (int32_t k = 0; k < nkeys; ++k)
+    for (int32_t i : rres[k]) {
+      cx->rec_heavy_xyz.insert(cx->rec_heavy_xyz.end(), {cx->rec[i].x, cx->rec[i].y, cx->rec[i].z});
+      cx->rec_heavy_key.push_back(k);
+    }
+  for (int32_t k = 0; k < nkeys; ++k)
+    for (int32_t i : lres[k]) {
+      cx->lig_heavy_xyz.insert(cx->lig_heavy_xyz.end(), {cx->lig[i].x, cx->lig[i].y, cx->lig[i].z});
+      cx->lig_heavy_key.push_back(k);
+      cx->lig_heavy_slot.push_back(cx->lig_slot[i]);
+    }
+
   std::vector<float4> rxyz;
   std::vector<double> rx, ry, rz;
   std::vector<int32_t> rstart{0}, latom, lresi;
@@ -70,10 +131,9 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
     origin[0] = sx / n; origin[1] = sy / n; origin[2] = sz / n;
   }
   std::vector<float4> rsph;
-  for (auto& kv : rres) {
-    cx->rec_res_key.push_back(kv.first);
+  for (int32_t k : cx->rec_group_key) {
     const size_t first = rxyz.size();
-    for (int32_t i : kv.second) {
+    for (int32_t i : rres[k]) {
       const gdock_atom& a = cx->rec[i];
       rxyz.push_back(make_float4(static_cast<float>(a.x - origin[0]), static_cast<float>(a.y - origin[1]),
                                  static_cast<float>(a.z - origin[2]), 0.0f));
@@ -99,32 +159,22 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
   }
   // ligand heavy atoms in blocks of kLigBlock spatially compact atoms (a wave's slots)
   std::vector<int32_t> lheavy, res_of(cx->lig.size(), -1);
-  int32_t li = 0;
-  for (auto& kv : lres) {
-    cx->lig_res_key.push_back(kv.first);
-    for (int32_t i : kv.second) { lheavy.push_back(i); res_of[i] = li; }
-    ++li;
-  }
+  for (int32_t lg = 0; lg < n_lg; ++lg)
+    for (int32_t i : lres[cx->lig_group_key[lg]]) { lheavy.push_back(i); res_of[i] = lg; }
   bisect_order(cx->lig.data(), lheavy, 0, lheavy.size(), kLigBlock);
   for (int32_t i : lheavy) { latom.push_back(cx->lig_slot[i]); lresi.push_back(res_of[i]); }
   DevContacts& c = cx->con;
   c.n_rec_heavy = static_cast<int32_t>(rxyz.size());
   c.n_lig_heavy = static_cast<int32_t>(latom.size());
-  c.n_rec_res = static_cast<int32_t>(cx->rec_res_key.size());
-  c.n_lig_res = static_cast<int32_t>(cx->lig_res_key.size());
-  const int64_t bits = static_cast<int64_t>(c.n_rec_res) * c.n_lig_res;
-  c.words64 = static_cast<int32_t>((bits + 63) / 64);
-  if (contacts_lds_bytes(c) > 64 * 1024) {
-    cx->con_unsupported = "contacts: residue-pair bitset and residue masks exceed the 64 KiB workgroup budget";
-    c.words64 = 0;
-    return;
-  }
+  c.n_rec_res = n_rg;
+  c.n_lig_res = n_lg;
+  c.n_dyn = static_cast<int32_t>(cx->dyn_keys.size());
   c.supported = 1;
   const size_t begin = cx->contacts_layout_begin;
   const size_t o1 = L->add<float4>(rxyz.size()), o2 = L->add<double>(rx.size()), o3 = L->add<double>(ry.size()),
                o4 = L->add<double>(rz.size()), o5 = L->add<int32_t>(rstart.size()),
                o6 = L->add<int32_t>(latom.size()), o7 = L->add<int32_t>(lresi.size()),
-               o8 = L->add<float4>(rsph.size());
+               o8 = L->add<float4>(rsph.size()), o9 = L->add<int32_t>(cx->pair_bit.size());
   host->assign(L->size - begin, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) {
     if (bytes) std::memcpy(host->data() + (off - begin), src, bytes);
@@ -133,7 +183,8 @@ void build_contact_tables(gdock_complex* cx, Layout* L, std::vector<uint8_t>* ho
   put(o2, rx.data(), rx.size() * 8); put(o3, ry.data(), ry.size() * 8); put(o4, rz.data(), rz.size() * 8);
   put(o5, rstart.data(), rstart.size() * 4); put(o6, latom.data(), latom.size() * 4); put(o7, lresi.data(), lresi.size() * 4);
   put(o8, rsph.data(), rsph.size() * sizeof(float4));
-  cx->con_offsets = {o1, o2, o3, o4, o5, o6, o7, o8};
+  put(o9, cx->pair_bit.data(), cx->pair_bit.size() * 4);
+  cx->con_offsets = {o1, o2, o3, o4, o5, o6, o7, o8, o9};
 }
 
 void bind_contact_tables(gdock_complex* cx) {
@@ -149,6 +200,163 @@ void bind_contact_tables(gdock_complex* cx) {
   c.lig_atom = reinterpret_cast<const int32_t*>(m + o[5]);
   c.lig_res = reinterpret_cast<const int32_t*>(m + o[6]);
   c.rec_res_sphere = reinterpret_cast<const float4*>(m + o[7]);
+  c.pair_bit = cx->pair_bit.empty() ? nullptr : reinterpret_cast<const int32_t*>(m + o[8]);
+}
+
+// Pairs (i, j) of points (FP64 x, y, z triples) closer than `reach` in
+// each coordinate's cell neighbourhood, i < j, whose keys have different
+// chains: f(i, j, d2) with d2 = ((xi - xj)^2 + (yi - yj)^2) + (zi - zj)^2,
+// the operation order of clustering.rs:97.  Cells of side `reach` on a
+// sorted grid, so every pair within `reach` is visited exactly once.
+template <class F>
+void near_pairs(const std::vector<double>& xyz, const std::vector<int32_t>& key,
+                const std::vector<std::pair<char, int16_t>>& res_key, double reach, F&& f) {
+  const size_t n = key.size();
+  if (n < 2) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY};
+  for (size_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], xyz[3 * i + a]);
+  const double cell = std::max(reach, 0.5) * (1.0 + 1e-9) + 1e-9;  // pairs within `reach` sit in adjacent cells
+  struct Cell { int64_t x, y, z; };
+  std::vector<Cell> cc(n);
+  std::vector<int32_t> order(n);
+  for (size_t i = 0; i < n; ++i) {
+    cc[i] = {static_cast<int64_t>(std::floor((xyz[3 * i] - lo[0]) / cell)),
+             static_cast<int64_t>(std::floor((xyz[3 * i + 1] - lo[1]) / cell)),
+             static_cast<int64_t>(std::floor((xyz[3 * i + 2] - lo[2]) / cell))};
+    order[i] = static_cast<int32_t>(i);
+  }
+  auto less = [&](const Cell& a, const Cell& b) {
+    return a.x != b.x ? a.x < b.x : (a.y != b.y ? a.y < b.y : a.z < b.z);
+  };
+  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return less(cc[a], cc[b]) || (!less(cc[b], cc[a]) && a < b);
+  });
+  std::map<std::tuple<int64_t, int64_t, int64_t>, std::pair<size_t, size_t>> span;
+  for (size_t s = 0; s < n;) {
+    size_t e = s + 1;
+    const Cell& c0 = cc[order[s]];
+    while (e < n && !less(c0, cc[order[e]])) ++e;
+    span[{c0.x, c0.y, c0.z}] = {s, e};
+    s = e;
+  }
+  for (auto& kv : span) {
+    const auto [x, y, z] = kv.first;
+    for (int64_t dx = -1; dx <= 1; ++dx)
+      for (int64_t dy = -1; dy <= 1; ++dy)
+        for (int64_t dz = -1; dz <= 1; ++dz) {
+          const std::tuple<int64_t, int64_t, int64_t> nb{x + dx, y + dy, z + dz};
+          if (nb < kv.first) continue;  // each unordered cell pair once
+          auto it = span.find(nb);
+          if (it == span.end()) continue;
+          for (size_t p = kv.second.first; p < kv.second.second; ++p)
+            for (size_t q = (nb == kv.first ? p + 1 : it->second.first); q < it->second.second; ++q) {
+              const int32_t i = std::min(order[p], order[q]), j = std::max(order[p], order[q]);
+              if (res_key[key[i]].first == res_key[key[j]].first) continue;
+              const double ex = xyz[3 * i] - xyz[3 * j], ey = xyz[3 * i + 1] - xyz[3 * j + 1], ez = xyz[3 * i + 2] - xyz[3 * j + 2];
+              f(i, j, (ex * ex + ey * ey) + ez * ez);
+            }
+        }
+  }
+}
+
+// Window of the ligand x ligand decisions: an atom pair whose unmoved FP64
+// squared distance is within kLigPairWindow of the threshold is re-decided
+// per pose; the others are decided once, which is exact while the rounding
+// of the pose transform moves squared distances by less than the window.
+constexpr double kLigPairWindow = 1e-6;
+
+// Bound on |d2(posed, FP64) - d2(unmoved)| for a ligand atom pair at distance
+// at most d, when every posed coordinate is within e of the transform of the
+// unmoved atom under the (FP-valued, nearly orthogonal) rotation matrix.
+double lig_pair_error(double d, double e) {
+  const double u = 1.1102230246251565e-16;
+  const double dl = 2.0 * e + u * (d + 2.0 * e);  // error of one coordinate difference
+  return 2.0 * 1.7320508075688772 * d * dl + 3.0 * dl * dl + 4.0 * u * (d + 1.7320508075688772 * dl) * (d + 1.7320508075688772 * dl) +
+         1e-14 * d * d;
+}
+
+// The layout of the contact bits at squared-distance threshold d2 (cached on
+// the complex; callers hold cx->con_mu).
+const ContactCut& contact_cut(gdock_complex* cx, double d2) {
+  ContactCut& cut = cx->cut;
+  if (cut.valid && cut.d2 == d2) return cut;
+  cut = ContactCut{};
+  cut.d2 = d2;
+  cut.valid = true;
+  const int32_t n_dyn = cx->con.n_dyn, nkeys = static_cast<int32_t>(cx->res_key.size());
+  auto dyn_bit = [&](int32_t k1, int32_t k2) -> int32_t {  // bit of key pair {k1, k2} among [0, n_dyn), or -1
+    const int32_t n_lg = cx->con.n_lig_res;
+    for (int side = 0; side < 2; ++side) {
+      const int32_t a = side ? k2 : k1, b = side ? k1 : k2;
+      const int32_t rg = cx->rg_of_key[a], lg = cx->lg_of_key[b];
+      if (rg < 0 || lg < 0) continue;
+      const size_t at = static_cast<size_t>(rg) * n_lg + lg;
+      return cx->pair_bit.empty() ? static_cast<int32_t>(at) : cx->pair_bit[at];
+    }
+    return -1;
+  };
+  std::set<std::pair<int32_t, int32_t>> fixed_in;            // key pairs in contact in every pose
+  std::map<std::pair<int32_t, int32_t>, std::vector<std::pair<int32_t, int32_t>>> win;  // key pair -> ligand atom pairs
+  if (cx->rec_chains > 1)  // receptor x receptor: the receptor never moves, decided exactly
+    near_pairs(cx->rec_heavy_xyz, cx->rec_heavy_key, cx->res_key, std::sqrt(d2), [&](int32_t i, int32_t j, double e2) {
+      if (e2 <= d2) fixed_in.insert({std::min(cx->rec_heavy_key[i], cx->rec_heavy_key[j]), std::max(cx->rec_heavy_key[i], cx->rec_heavy_key[j])});
+    });
+  if (cx->lig_chains > 1) {
+    near_pairs(cx->lig_heavy_xyz, cx->lig_heavy_key, cx->res_key, std::sqrt(d2 + kLigPairWindow),
+               [&](int32_t i, int32_t j, double e2) {
+                 const std::pair<int32_t, int32_t> kp{std::min(cx->lig_heavy_key[i], cx->lig_heavy_key[j]),
+                                                      std::max(cx->lig_heavy_key[i], cx->lig_heavy_key[j])};
+                 if (e2 <= d2 - kLigPairWindow) fixed_in.insert(kp);
+                 else if (e2 <= d2 + kLigPairWindow) win[kp].push_back({i, j});
+               });
+    // the largest translation for which every ligand pair decision above holds
+    double A = 0.0, C = 0.0;
+    for (int a = 0; a < 3; ++a) C = std::max(C, std::fabs(cx->dev.com[a]));
+    for (size_t i = 0; i < cx->lig_heavy_key.size(); ++i)
+      for (int a = 0; a < 3; ++a) A = std::max(A, std::fabs(cx->lig_heavy_xyz[3 * i + a] - cx->dev.com[a]));
+    const double u = 1.1102230246251565e-16, dmax = std::sqrt(d2 + kLigPairWindow);
+    double D = 0.0;
+    if (lig_pair_error(dmax, 32.0 * u * (A + C)) < kLigPairWindow) {
+      double hi = 1e12;
+      for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (D + hi);
+        (lig_pair_error(dmax, 32.0 * u * (A + C + mid)) < kLigPairWindow ? D : hi) = mid;
+      }
+    } else {
+      D = -1.0;  // no translation is safe: the ligand itself is too large for the window
+    }
+    cut.max_translation = D;
+  } else {
+    cut.max_translation = INFINITY;
+  }
+  // bits beyond the receptor x ligand pairs, in key-pair order
+  std::set<std::pair<int32_t, int32_t>> extra;
+  for (auto& kp : fixed_in) if (dyn_bit(kp.first, kp.second) < 0) extra.insert(kp);
+  for (auto& kv : win) if (!fixed_in.count(kv.first) && dyn_bit(kv.first.first, kv.first.second) < 0) extra.insert(kv.first);
+  cut.extra_keys.assign(extra.begin(), extra.end());
+  std::map<std::pair<int32_t, int32_t>, int32_t> extra_bit;
+  for (size_t i = 0; i < cut.extra_keys.size(); ++i) extra_bit[cut.extra_keys[i]] = n_dyn + static_cast<int32_t>(i);
+  auto bit_of = [&](const std::pair<int32_t, int32_t>& kp) {
+    const int32_t b = dyn_bit(kp.first, kp.second);
+    return b >= 0 ? b : extra_bit.at(kp);
+  };
+  cut.n_bits = n_dyn + static_cast<int32_t>(cut.extra_keys.size());
+  cut.words64 = (cut.n_bits + 63) / 64;
+  cut.static_mask.assign(cut.words64, 0ull);
+  for (auto& kp : fixed_in) { const int32_t b = bit_of(kp); cut.static_mask[b >> 6] |= 1ull << (b & 63); }
+  for (auto& kv : win) {
+    if (fixed_in.count(kv.first)) continue;
+    const int32_t b = bit_of(kv.first);
+    for (auto& ij : kv.second)
+      cut.window.push_back(make_int4(cx->lig_heavy_slot[ij.first], cx->lig_heavy_slot[ij.second], b, 0));
+  }
+  (void)nkeys;
+  DevContacts probe = cx->con;
+  probe.words64 = cut.words64;
+  if (contacts_lds_bytes(probe) > 64 * 1024)
+    cut.err = "contacts: residue-pair bitset and residue masks exceed the 64 KiB workgroup budget";
+  return cut;
 }
 
 void release_fcc_scratch(gdock_ctx* ctx) { ctx->fcc.release(); }
@@ -246,14 +454,25 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
   uint8_t* my_mask = reinterpret_cast<uint8_t*>(reinterpret_cast<float*>(s_bits + words32) +
                                                 4 * kContactBoxFloats * kMaxAtomsPerLane) +
                      wave * ((ct.n_rec_res + 3) & ~3);
-  for (int i = tid; i < words32; i += 256) s_bits[i] = 0u;
+  const uint32_t* fixed_in = reinterpret_cast<const uint32_t*>(ct.static_mask);
+  for (int i = tid; i < words32; i += 256) s_bits[i] = fixed_in ? fixed_in[i] : 0u;
   const PoseT pc = pose_of(poses + static_cast<size_t>(kPoseDoubles) * pose);
   const double X = fmax(cx.rec_extent, cx.lig_extent + fmax(fabs(pc.dx), fmax(fabs(pc.dy), fabs(pc.dz))));
   float lo, hi;
   band_expanded(d2_incl, X, lo, hi);
   __syncthreads();
+  // Ligand x ligand atom pairs near the cutoff: the FP64 distance of the
+  // posed atoms, as clustering.rs:97-99 computes it.
+  for (int p = tid; p < ct.n_window; p += 256) {
+    const int4 wp = ct.window[p];
+    double ax, ay, az, bx, by, bz;
+    pose_atom(cx, pc, wp.x, ax, ay, az);
+    pose_atom(cx, pc, wp.y, bx, by, bz);
+    const double ex = ax - bx, ey = ay - by, ez = az - bz;
+    if ((ex * ex + ey * ey) + ez * ez <= d2_incl) atomicOr(&s_bits[wp.z >> 5], 1u << (wp.z & 31));
+  }
 
-  for (int base = 0; base < ct.n_lig_heavy; base += 256 * K) {
+  for (int base = 0; ct.n_dyn > 0 && base < ct.n_lig_heavy; base += 256 * K) {
     float lx[K], ly[K], lz[K], ll[K];
     int32_t lres[K];
 #pragma unroll
@@ -341,8 +560,9 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
 #pragma unroll
       for (int k = 0; k < K; ++k)
         if (hit[k]) {
-          const uint32_t bit = static_cast<uint32_t>(ri) * static_cast<uint32_t>(ct.n_lig_res) + static_cast<uint32_t>(lres[k]);
-          atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+          const int32_t at = ri * ct.n_lig_res + lres[k];
+          const int32_t bit = ct.pair_bit ? ct.pair_bit[at] : at;
+          if (bit >= 0) atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
         }
     }
   }
@@ -596,28 +816,38 @@ __global__ __launch_bounds__(kGreedyThreads) void greedy_kernel(const unsigned l
 }
 
 template <int K>
-hipError_t launch_contacts_k(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
-                             unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
+hipError_t launch_contacts_k(const gdock_complex* cx, const DevContacts& ct, const double* d_poses, int n,
+                             double d2_incl, unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
   const float reach = static_cast<float>(std::sqrt(std::max(d2_incl, 0.0))) + 0.01f;
-  hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), contacts_lds_bytes(cx->con), s, cx->dev,
-                     cx->con, d_poses, d2_incl, reach, d_bits, d_sizes);
+  hipLaunchKernelGGL(contacts_kernel<K>, dim3(n), dim3(256), contacts_lds_bytes(ct), s, cx->dev, ct, d_poses, d2_incl,
+                     reach, d_bits, d_sizes);
   return hipGetLastError();
 }
 
-hipError_t launch_contacts(const gdock_complex* cx, const double* d_poses, int n, double d2_incl,
+// Contact bitsets of n poses under the layout `cut` of threshold d2_incl;
+// d_static / d_window hold cut.static_mask / cut.window on the device.
+// Without any bit the sets are empty: sizes are cleared, no kernel runs.
+hipError_t launch_contacts(const gdock_complex* cx, const ContactCut& cut, const uint64_t* d_static,
+                           const int4* d_window, const double* d_poses, int n, double d2_incl,
                            unsigned long long* d_bits, int32_t* d_sizes, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  int k = (cx->con.n_lig_heavy + 255) / 256;
+  if (cut.words64 == 0) return hipMemsetAsync(d_sizes, 0, static_cast<size_t>(n) * 4, s);
+  DevContacts ct = cx->con;
+  ct.words64 = cut.words64;
+  ct.static_mask = d_static;
+  ct.window = d_window;
+  ct.n_window = static_cast<int32_t>(cut.window.size());
+  int k = (ct.n_lig_heavy + 255) / 256;
   k = k < 1 ? 1 : (k > 8 ? 8 : k);
   switch (k) {
-    case 1: return launch_contacts_k<1>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 2: return launch_contacts_k<2>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 3: return launch_contacts_k<3>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 4: return launch_contacts_k<4>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 5: return launch_contacts_k<5>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 6: return launch_contacts_k<6>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    case 7: return launch_contacts_k<7>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
-    default: return launch_contacts_k<8>(cx, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 1: return launch_contacts_k<1>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 2: return launch_contacts_k<2>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 3: return launch_contacts_k<3>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 4: return launch_contacts_k<4>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 5: return launch_contacts_k<5>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 6: return launch_contacts_k<6>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    case 7: return launch_contacts_k<7>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
+    default: return launch_contacts_k<8>(cx, ct, d_poses, n, d2_incl, d_bits, d_sizes, s);
   }
 }
 
@@ -647,53 +877,111 @@ using namespace gdock;
 
 extern "C" {
 
-int32_t gdock_contact_words(const gdock_complex* cx) { return cx && cx->con.supported ? cx->con.words64 : 0; }
-
-int32_t gdock_contact_candidates(const gdock_complex* cx) {
-  return cx && cx->con.supported ? cx->con.n_rec_res * cx->con.n_lig_res : 0;
+// Layout of cutoff `contact_distance` (callers hold cx->con_mu), or null
+// with the reason in *why.
+static const ContactCut* cut_of(gdock_complex* cx, double contact_distance, const char** why) {
+  if (!cx->con.supported) { *why = cx->con_unsupported.c_str(); return nullptr; }
+  if (!(contact_distance >= 0.0) || !std::isfinite(contact_distance)) { *why = "contacts: bad cutoff"; return nullptr; }
+  const ContactCut& cut = contact_cut(cx, contact_d2_threshold(contact_distance));
+  if (!cut.err.empty()) { *why = cut.err.c_str(); return nullptr; }
+  return &cut;
 }
 
-gdock_status gdock_contact_pair(const gdock_complex* cx, int32_t bit, char* chain_a, int16_t* res_a, char* chain_b,
-                                int16_t* res_b) {
-  if (!cx || !cx->con.supported || bit < 0 || bit >= cx->con.n_rec_res * cx->con.n_lig_res) return GDOCK_ERR_INVALID_ARG;
-  const auto& r = cx->rec_res_key[bit / cx->con.n_lig_res];
-  const auto& l = cx->lig_res_key[bit % cx->con.n_lig_res];
+int32_t gdock_contact_words(gdock_complex* cx, double contact_distance) {
+  if (!cx) return -1;
+  std::lock_guard<std::mutex> lk(cx->con_mu);
+  const char* why = nullptr;
+  const ContactCut* cut = cut_of(cx, contact_distance, &why);
+  return cut ? cut->words64 : -1;
+}
+
+int32_t gdock_contact_candidates(gdock_complex* cx, double contact_distance) {
+  if (!cx) return -1;
+  std::lock_guard<std::mutex> lk(cx->con_mu);
+  const char* why = nullptr;
+  const ContactCut* cut = cut_of(cx, contact_distance, &why);
+  return cut ? cut->n_bits : -1;
+}
+
+gdock_status gdock_contact_pair(gdock_complex* cx, double contact_distance, int32_t bit, char* chain_a, int16_t* res_a,
+                                char* chain_b, int16_t* res_b) {
+  if (!cx || !chain_a || !res_a || !chain_b || !res_b) return GDOCK_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(cx->con_mu);
+  const char* why = nullptr;
+  const ContactCut* cut = cut_of(cx, contact_distance, &why);
+  if (!cut || bit < 0 || bit >= cut->n_bits) return GDOCK_ERR_INVALID_ARG;
+  const int32_t n_dyn = cx->con.n_dyn;
+  const auto kp = bit < n_dyn ? cx->dyn_keys[bit] : cut->extra_keys[bit - n_dyn];
+  const auto& r = cx->res_key[kp.first];
+  const auto& l = cx->res_key[kp.second];
   const bool rf = r.first < l.first;  // smaller chain first (clustering.rs:101-105)
   *chain_a = rf ? r.first : l.first; *res_a = rf ? r.second : l.second;
   *chain_b = rf ? l.first : r.first; *res_b = rf ? l.second : r.second;
   return GDOCK_OK;
 }
 
-static gdock_status upload_poses(gdock_ctx* ctx, const double* genes, int32_t n, double* d_poses) {
+// Uploads the poses of n gene vectors; *max_shift gets their largest
+// translation component.
+static gdock_status upload_poses(gdock_ctx* ctx, const double* genes, int32_t n, double* d_poses, double* max_shift) {
   std::vector<double> poses(static_cast<size_t>(n) * kPoseDoubles);
   if (gdock_prepare_poses(genes, n, poses.data()) != GDOCK_OK) {
     set_error(ctx, "non-finite gene");
     return GDOCK_ERR_INVALID_ARG;
   }
+  double m = 0.0;
+  for (int32_t i = 0; i < n; ++i)
+    for (int a = 3; a < 6; ++a) m = std::max(m, std::fabs(poses[static_cast<size_t>(i) * kPoseDoubles + a]));
+  *max_shift = m;
   hipError_t e = hipMemcpyAsync(d_poses, poses.data(), poses.size() * 8, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // `poses` is freed on return
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "upload poses");
 }
 
+// Poses, the layout's static words and window pairs on the device.
+static gdock_status stage_contacts(gdock_ctx* ctx, const ContactCut& cut, const double* genes, int32_t n, char* m,
+                                   size_t o_p, size_t o_st, size_t o_w, const char* what) {
+  double shift = 0.0;
+  gdock_status st = upload_poses(ctx, genes, n, reinterpret_cast<double*>(m + o_p), &shift);
+  if (st != GDOCK_OK) return st;
+  if (!(shift <= cut.max_translation)) {
+    set_error(ctx, "%s: a pose translation of %g A exceeds the %g A within which the ligand's inter-chain contacts are exact",
+              what, shift, cut.max_translation);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSuccess;
+  if (!cut.static_mask.empty())
+    e = hipMemcpyAsync(m + o_st, cut.static_mask.data(), cut.static_mask.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && !cut.window.empty())
+    e = hipMemcpyAsync(m + o_w, cut.window.data(), cut.window.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, what);
+}
+
 gdock_status gdock_contacts(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n, double contact_distance,
                             uint64_t* bitsets, int32_t* set_sizes) {
-  if (!ctx || !cx || n < 0 || (n && (!genes || !bitsets))) return GDOCK_ERR_INVALID_ARG;
-  if (!cx->con.supported) { set_error(ctx, "%s", cx->con_unsupported.c_str()); return GDOCK_ERR_INVALID_ARG; }
-  if (!(contact_distance >= 0.0) || !std::isfinite(contact_distance)) { set_error(ctx, "contacts: bad cutoff"); return GDOCK_ERR_INVALID_ARG; }
+  if (!ctx || !cx || n < 0 || (n && !genes)) return GDOCK_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(cx->con_mu);
+  const char* why = nullptr;
+  const ContactCut* cut = cut_of(cx, contact_distance, &why);
+  if (!cut) { set_error(ctx, "%s", why); return GDOCK_ERR_INVALID_ARG; }
+  if (n && cut->words64 && !bitsets) return GDOCK_ERR_INVALID_ARG;
   if (n == 0) return GDOCK_OK;
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "contacts: set device");
+  const int W64 = cut->words64;
   Layout L;
   const size_t o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
-               o_b = L.add<uint64_t>(static_cast<size_t>(n) * cx->con.words64), o_s = L.add<int32_t>(n);
+               o_b = L.add<uint64_t>(static_cast<size_t>(n) * W64), o_s = L.add<int32_t>(n),
+               o_st = L.add<uint64_t>(W64), o_w = L.add<int4>(cut->window.size());
   if (!ctx->fcc.reserve(L.size, &e)) return hip_fail(ctx, e, "contacts: scratch");
   char* m = static_cast<char*>(ctx->fcc.ptr);
-  gdock_status st = upload_poses(ctx, genes, n, reinterpret_cast<double*>(m + o_p));
+  gdock_status st = stage_contacts(ctx, *cut, genes, n, m, o_p, o_st, o_w, "contacts");
   if (st != GDOCK_OK) return st;
-  e = launch_contacts(cx, reinterpret_cast<double*>(m + o_p), n, contact_d2_threshold(contact_distance),
+  e = launch_contacts(cx, *cut, cut->static_mask.empty() ? nullptr : reinterpret_cast<const uint64_t*>(m + o_st),
+                      reinterpret_cast<const int4*>(m + o_w), reinterpret_cast<double*>(m + o_p), n, cut->d2,
                       reinterpret_cast<unsigned long long*>(m + o_b), reinterpret_cast<int32_t*>(m + o_s), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "contacts: launch");
-  e = hipMemcpyAsync(bitsets, m + o_b, static_cast<size_t>(n) * cx->con.words64 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (W64 > 0)
+    e = hipMemcpyAsync(bitsets, m + o_b, static_cast<size_t>(n) * W64 * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && set_sizes) e = hipMemcpyAsync(set_sizes, m + o_s, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "contacts: download");
@@ -705,20 +993,24 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   if (!ctx || !cx || !cfg || !n_clusters || n < 0 || (n && (!genes || !out || !members))) return GDOCK_ERR_INVALID_ARG;
   *n_clusters = 0;
   if (n == 0) return GDOCK_OK;  // clustering.rs:269-271
-  if (!cx->con.supported) { set_error(ctx, "%s", cx->con_unsupported.c_str()); return GDOCK_ERR_INVALID_ARG; }
   if (n > kGreedyMaxN) { set_error(ctx, "cluster_structures: %d structures exceeds the supported %d", n, kGreedyMaxN); return GDOCK_ERR_INVALID_ARG; }
   if (!std::isfinite(cfg->contact_distance) || cfg->contact_distance < 0.0 || !std::isfinite(cfg->fcc_cutoff) ||
       !std::isfinite(cfg->strictness)) {
     set_error(ctx, "cluster_structures: bad config");
     return GDOCK_ERR_INVALID_ARG;
   }
+  std::lock_guard<std::mutex> lk(cx->con_mu);
+  const char* why = nullptr;
+  const ContactCut* cut = cut_of(cx, cfg->contact_distance, &why);
+  if (!cut) { set_error(ctx, "%s", why); return GDOCK_ERR_INVALID_ARG; }
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: set device");
-  const int W64 = cx->con.words64, W32 = 2 * W64;
+  const int W64 = cut->words64, W32 = 2 * W64;
   const int n_pad = (n + 63) & ~63;
   const int nwords = (n + 63) / 64;
-  const int max_set = cx->con.n_rec_res * cx->con.n_lig_res;
+  const int max_set = cut->n_bits;
   Layout L;
+  const size_t o_st = L.add<uint64_t>(W64), o_w = L.add<int4>(cut->window.size());
   const size_t o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
                o_b = L.add<uint64_t>(static_cast<size_t>(n) * W64), o_s = L.add<int32_t>(n_pad),
                o_u = L.add<uint32_t>(W32), o_col = L.add<int32_t>(W32),
@@ -730,9 +1022,10 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   if (!ctx->fcc.reserve(L.size, &e)) return hip_fail(ctx, e, "cluster: scratch");
   char* m = static_cast<char*>(ctx->fcc.ptr);
   auto P = [&](size_t o) { return static_cast<void*>(m + o); };
-  gdock_status st = upload_poses(ctx, genes, n, static_cast<double*>(P(o_p)));
+  gdock_status st = stage_contacts(ctx, *cut, genes, n, m, o_p, o_st, o_w, "cluster_structures");
   if (st != GDOCK_OK) return st;
-  e = launch_contacts(cx, static_cast<double*>(P(o_p)), n, contact_d2_threshold(cfg->contact_distance),
+  e = launch_contacts(cx, *cut, cut->static_mask.empty() ? nullptr : static_cast<const uint64_t*>(P(o_st)),
+                      static_cast<const int4*>(P(o_w)), static_cast<double*>(P(o_p)), n, cut->d2,
                       static_cast<unsigned long long*>(P(o_b)), static_cast<int32_t*>(P(o_s)), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: contacts");
   // Integer overlap minima per set size (host), computed while the device
@@ -740,19 +1033,22 @@ gdock_status gdock_cluster_structures(gdock_ctx* ctx, gdock_complex* cx, const d
   std::vector<int32_t> ca, cb;
   fcc_minima(cfg->fcc_cutoff, max_set, &ca);
   fcc_minima(cfg->fcc_cutoff * cfg->strictness, max_set, &cb);
-  e = hipMemsetAsync(P(o_u), 0, W32 * 4, ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
-  hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256, (n + kUnionPoses - 1) / kUnionPoses), dim3(256), 0,
-                     ctx->stream, static_cast<const uint32_t*>(P(o_b)), n, W32, static_cast<uint32_t*>(P(o_u)));
+  // Words set in any pose (none without candidate bits: every set is empty).
   std::vector<uint32_t> uni(W32);
-  e = hipMemcpyAsync(uni.data(), P(o_u), W32 * 4, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
+  if (W32 > 0) {
+    e = hipMemsetAsync(P(o_u), 0, W32 * 4, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
+    hipLaunchKernelGGL(union_kernel, dim3((W32 + 255) / 256, (n + kUnionPoses - 1) / kUnionPoses), dim3(256), 0,
+                       ctx->stream, static_cast<const uint32_t*>(P(o_b)), n, W32, static_cast<uint32_t*>(P(o_u)));
+    e = hipMemcpyAsync(uni.data(), P(o_u), W32 * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cluster: union");
+  }
   std::vector<int32_t> col;
   for (int w = 0; w < W32; ++w) if (uni[w]) col.push_back(w);
   const int wc = static_cast<int>(col.size());
   const int zero_ok = (0.0 >= cfg->fcc_cutoff && 0.0 >= cfg->fcc_cutoff * cfg->strictness) ? 1 : 0;
-  e = hipMemcpyAsync(P(o_col), col.data(), std::max(1, wc) * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (wc > 0) e = hipMemcpyAsync(P(o_col), col.data(), wc * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(P(o_ca), ca.data(), ca.size() * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(P(o_cb), cb.data(), cb.size() * 4, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cluster: tables");

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <string_view>
 #include <vector>
@@ -20,19 +21,48 @@ struct DeviceArena {
   void release();
 };
 
-// Device tables of the inter-chain residue contact candidates of a complex.
+// Device tables of the inter-chain residue contacts of a complex.
+//
+// Residue keys are (chain, resseq) over the heavy atoms of receptor and
+// ligand together, so a key may hold receptor atoms (a receptor group),
+// ligand atoms (a ligand group) or both.  The contacts of a pose are the key
+// pairs of different chains with an atom pair within the cutoff: receptor x
+// ligand atom pairs move with the pose (the kernel), receptor x receptor
+// pairs never do, ligand x ligand pairs only by the rounding of the pose
+// transform (decided on the host, except pairs within a rounding window of
+// the cutoff, which the kernel re-decides per pose).
 struct DevContacts {
-  const float4* rec_c4;       // receptor heavy atoms grouped by residue: {-2x, -2y, -2z, x^2+y^2+z^2} of the FP32 origin-frame coordinates
+  const float4* rec_c4;       // receptor heavy atoms grouped by receptor group: {-2x, -2y, -2z, x^2+y^2+z^2} of the FP32 origin-frame coordinates
   const double* rec_x;        // FP64 coordinates of the same atoms
   const double* rec_y;
   const double* rec_z;
   const int32_t* rec_res_start;  // n_rec_res + 1
   const int32_t* lig_atom;    // ligand heavy atom -> ligand atom index
-  const int32_t* lig_res;     // ligand heavy atom -> ligand residue index
-  const float4* rec_res_sphere;  // per receptor residue: bounding sphere of its heavy atoms (FP32, origin frame)
+  const int32_t* lig_res;     // ligand heavy atom -> ligand group index
+  const float4* rec_res_sphere;  // per receptor group: bounding sphere of its heavy atoms (FP32, origin frame)
+  const int32_t* pair_bit;    // (receptor group * n_lig_res + ligand group) -> bit, -1: same chain; null: the identity
+  const uint64_t* static_mask;  // per call: bits set in every pose (words64), or null
+  const int4* window;         // per call: {ligand atom a, ligand atom b, bit, 0} pairs re-decided per pose
+  int32_t n_window;
   int32_t n_rec_heavy, n_lig_heavy, n_rec_res, n_lig_res;
-  int32_t words64;            // uint64 words per pose bitset (n_rec_res * n_lig_res bits)
-  int32_t supported;          // 0: topology outside the device contact path
+  int32_t n_dyn;              // bits of receptor x ligand group pairs (0: no such candidate)
+  int32_t words64;            // per call: uint64 words per pose bitset
+  int32_t supported;          // 0: tables not built
+};
+
+// Contact bit layout of one cutoff: bits [0, n_dyn) are the receptor x
+// ligand key pairs (cutoff independent), then key pairs that only receptor x
+// receptor or ligand x ligand atom pairs can bring into contact at this
+// cutoff.
+struct ContactCut {
+  bool valid = false;
+  double d2 = -1.0;                                   // FP64 squared-distance threshold it was built for
+  int32_t n_bits = 0, words64 = 0;
+  std::vector<std::pair<int32_t, int32_t>> extra_keys;  // key pairs of bits n_dyn..
+  std::vector<uint64_t> static_mask;                  // bits set in every pose
+  std::vector<int4> window;                           // ligand atom pairs re-decided per pose
+  double max_translation = 0.0;                       // largest pose translation for which the ligand pairs' decisions hold
+  std::string err;                                    // non-empty: this cutoff is not supported
 };
 
 }  // namespace gdock
@@ -60,8 +90,18 @@ struct gdock_complex {
   // contact tables (host)
   size_t contacts_layout_begin = 0;
   std::vector<size_t> con_offsets;
-  std::vector<std::pair<char, int16_t>> rec_res_key, lig_res_key;
+  std::vector<std::pair<char, int16_t>> res_key;        // residue keys, ascending
+  std::vector<int32_t> rec_group_key, lig_group_key;    // group -> key
+  std::vector<int32_t> rg_of_key, lg_of_key;            // key -> group or -1
+  std::vector<int32_t> pair_bit;                        // host copy (empty: the identity)
+  std::vector<std::pair<int32_t, int32_t>> dyn_keys;    // bit < n_dyn -> key pair
+  std::vector<double> rec_heavy_xyz, lig_heavy_xyz;     // FP64 coordinates (x, y, z) of heavy atoms
+  std::vector<int32_t> rec_heavy_key, lig_heavy_key;    // their keys
+  std::vector<int32_t> lig_heavy_slot;                  // ligand heavy atom -> device ligand index
+  int32_t rec_chains = 0, lig_chains = 0;               // distinct chains among receptor / ligand heavy atoms
   std::string con_unsupported;
+  std::mutex con_mu;                                    // guards `cut`
+  gdock::ContactCut cut;                                // layout of the last cutoff used
 };
 
 namespace gdock {

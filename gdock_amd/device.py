@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 from . import constants
-from ._lib import (ChromosomeC, ClusterResultC, ClusteringConfigC, Weights, check, lib, ptr)
+from ._lib import (ChromosomeC, ClusterResultC, ClusteringConfigC, GdockError, Weights, check, lib, ptr)
 from .structure import Molecule
 
 _contexts: dict[int, "Context"] = {}
@@ -109,34 +109,38 @@ class Complex:
             c.restraint_penalty = a.restraint_penalty
 
     # -- FCC -----------------------------------------------------------------
-    @property
-    def contact_words(self) -> int:
-        return lib().gdock_contact_words(self.handle)
+    def contact_words(self, contact_distance: float = 5.0) -> int:
+        """uint64 words of a pose's contact bitset at this cutoff."""
+        w = lib().gdock_contact_words(self.handle, contact_distance)
+        if w < 0:
+            raise GdockError(f"contacts unsupported at cutoff {contact_distance} for this complex")
+        return w
 
-    def contact_pair(self, bit: int) -> tuple[str, int, str, int]:
+    def contact_pair(self, bit: int, contact_distance: float = 5.0) -> tuple[str, int, str, int]:
         ca, cb = C.create_string_buffer(1), C.create_string_buffer(1)
         ra, rb = C.c_int16(), C.c_int16()
-        check(lib().gdock_contact_pair(self.handle, bit, ca, C.byref(ra), cb, C.byref(rb)), what="contact_pair")
+        check(lib().gdock_contact_pair(self.handle, contact_distance, bit, ca, C.byref(ra), cb, C.byref(rb)),
+              what="contact_pair")
         return ca.raw.decode(), ra.value, cb.raw.decode(), rb.value
 
     def contacts(self, genes: np.ndarray, contact_distance: float = 5.0):
         """(bitsets (n, words) uint64, set sizes (n,))."""
         g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
-        w = self.contact_words
+        w = self.contact_words(contact_distance)
         bits = np.zeros((len(g), max(w, 1)), np.uint64)
         sizes = np.zeros(len(g), np.int32)
         check(lib().gdock_contacts(self.ctx.handle, self.handle, ptr(g), len(g), contact_distance,
                                    ptr(bits, C.c_uint64), ptr(sizes, C.c_int32)), self.ctx.handle, "gdock_contacts")
         return bits[:, :w], sizes
 
-    def contact_set(self, bitset: np.ndarray) -> set[str]:
-        """Decode one pose's bitset into the reference's contact strings."""
+    def contact_set(self, bitset: np.ndarray, contact_distance: float = 5.0) -> set[str]:
+        """Decode one pose's bitset (at this cutoff) into the reference's contact strings."""
         out = set()
         for wi, word in enumerate(bitset.tolist()):
             while word:
                 b = (word & -word).bit_length() - 1
                 word &= word - 1
-                ca, ra, cb, rb = self.contact_pair(wi * 64 + b)
+                ca, ra, cb, rb = self.contact_pair(wi * 64 + b, contact_distance)
                 out.add(f"{ca} {ra} {cb} {rb}")
         return out
 

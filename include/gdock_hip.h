@@ -346,13 +346,23 @@ gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out);
 
 /* ---- FCC clustering ---------------------------------------------------------- */
 /* Contact bitsets (clustering::calculate_contacts, src/clustering.rs:62-115) of
- * the complexes receptor + (ligand moved by genes i).  Bit b of a pose's set
- * stands for the residue pair gdock_contact_pair(cx, b).  bitsets: n x
- * gdock_contact_words(cx) uint64; set_sizes (optional): n. */
-int32_t gdock_contact_words(const gdock_complex* cx);
-int32_t gdock_contact_candidates(const gdock_complex* cx);
-gdock_status gdock_contact_pair(const gdock_complex* cx, int32_t bit, char* chain_a, int16_t* res_a,
-                                char* chain_b, int16_t* res_b);
+ * the complexes receptor + (ligand moved by genes i), for any chain layout:
+ * residues are (chain, resseq) keys over the heavy atoms of both molecules,
+ * and every key pair of different chains is a candidate (receptor-receptor
+ * and ligand-ligand pairs included; a chain ID shared by receptor and
+ * ligand merges their residues of equal number, as the reference does).
+ * Bit b of a pose's set stands for the residue pair
+ * gdock_contact_pair(cx, contact_distance, b).  The bit layout depends on the
+ * cutoff: gdock_contact_words gives the uint64 words per pose (-1: cutoff not
+ * supported), gdock_contact_candidates the bits.  bitsets: n x words uint64;
+ * set_sizes (optional): n.  With a ligand of several chains, poses whose
+ * translation exceeds a bound (about 10^6 A for a 5 A cutoff) are rejected:
+ * beyond it the rounding of the pose transform could change a ligand-ligand
+ * decision that the library takes once per cutoff. */
+int32_t gdock_contact_words(gdock_complex* cx, double contact_distance);
+int32_t gdock_contact_candidates(gdock_complex* cx, double contact_distance);
+gdock_status gdock_contact_pair(gdock_complex* cx, double contact_distance, int32_t bit, char* chain_a,
+                                int16_t* res_a, char* chain_b, int16_t* res_b);
 gdock_status gdock_contacts(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
                             double contact_distance, uint64_t* bitsets, int32_t* set_sizes);
 

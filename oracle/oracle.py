@@ -50,58 +50,57 @@ def load(path) -> C.CDLL:
     path = str(path)
     if path in _loaded:
         return _loaded[path]
-    if True:
-        L = C.CDLL(path)
-        sig = {
-            "or_pdb_count_models": (C.c_int, [C.c_char_p]),
-            "or_pdb_model": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int]),
-            "or_position_ligand": (None, [P, C.c_int, P, C.c_int, P]),
-            "or_apply_genes": (None, [P, C.c_int, D, P]),
-            "or_center_of_mass": (None, [P, C.c_int, D]),
-            "or_vdw_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
-            "or_elec_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
-            "or_desolv_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
-            "or_air_energy": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
-            "or_satisfaction_ratio": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
-            "or_restraints_resolved": (C.c_int, [P, C.c_int, P, C.c_int, I32, C.c_int]),
-            "or_score_poses": (None, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_int, D, C.c_int, C.c_int, D, I32]),
-            "or_burial_counts": (None, [P, C.c_int, P, C.c_int, D, I32, I32]),
-            "or_contacts": (C.c_int, [P, C.c_int, C.c_double, I32, C.c_int]),
-            "or_calculate_fcc": (None, [I32, C.c_int, I32, C.c_int, D]),
-            "or_cluster_elements": (C.c_int, [C.POINTER(C.c_uint8), C.c_int, C.c_int, I64, I64, I64]),
-            "or_cluster_poses": (C.c_int, [P, C.c_int, P, C.c_int, D, C.c_int, D, C.c_int, I64, I64, I64, I32]),
-            "or_ga_run": (P, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]),
-            "or_ga_free": (None, [P]),
-            "or_ga_run_dockq": (P, [P, C.c_int, P, C.c_int, P, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int]),
-            "or_ga_generations": (C.c_int, [P]),
-            "or_ga_converged_at": (C.c_int, [P]),
-            "or_ga_trace": (None, [P, D, D, I32]),
-            "or_ga_hof": (C.c_int, [P, D, D, I32]),
-            "or_ga_clusters": (C.c_int, [P, I64, I64]),
-            "or_ga_selected": (C.c_int, [P, I64, I64]),
-            "or_ga_final_population": (C.c_int, [P, D]),
-            "or_hof_new": (P, [C.c_int]),
-            "or_hof_try_add": (C.c_int, [P, D, C.c_double, C.c_int]),
-            "or_hof_entries": (C.c_int, [P, D]),
-            "or_hof_angular_difference": (C.c_double, [C.c_double, C.c_double]),
-            "or_hof_free": (None, [P]),
-            "or_rng_stream": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
-            "or_evolve": (None, [D, D, C.c_int, C.c_uint64, D, D]),
-            "or_evaluator_new": (P, [P, C.c_int, P, C.c_int]),
-            "or_evaluator_free": (None, [P]),
-            "or_evaluator_info": (None, [P, I32, I32, I32, C.POINTER(C.c_int16), C.POINTER(C.c_int16)]),
-            "or_calc_metrics": (None, [P, P, C.c_int, C.c_int, D]),
-            "or_eval_contacts": (C.c_int, [P, C.c_int, P, C.c_int]),
-            "or_evolve_scripted": (C.c_int, [D, D, C.c_int, D, C.c_int, D, D, I32, D, C.c_int]),
-            "or_ga_fitness_sequence": (C.c_int, [C.c_int, C.c_int, D, C.c_int, C.c_uint64, I32]),
-            "or_select_models": (C.c_int, [D, C.c_int, I64, I64, C.c_int, I64, I64]),
-            "or_hof_add_population": (None, [P, D, D, C.c_int, C.c_int]),
-            "or_hof_genes": (C.c_int, [P, D]),
-        }
-        for k, (r, a) in sig.items():
-            f = getattr(L, k)
-            f.restype, f.argtypes = r, a
-        _loaded[path] = L
+    L = C.CDLL(path)
+    sig = {
+        "or_pdb_count_models": (C.c_int, [C.c_char_p]),
+        "or_pdb_model": (C.c_int, [C.c_char_p, C.c_int, P, C.c_int]),
+        "or_position_ligand": (None, [P, C.c_int, P, C.c_int, P]),
+        "or_apply_genes": (None, [P, C.c_int, D, P]),
+        "or_center_of_mass": (None, [P, C.c_int, D]),
+        "or_vdw_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+        "or_elec_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+        "or_desolv_energy": (C.c_double, [P, C.c_int, P, C.c_int]),
+        "or_air_energy": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+        "or_satisfaction_ratio": (C.c_double, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+        "or_restraints_resolved": (C.c_int, [P, C.c_int, P, C.c_int, I32, C.c_int]),
+        "or_score_poses": (None, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_int, D, C.c_int, C.c_int, D, I32]),
+        "or_burial_counts": (None, [P, C.c_int, P, C.c_int, D, I32, I32]),
+        "or_contacts": (C.c_int, [P, C.c_int, C.c_double, I32, C.c_int]),
+        "or_calculate_fcc": (None, [I32, C.c_int, I32, C.c_int, D]),
+        "or_cluster_elements": (C.c_int, [C.POINTER(C.c_uint8), C.c_int, C.c_int, I64, I64, I64]),
+        "or_cluster_poses": (C.c_int, [P, C.c_int, P, C.c_int, D, C.c_int, D, C.c_int, I64, I64, I64, I32]),
+        "or_ga_run": (P, [P, C.c_int, P, C.c_int, I32, C.c_int, D, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "or_ga_free": (None, [P]),
+        "or_ga_run_dockq": (P, [P, C.c_int, P, C.c_int, P, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int]),
+        "or_ga_generations": (C.c_int, [P]),
+        "or_ga_converged_at": (C.c_int, [P]),
+        "or_ga_trace": (None, [P, D, D, I32]),
+        "or_ga_hof": (C.c_int, [P, D, D, I32]),
+        "or_ga_clusters": (C.c_int, [P, I64, I64]),
+        "or_ga_selected": (C.c_int, [P, I64, I64]),
+        "or_ga_final_population": (C.c_int, [P, D]),
+        "or_hof_new": (P, [C.c_int]),
+        "or_hof_try_add": (C.c_int, [P, D, C.c_double, C.c_int]),
+        "or_hof_entries": (C.c_int, [P, D]),
+        "or_hof_angular_difference": (C.c_double, [C.c_double, C.c_double]),
+        "or_hof_free": (None, [P]),
+        "or_rng_stream": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
+        "or_evolve": (None, [D, D, C.c_int, C.c_uint64, D, D]),
+        "or_evaluator_new": (P, [P, C.c_int, P, C.c_int]),
+        "or_evaluator_free": (None, [P]),
+        "or_evaluator_info": (None, [P, I32, I32, I32, C.POINTER(C.c_int16), C.POINTER(C.c_int16)]),
+        "or_calc_metrics": (None, [P, P, C.c_int, C.c_int, D]),
+        "or_eval_contacts": (C.c_int, [P, C.c_int, P, C.c_int]),
+        "or_evolve_scripted": (C.c_int, [D, D, C.c_int, D, C.c_int, D, D, I32, D, C.c_int]),
+        "or_ga_fitness_sequence": (C.c_int, [C.c_int, C.c_int, D, C.c_int, C.c_uint64, I32]),
+        "or_select_models": (C.c_int, [D, C.c_int, I64, I64, C.c_int, I64, I64]),
+        "or_hof_add_population": (None, [P, D, D, C.c_int, C.c_int]),
+        "or_hof_genes": (C.c_int, [P, D]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(L, k)
+        f.restype, f.argtypes = r, a
+    _loaded[path] = L
     return L
 
 

@@ -69,3 +69,51 @@ def test_atom(x, y, z, element="C", charge=0.0, serial=1, name="CA", resseq=1, c
 def mol(*atoms):
     from gdock_amd._lib import ATOM_DTYPE
     return np.concatenate(atoms) if atoms else np.zeros(0, ATOM_DTYPE)
+
+
+def relabel(atoms: np.ndarray, rule) -> np.ndarray:
+    """Copy of atom records with (chainid, resseq) replaced by
+    rule(chain, resseq) -> (chain, resseq)."""
+    out = atoms.copy()
+    for i in range(len(out)):
+        c, r = rule(out["chainid"][i].decode(), int(out["resseq"][i]))
+        out["chainid"][i] = c.encode()
+        out["resseq"][i] = r
+    return out
+
+
+def _same(c, r):
+    return c, r
+
+
+# Chain layouts of 2oob (receptor chain A, residues 929-972; ligand chain B,
+# residues 1-72) as (receptor rule, ligand rule).  clustering.rs:62-115 keys
+# residues by (chain, resseq) over the whole complex, so each layout brings
+# in other candidate pairs: receptor-receptor, ligand-ligand, none at all
+# (one chain ID for both), or residues holding receptor and ligand atoms.
+TOPOLOGIES = {
+    "receptor_two_chains": (lambda c, r: ("C", r) if r >= 951 else (c, r), _same),
+    "shared_chain_id": (_same, lambda c, r: ("A", r)),
+    "ligand_two_chains": (_same, lambda c, r: ("D", r) if r >= 37 else (c, r)),
+    "both_two_chains": (lambda c, r: ("C", r) if r >= 951 else (c, r), lambda c, r: ("D", r) if r >= 37 else (c, r)),
+    # ligand residues 1-36 -> (A, 929-964), 37-72 -> (C, 951-986): keys of
+    # both chains hold receptor and ligand atoms
+    "merged_keys": (lambda c, r: ("C", r) if r >= 951 else (c, r),
+                    lambda c, r: ("A", r + 928) if r <= 36 else ("C", r + 914)),
+}
+
+
+def contacts_brute_force(atoms: np.ndarray, cutoff: float) -> set[str]:
+    """clustering.rs:62-115 over every heavy atom pair: residue keys (chain,
+    resseq), pairs of different chains, distance <= cutoff, smaller chain
+    first."""
+    heavy = atoms[atoms["element"] != b"H"]
+    xyz = np.stack([heavy["x"], heavy["y"], heavy["z"]], 1)
+    d = np.sqrt(((xyz[:, None] - xyz[None]) ** 2).sum(-1))
+    ch = heavy["chainid"]
+    ii, jj = np.nonzero((d <= cutoff) & (ch[:, None] != ch[None]))
+    out = set()
+    for i, j in zip(ii.tolist(), jj.tolist()):
+        a, b = (i, j) if ch[i] < ch[j] else (j, i)
+        out.add(f"{ch[a].decode()} {heavy['resseq'][a]} {ch[b].decode()} {heavy['resseq'][b]}")
+    return out

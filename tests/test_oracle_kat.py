@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle.oracle as orc
-from helpers import RESTRAINTS_2OOB, mol, read_text, test_atom as A
+from helpers import RESTRAINTS_2OOB, TOPOLOGIES, contacts_brute_force, mol, read_text, relabel, test_atom as A
 
 PI = math.pi
 
@@ -225,6 +225,25 @@ def test_contacts_brute_force_and_monotone():
         brute.add(f"{heavy['chainid'][a].decode()} {heavy['resseq'][a]} {heavy['chainid'][b].decode()} {heavy['resseq'][b]}")
     assert orc.contacts(cplx, 5.0) == brute
     assert orc.contacts(cplx, 4.0) <= orc.contacts(cplx, 5.0) <= orc.contacts(cplx, 6.0)
+
+
+@pytest.mark.parametrize("name", sorted(TOPOLOGIES))
+def test_contacts_chain_layouts_brute_force(name):
+    """Every chain layout (two receptor chains, two ligand chains, one chain ID
+    for both, residues shared by receptor and ligand): the oracle's contacts
+    equal the all-atom-pair transcription at two cutoffs."""
+    rr, lr = TOPOLOGIES[name]
+    rec = relabel(orc.read_pdb_text(read_text("2oob_A.pdb"))[0], rr)
+    lig = relabel(orc.read_pdb_text(read_text("2oob_B.pdb"))[0], lr)
+    for genes in (np.zeros(6), np.array([0.3, 1.1, 2.0, 4.0, -3.0, 1.5])):
+        cplx = np.concatenate([rec, orc.apply_genes(lig, genes)])
+        for cutoff in (3.5, 5.0):
+            got = orc.contacts(cplx, cutoff)
+            assert got == contacts_brute_force(cplx, cutoff), (name, cutoff)
+            if name == "shared_chain_id":
+                assert got == set()
+            else:
+                assert got
 
 
 def _greedy_transcription(nb, min_size):
