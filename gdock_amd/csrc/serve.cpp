@@ -81,6 +81,9 @@ struct gdock_server {
   std::mutex mu;
   std::condition_variable cv;      // worker: new work or stop
   std::condition_variable done_cv; // waiters: a job finished
+  std::condition_variable idle_cv; // destroy: the last waiter left
+  int waiters = 0;                 // threads inside gdock_server_wait
+  bool closing = false;            // destroy has begun: no new waits
   std::deque<Job*> queue;
   std::map<int64_t, std::unique_ptr<Job>> jobs;
   std::vector<Job*> slots;
@@ -506,9 +509,24 @@ void gdock_server_destroy(gdock_server* s) {
   }
   s->fin_cv.notify_all();
   if (s->finisher.joinable()) s->finisher.join();
-  for (auto& kv : s->jobs) {
-    if (kv.second->cx) gdock_complex_destroy(s->ctx, kv.second->cx);
-    if (kv.second->split_mem) (void)hipFree(kv.second->split_mem);
+  {  // every job still queued or running ends as failed; waiters wake, read it and leave
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->closing = true;
+    for (auto& kv : s->jobs) {
+      Job* j = kv.second.get();
+      if (j->cx) gdock_complex_destroy(s->ctx, j->cx);
+      if (j->split_mem) (void)hipFree(j->split_mem);
+      j->cx = nullptr;
+      j->split_mem = nullptr;
+      if (j->state != kFinished && j->state != kFailed) {
+        j->state = kFailed;
+        j->error = GDOCK_ERR_NOT_FOUND;
+        j->message = "server destroyed before the job ended";
+        j->result.reset();
+      }
+    }
+    s->done_cv.notify_all();
+    s->idle_cv.wait(lk, [s] { return s->waiters == 0; });
   }
   if (s->ctx) (void)hipSetDevice(s->ctx->device);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -557,15 +575,23 @@ gdock_status gdock_server_wait(gdock_server* s, int64_t job_id, double timeout_s
                                gdock_job_stats* stats) {
   if (!s) return GDOCK_ERR_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
+  if (s->closing) return GDOCK_ERR_NOT_FOUND;
+  if (s->jobs.find(job_id) == s->jobs.end()) return GDOCK_ERR_NOT_FOUND;
+  auto finished = [s, job_id] {  // ended, or taken by another waiter of the same id
+    auto f = s->jobs.find(job_id);
+    return f == s->jobs.end() || f->second->state == kFinished || f->second->state == kFailed;
+  };
+  ++s->waiters;
+  const bool ended = timeout_s < 0 ? (s->done_cv.wait(lk, finished), true)
+                                   : s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
+  struct Leave {  // the last waiter out lets a pending destroy go on
+    gdock_server* s;
+    ~Leave() { if (--s->waiters == 0 && s->closing) s->idle_cv.notify_all(); }
+  } leave{s};
+  if (!ended) return GDOCK_ERR_BUSY;
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end()) return GDOCK_ERR_NOT_FOUND;
   Job* j = it->second.get();
-  auto finished = [j] { return j->state == kFinished || j->state == kFailed; };
-  if (timeout_s < 0) {
-    s->done_cv.wait(lk, finished);
-  } else if (!s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished)) {
-    return GDOCK_ERR_BUSY;
-  }
   if (stats) {
     stats->submit_s = j->t_submit;
     stats->admit_s = j->t_admit;

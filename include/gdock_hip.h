@@ -330,7 +330,10 @@ typedef struct gdock_server_stats_t {
 typedef struct gdock_server gdock_server;
 gdock_status gdock_server_default_config(gdock_server_config* c);
 gdock_status gdock_server_create(int device, const gdock_server_config* config, gdock_server** out);
-/* Stops the worker; jobs still queued or running are dropped. */
+/* Stops the server's threads.  Jobs still queued or running end as failed
+ * (GDOCK_ERR_NOT_FOUND): threads blocked in gdock_server_wait return that
+ * status, and destroy returns once the last of them has left.  No call may
+ * name the server after destroy returns. */
 void gdock_server_destroy(gdock_server* s);
 /* Thread safe.  The ligand is positioned on the receptor (utils::
  * position_ligand) at admission.  params: NULL for the defaults. */

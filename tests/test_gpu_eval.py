@@ -80,6 +80,7 @@ def test_ga_debug_mode_dockq_matches_oracle():
     res = ga.run_dockq(cx, ev, seed=5, keep_trace=1, population_size=60, max_generations=30)
     ora = orc.GaRun(rec.atoms, lig.atoms, [], seed=5, pop_size=60, max_gen=30, reference=native.atoms)
     assert np.allclose(res.fitness, ora.fitness[:len(res.fitness)], rtol=1e-12, atol=1e-14)
-    outcome = _compare(res, ora, 60)
+    outcome, compared = _compare(res, ora, 60)
+    assert compared >= min(10, res.generations), outcome
     assert (res.fitness <= 0).all() and res.fitness.min() >= -1.0 - 1e-12
     print("debug-mode GA vs oracle:", outcome, res.generations, "generations, best DockQ", -res.fitness.min())

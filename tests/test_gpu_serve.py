@@ -73,7 +73,9 @@ def test_split_job_alone_equals_in_mix():
     _same(alone, got[2])
     rec, lig, pairs = big
     plain = ga.run(Complex(rec, structure.position_ligand(rec, lig), pairs), seed=8, **params)
-    print("split served vs plain:", _compare(alone, plain, 40))
+    outcome, compared = _compare(alone, plain, 40)
+    print("split served vs plain:", outcome)
+    assert compared >= min(5, alone.generations), outcome
 
 
 def test_served_job_matches_oracle_ga():
@@ -86,7 +88,9 @@ def test_served_job_matches_oracle_ga():
         srv.close()
     lig_pos = structure.position_ligand(rec, lig)
     ora = orc.GaRun(rec.atoms, lig_pos.atoms, pairs, W, seed=42, pop_size=150, max_gen=250, nthreads=16)
-    print("served vs oracle:", _compare(res, ora, 150), f"latency {t.latency_s:.2f} s")
+    outcome, compared = _compare(res, ora, 150)
+    print("served vs oracle:", outcome, f"latency {t.latency_s:.2f} s")
+    assert compared >= min(10, res.generations), outcome
 
 
 def test_memory_admission_limits_active_jobs():
@@ -138,3 +142,30 @@ def test_job_failing_in_preparation_leaves_the_server_working():
         assert st["jobs_finished"] == 2 and st["bytes_in_use"] == 0
     finally:
         srv.close()
+
+
+def test_destroy_ends_pending_waits():
+    """Destroying the server while another thread waits on a job that has not
+    ended: the job ends as failed, the wait returns NOT_FOUND, and destroy
+    returns once the waiter has left.  One job slot and three long jobs: the
+    third is still queued when destroy begins."""
+    import threading
+    import time
+    jobs = [_job(95 + i, 120, 90) for i in range(3)]
+    srv = Server(max_jobs=1)
+    ids = [srv.submit(*j, seed=1 + k, max_generations=250, early_stopping=0) for k, j in enumerate(jobs)]
+    got = {}
+
+    def waiter():
+        try:
+            srv.wait(ids[2], 300.0)
+            got["status"] = _lib.OK
+        except _lib.GdockError as e:
+            got["status"] = e.status
+
+    t = threading.Thread(target=waiter)
+    t.start()
+    time.sleep(0.3)
+    srv.close()
+    t.join(60.0)
+    assert not t.is_alive() and got["status"] == _lib.NOT_FOUND
