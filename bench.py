@@ -4,9 +4,15 @@ One step = one launch of the gfx950 scoring kernel over 4096 poses whose
 records already sit in HBM (gdock_score_poses_device).  Prints one JSON line:
 pose-evals/s for the whole job, the roofline entry of the scoring kernel (26
 algorithmic FP32 FLOP per receptor-ligand pair over the kernel's event-timed
-average duration, against the MI355X FP32 peak) and the CPU baseline (the FP64
-oracle restatement of the reference, timed on this host's cores on a bounded
-sample of the same poses).
+average duration, against the MI355X FP32 peak), the pairs the kernel steps
+through after culling beside the brute-force pairs, and the CPU baseline (the
+FP64 oracle restatement of the reference, timed on this host at its usable
+thread count and at two fewer, on a bounded sample of the same poses).  The
+same line carries the other two parts of the metric, measured after the
+timed region: `uneven_batch` (64 uneven complexes x 150 poses: batched
+persistent launch against the plain path, one complex after another) and
+`served` (GA docking jobs arriving as a Poisson process at 20 jobs/s:
+jobs/min and latency percentiles, served and plain).
 """
 from __future__ import annotations
 
@@ -38,8 +44,12 @@ def parse():
     ap.add_argument("--jobs", type=int, default=160, help="serve: number of docking jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate, jobs/s (0: all at once)")
     ap.add_argument("--poses", type=int, default=4096)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=5.0,
+                    help="target wall time of each CPU baseline sample (two thread counts)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="batched_scoring: skip the uneven-batch and served-stream measurements")
+    ap.add_argument("--extra-jobs", type=int, default=32, help="served-stream jobs measured with the default line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for PMC byte counters)")
     return ap.parse_args()
@@ -196,12 +206,46 @@ def main():
             "flop_per_launch": flop,
         },
     }
+    # culling: lane-pairs the kernel steps through (counting instantiation, untimed)
+    stepped, within = cx.count_pairs(genes)
+    result["pairs_brute_force_per_launch"] = pairs
+    result["pairs_evaluated_per_launch"] = stepped
+    result["pairs_within_15A_per_launch"] = within
+    result["pairs_evaluated_frac"] = stepped / pairs
+    result["pairs_evaluated_per_s"] = stepped / (kern_ms * 1e-3)
+    cx.close()
+    if not args.no_extras:
+        u = measure_uneven(max(5, args.steps // 2), 2)
+        result["uneven_batch"] = {k: u[k] for k in ("workload", "pose_evals_per_s", "plain_pose_evals_per_s",
+                                                   "speedup_vs_plain", "kernel_ms", "kernel_pose_evals_per_s",
+                                                   "ms_per_step", "plain_ms_per_step")}
+        v = measure_serve(args.extra_jobs, 20.0)
+        result["served"] = {k: v[k] for k in ("workload", "jobs_per_min", "latency_p50_s", "latency_p99_s",
+                                              "generations", "peak_active_jobs", "span_s", "plain",
+                                              "speedup_jobs_per_min")}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(rec, lig, c.restraints, genes, args.cpu_seconds)
+        result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
 
 
 def bench_uneven(args):
+    u = measure_uneven(args.steps, args.warmup)
+    result = {
+        "metric": METRIC,
+        "value": u["pose_evals_per_s"],
+        "unit": "pose-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": u["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded complexes and poses)",
+        "config": {"workload": u["workload"], "parallelism": "single GPU"},
+    }
+    result.update({k: v for k, v in u.items() if k not in ("workload", "pose_evals_per_s", "ms_per_step")})
+    print(json.dumps(result), flush=True)
+
+
+def measure_uneven(steps, warmup):
     """64 complexes (receptor 500-8000, ligand 300-3000 atoms, seeds 100-163),
     150 poses each (the GA's initial population: angles U[0, 2 pi], shifts
     U[-50, 50] A).  Batched: one replay of the captured generation graph
@@ -213,6 +257,7 @@ def bench_uneven(args):
     from gdock_amd.device import Batch, Complex, default_context
 
     per = 150
+    steps = max(1, steps)
     cs = synth.uneven_batch(64, 100)
     ctx = default_context(0)
     cxs, genes, pairs = [], [], 0.0
@@ -226,45 +271,43 @@ def bench_uneven(args):
     b = Batch(cxs, [per] * len(cxs), ctx)
     create_s = time.perf_counter() - t0
     g = np.concatenate(genes)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.score(g)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         out, _ = b.score(g)
-    wall_b = (time.perf_counter() - t0) / args.steps
+    wall_b = (time.perf_counter() - t0) / steps
     assert np.isfinite(out).all()
     # kernel-only: graph replays on one stream, inputs already in HBM
     stream = torch.cuda.Stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.launch(None, stream.cuda_stream)
     ev1.record(stream)
     torch.cuda.synchronize()
-    kern_b = ev0.elapsed_time(ev1) / args.steps * 1e-3
+    kern_b = ev0.elapsed_time(ev1) / steps * 1e-3
     # plain drop-in, one job after another
     for cx, gi in zip(cxs, genes):
         cx.score_poses(gi)
     t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 2)):
+    for _ in range(max(1, steps // 2)):
         for cx, gi in zip(cxs, genes):
             cx.score_poses(gi)
-    wall_p = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    wall_p = (time.perf_counter() - t0) / max(1, steps // 2)
     n = per * len(cxs)
     flop = FLOP_PER_PAIR * pairs
     plan = b.plan()
-    result = {
-        "metric": METRIC,
-        "value": n / wall_b,
-        "unit": "pose-evals/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    b.close()
+    for cx in cxs:
+        cx.close()
+    return {
+        "workload": f"uneven_batch: 64 complexes (seeds 100-163, receptor 500-8000 and ligand "
+                    f"300-3000 atoms), {per} GA-initial poses each, host genes in / results out",
+        "pose_evals_per_s": n / wall_b,
         "ms_per_step": wall_b * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic (seeded complexes and poses)",
-        "config": {"workload": f"uneven_batch: 64 complexes (seeds 100-163, receptor 500-8000 and ligand "
-                               f"300-3000 atoms), {per} GA-initial poses each, host genes in / results out",
-                   "parallelism": "single GPU"},
+        "plain_ms_per_step": wall_p * 1e3,
         "plain_pose_evals_per_s": n / wall_p,
         "speedup_vs_plain": wall_p / wall_b,
         "kernel_ms": kern_b * 1e3,
@@ -279,10 +322,25 @@ def bench_uneven(args):
                      "basis": "brute-force pairs; the culling skips most pairs of GA-spread poses, so this "
                               "algorithmic rate can exceed what the ALUs execute"},
     }
-    print(json.dumps(result), flush=True)
 
 
 def bench_serve(args):
+    v = measure_serve(args.jobs, args.rate)
+    result = {
+        "metric": METRIC,
+        "value": v["jobs_per_min"],
+        "unit": "jobs/min",
+        "n_gpus": 1, "steps": args.jobs, "warmup": 1,
+        "ms_per_step": v["span_s"] / args.jobs * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded complexes, Poisson arrivals)",
+        "config": {"workload": v["workload"], "parallelism": "single GPU"},
+    }
+    result.update({k: x for k, x in v.items() if k not in ("workload", "jobs_per_min")})
+    print(json.dumps(result), flush=True)
+
+
+def measure_serve(n_jobs, rate):
     """Served docking stream: jobs (complexes drawn as in the uneven batch,
     per-job seeds 1000+i, 4-12 restraints) arrive as a Poisson process and run
     the full GA (population 150, early stop after 10 flat generations) and the
@@ -292,25 +350,22 @@ def bench_serve(args):
     times to the measured service times (first come, first served)."""
     import torch  # noqa: F401  (one HIP runtime, see gdock_amd._lib)
     from gdock_amd import ga, structure, synth
-    from gdock_amd.device import Batch, Complex, default_context
+    from gdock_amd.device import Complex, default_context
     from gdock_amd.serve import Server
 
-    cs = synth.uneven_batch(args.jobs, 1000)
+    cs = synth.uneven_batch(n_jobs, 1000)
     jobs = []
     for c in cs:
         rec = structure.read_pdb_text(c.receptor_pdb)[0]
         lig = structure.read_pdb_text(c.ligand_pdb)[0]
         jobs.append((rec, lig, c.restraints))
     rng = np.random.default_rng(7)
-    if args.rate > 0:
-        arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(jobs)))
+    if rate > 0:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(jobs)))
         arrivals -= arrivals[0]
     else:  # burst: every job queued at once (the server's capacity)
         arrivals = np.zeros(len(jobs))
     ctx = default_context(0)
-    # lane choices are measured once per complex shape and cached for the
-    # process, as a deployed autotuning table would be
-    Batch([Complex(r, structure.position_ligand(r, l), p, ctx) for r, l, p in jobs], [1] * len(jobs), ctx).close()
 
     srv = Server(max_jobs=64, max_receptor_atoms=8192)
     warm = srv.submit(*jobs[0], seed=1, max_generations=2, cluster=0)
@@ -346,18 +401,11 @@ def bench_serve(args):
         plat.append(free_at - a)
     plat = np.array(plat)
     pspan = free_at - arrivals[0]
-    result = {
-        "metric": METRIC,
-        "value": len(jobs) / span * 60.0,
-        "unit": "jobs/min",
-        "n_gpus": 1, "steps": len(jobs), "warmup": 1,
-        "ms_per_step": span / len(jobs) * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic (seeded complexes, Poisson arrivals)",
-        "config": {"workload": f"serve: {len(jobs)} GA docking jobs (receptor 500-8000, ligand 300-3000 atoms, "
-                               f"population 150, early stop 10, FCC of the Hall of Fame), "
-                               + (f"Poisson arrivals {args.rate:g} jobs/s" if args.rate > 0 else "all submitted at once"),
-                   "parallelism": "single GPU"},
+    return {
+        "workload": f"serve: {len(jobs)} GA docking jobs (receptor 500-8000, ligand 300-3000 atoms, "
+                    f"population 150, early stop 10, FCC of the Hall of Fame), "
+                    + (f"Poisson arrivals {rate:g} jobs/s" if rate > 0 else "all submitted at once"),
+        "jobs_per_min": len(jobs) / span * 60.0,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "pose_evals_per_s": st["poses_scored"] / span,
         "generations": gens, "peak_active_jobs": st["peak_active"], "steps_run": st["steps"],
@@ -370,7 +418,6 @@ def bench_serve(args):
                   "generations": pgens, "pose_evals_per_s": pgens * 150 / float(np.sum(service))},
         "speedup_jobs_per_min": pspan / span,
     }
-    print(json.dumps(result), flush=True)
 
 
 def bench_fcc(args):
@@ -455,7 +502,8 @@ def bench_ga(args):
     }
     if not args.no_cpu:
         import oracle.oracle as orc
-        threads = min(16, os.cpu_count() or 1)
+        cpu = host_cpus()
+        threads = max(1, min(cpu["nproc"], cpu["share"] or cpu["nproc"]))
         t0 = time.perf_counter()
         ora = orc.GaRun(rec.atoms, lig.atoms, c.restraints, seed=42, nthreads=threads)
         cpu = time.perf_counter() - t0
@@ -467,21 +515,56 @@ def bench_ga(args):
     print(json.dumps(result), flush=True)
 
 
+def host_cpus():
+    """This host's CPU model, nproc (the CPUs this process may run on), the
+    logical CPU count, and the CPU share granted to it (the cgroup quota, or
+    OMP_NUM_THREADS where the host sets that instead)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0))
+    share = None
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            share = int(quota) // int(period)
+    except (OSError, ValueError):
+        pass
+    if share is None and os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = int(os.environ["OMP_NUM_THREADS"])
+    return {"model": model, "nproc": nproc, "logical_cpus": os.cpu_count(), "share": share}
+
+
 def cpu_baseline(rec, lig, pairs, genes, target_s):
-    """FP64 oracle (reference restatement, one pose per task on a thread pool as
-    rayon's par_iter_mut) on a sample of the same poses."""
+    """FP64 oracle (reference restatement, one pose per task on a thread pool
+    as rayon's par_iter_mut) on a sample of the same poses, at the host's
+    usable thread count and at two fewer (the reference's default thread
+    count is the CPU count minus 2, main.rs:195-198).  The usable count is
+    nproc, capped at the CPU share this process is granted."""
     import oracle.oracle as orc
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    k = threads
-    orc.score_poses(rec.atoms, lig.atoms, pairs, genes[:k], nthreads=threads)
-    dt = time.perf_counter() - t0
-    m = int(min(len(genes), max(k, k * round(target_s / max(dt, 1e-3)))))
-    t0 = time.perf_counter()
-    orc.score_poses(rec.atoms, lig.atoms, pairs, genes[:m], nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": m / dt, "unit": "pose-evals/s", "cores": threads, "kind": "port",
-            "sample": f"first {m} of the same poses, {dt:.1f} s wall on {threads} threads (FP64 oracle, -O3 generic x86-64)"}
+    cpu = host_cpus()
+    full = max(1, min(cpu["nproc"], cpu["share"] or cpu["nproc"]))
+    runs = []
+    for threads in dict.fromkeys([full, max(1, full - 2)]):
+        t0 = time.perf_counter()
+        orc.score_poses(rec.atoms, lig.atoms, pairs, genes[:threads], nthreads=threads)
+        dt = time.perf_counter() - t0
+        m = int(min(len(genes), max(threads, threads * round(target_s / max(dt, 1e-3)))))
+        t0 = time.perf_counter()
+        orc.score_poses(rec.atoms, lig.atoms, pairs, genes[:m], nthreads=threads)
+        dt = time.perf_counter() - t0
+        runs.append({"threads": threads, "poses": m, "wall_s": dt, "value": m / dt})
+    return {"value": runs[0]["value"], "unit": "pose-evals/s", "cores": full, "kind": "port",
+            "sample": f"first {runs[0]['poses']} of the same poses on {full} threads, {runs[0]['wall_s']:.1f} s wall "
+                      f"(FP64 oracle, -O3 generic x86-64)",
+            "runs": runs, "cpu_model": cpu["model"], "nproc": cpu["nproc"], "logical_cpus": cpu["logical_cpus"],
+            "cpu_share": cpu["share"]}
 
 
 if __name__ == "__main__":

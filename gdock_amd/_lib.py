@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         "gdock_prepare_poses": (C.c_int, [D, C.c_int32, D]),
         "gdock_score_poses_device": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Weights), P, P, P]),
         "gdock_score_poses_device_noalloc": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Weights), P, P, P]),
+        "gdock_count_pairs": (C.c_int, [P, P, D, C.c_int32, C.POINTER(C.c_int64)]),
         "gdock_contact_words": (C.c_int32, [P, C.c_double]),
         "gdock_contact_candidates": (C.c_int32, [P, C.c_double]),
         "gdock_contact_pair": (C.c_int, [P, C.c_double, C.c_int32, C.c_char_p, C.POINTER(C.c_int16), C.c_char_p,

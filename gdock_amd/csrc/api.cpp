@@ -747,6 +747,38 @@ gdock_status gdock_score_poses(gdock_ctx* ctx, gdock_complex* cx, const double* 
   return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "score_poses: download");
 }
 
+gdock_status gdock_count_pairs(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n, int64_t* counts) {
+  if (!ctx || !cx || !counts || n < 0 || (n && !genes)) {
+    if (ctx) set_error(ctx, "count_pairs: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  counts[0] = counts[1] = 0;
+  if (n == 0) return GDOCK_OK;
+  std::vector<double> poses(static_cast<size_t>(n) * kPoseDoubles);
+  if (gdock_prepare_poses(genes, n, poses.data()) != GDOCK_OK) {
+    set_error(ctx, "count_pairs: non-finite gene");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "count_pairs: set device");
+  const size_t pb = poses.size() * 8, ob = static_cast<size_t>(n) * kOutDoubles * 8;
+  const size_t o_out = (pb + 255) & ~static_cast<size_t>(255), o_cnt = (o_out + ob + 255) & ~static_cast<size_t>(255),
+               o_rd = o_cnt + 256;
+  if (!ctx->staging.reserve(o_rd + plain_scratch_bytes(cx->dev, n), &e)) return hip_fail(ctx, e, "count_pairs: staging");
+  char* base = static_cast<char*>(ctx->staging.ptr);
+  e = hipMemcpyAsync(base, poses.data(), pb, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(base + o_cnt, 0, 16, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "count_pairs: upload");
+  const Weights ww{0.4, 0.05, 3.4, 1.0};
+  e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(base), nullptr, n, ww,
+                   reinterpret_cast<double*>(base + o_out), nullptr, base + o_rd, ctx->stream, 0,
+                   reinterpret_cast<unsigned long long*>(base + o_cnt));
+  if (e != hipSuccess) return hip_fail(ctx, e, "count_pairs: launch");
+  e = hipMemcpyAsync(counts, base + o_cnt, 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "count_pairs: download");
+}
+
 gdock_status gdock_score_coords(gdock_ctx* ctx, gdock_complex* cx, const double* lig_xyz, int32_t n,
                                 const gdock_weights* w, double* out, int32_t* n_satisfied) {
   if (!ctx || !cx || !w || n < 0 || (n && (!lig_xyz || !out))) return GDOCK_ERR_INVALID_ARG;

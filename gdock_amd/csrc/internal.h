@@ -115,10 +115,12 @@ void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::v
 // Scores n_poses poses: the fast kernel, then the exact kernel for any pose
 // whose settle buffer overflowed.  scratch: plain_scratch_bytes(cx, n_poses)
 // of device memory.  lanes_k: ligand atoms per lane (1-8); 0 picks
-// ceil(n_lig / 256) capped at 8.
+// ceil(n_lig / 256) capped at 8.  counts (diagnostic): when set, only the
+// counting instantiation of the fast kernel runs and adds {pair steps x 64,
+// pairs within 15 A} to counts[0..1] (the outputs are then not valid).
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
-                        hipStream_t stream, int lanes_k = 0);
+                        hipStream_t stream, int lanes_k = 0, unsigned long long* counts = nullptr);
 int score_atoms_per_lane(int n_lig);
 // Receptor-range parts of the plain path (1 unless the receptor exceeds one
 // workgroup's LDS table) and its scratch bytes (flags, split state).

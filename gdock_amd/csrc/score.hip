@@ -501,12 +501,16 @@ __device__ __forceinline__ double ligand_desolv(CxRef cx, int l, int32_t cnt) {
 // coordinates as given.  o: 6 outputs; *redo is set when the fast variant
 // could not settle every band pair (the exact variant then rescores the
 // pose).  The LDS receptor count table covers the part's slots [r0, r1).
-template <int K, bool EXACT>
+// COUNT (a diagnostic instantiation): counts[0] += the pair steps the wave
+// executes x 64 lanes, counts[1] += the pairs among them within 15 A (below
+// the 15 A guard band).
+template <int K, bool EXACT, bool COUNT = false>
 __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ pose_rec,
                                            const double* __restrict__ coords, const Weights& w,
                                            double* __restrict__ o, int32_t* __restrict__ nsat_out,
                                            int32_t* __restrict__ redo, int32_t* __restrict__ s_rcnt,
-                                           int32_t* s_overflow, const PosePart& pp) {
+                                           int32_t* s_overflow, const PosePart& pp,
+                                           unsigned long long* __restrict__ counts = nullptr) {
   const int range = pp.r1 - pp.r0;
   double* s_red = reinterpret_cast<double*>(s_rcnt + ((range + 1) & ~1));
   SettleEvent* s_ev = reinterpret_cast<SettleEvent*>(s_red + 4 * (kScoreThreads / 64));
@@ -528,6 +532,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
 
   double sum_v = 0.0, sum_e = 0.0, sum_d = 0.0;
   int overflow = 0;
+  uint32_t n_steps = 0, n_within = 0;  // COUNT only
   for (int base = 0; base < cx.n_lig; base += kScoreThreads * K) {
     LaneAtoms<K> a;
 #pragma unroll
@@ -599,6 +604,13 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         for (int k = 0; k < K; ++k) {  // wave-uniform branches
           const uint32_t mk = static_cast<uint32_t>(keep[k] >> sh);
           if ((mk & 15u) == 0u) continue;
+          if constexpr (COUNT) {
+            n_steps += __builtin_popcount(mk & 15u);
+            const RecAtom rr[4] = {r0, r1, r2, r3};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if ((mk >> q) & 1u) n_within += pair_s(rr[q], a.x[k], a.y[k], a.z[k]) <= B.lo225 ? 1u : 0u;
+          }
           if (mk & 1u) c0 += pair_fast<K>(r0, B, a, k, ambm);
           if (mk & 2u) c1 += pair_fast<K>(r1, B, a, k, ambm);
           if (mk & 4u) c2 += pair_fast<K>(r2, B, a, k, ambm);
@@ -650,6 +662,15 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           sum_d -= ligand_desolv(cx, l, a.cnt[k]);
         }
       }
+    }
+  }
+  if constexpr (COUNT) {
+    uint32_t c = n_within;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) {
+      atomicAdd(&counts[0], 64ull * n_steps);
+      atomicAdd(&counts[1], static_cast<unsigned long long>(c));
     }
   }
   if (overflow && lane == 0) *s_overflow = 1;
@@ -770,11 +791,12 @@ __device__ __forceinline__ int2 part_range(int n_rec, int part, int parts) {
 // part unless the receptor exceeds one workgroup's LDS table).  A split pose
 // keeps its shared state at split_* (cleared before the launch; the last part
 // clears it again).
-template <int K, bool EXACT>
+template <int K, bool EXACT, bool COUNT = false>
 __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4, EXACT ? 8 : 4))) void score_poses_kernel(
     const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
     const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo,
-    int parts, int32_t* __restrict__ split_lig, double* __restrict__ split_sums, int32_t* __restrict__ split_sync) {
+    int parts, int32_t* __restrict__ split_lig, double* __restrict__ split_sums, int32_t* __restrict__ split_sync,
+    unsigned long long* __restrict__ counts) {
   const int pose = blockIdx.x / parts, part = blockIdx.x % parts;
   if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
   extern __shared__ int32_t s_lds[];
@@ -787,10 +809,10 @@ __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(E
     pp.sums = split_sums + static_cast<size_t>(pose) * 3 * parts;
     pp.sync = split_sync + static_cast<size_t>(pose) * 2;
   }
-  score_pose<K, EXACT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
-                       coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr, w,
-                       out + static_cast<size_t>(kOutDoubles) * pose, nsat_out ? nsat_out + pose : nullptr,
-                       redo + pose, s_lds, s_flags, pp);
+  score_pose<K, EXACT, COUNT>(cx, coords ? nullptr : poses + static_cast<size_t>(kPoseDoubles) * pose,
+                              coords ? coords + static_cast<size_t>(pose) * cx.n_lig * 3 : nullptr, w,
+                              out + static_cast<size_t>(kOutDoubles) * pose, nsat_out ? nsat_out + pose : nullptr,
+                              redo + pose, s_lds, s_flags, pp, counts);
 }
 
 int score_atoms_per_lane(int n_lig) {
@@ -921,12 +943,18 @@ size_t plain_scratch_bytes(const DevComplex& cx, int n_poses) {
 template <int K>
 static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, int n_poses,
                         const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, int parts,
-                        const DevComplex& split, size_t lds, hipStream_t stream) {
+                        const DevComplex& split, size_t lds, hipStream_t stream, unsigned long long* counts) {
   const dim3 grid(static_cast<unsigned>(n_poses) * parts);
+  if (counts) {  // the counting instantiation of the fast kernel alone
+    hipLaunchKernelGGL((score_poses_kernel<K, false, true>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses,
+                       d_coords, w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync,
+                       counts);
+    return;
+  }
   hipLaunchKernelGGL((score_poses_kernel<K, false>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
-                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync);
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, nullptr);
   hipLaunchKernelGGL((score_poses_kernel<K, true>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
-                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync);
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, nullptr);
 }
 
 // Fast kernel over all poses, then the exact kernel (a no-op for every pose
@@ -935,7 +963,7 @@ static void launch_pair(const DevComplex* d_cx, const double* d_poses, const dou
 // larger than one workgroup's LDS table, cleared here).
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
-                        hipStream_t stream, int lanes_k) {
+                        hipStream_t stream, int lanes_k, unsigned long long* counts) {
   if (n_poses <= 0) return hipSuccess;
   const int parts = plain_parts(cx);
   const size_t lds = score_lds_bytes(part_slots(cx.n_rec, parts));
@@ -952,7 +980,7 @@ hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const doub
   switch (lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig)) {
 #define GDOCK_CASE(KK)                                                                                  \
   case KK:                                                                                              \
-    launch_pair<KK>(d_cx, d_poses, d_coords, n_poses, w, d_out, d_nsat, d_redo, parts, split, lds, stream); \
+    launch_pair<KK>(d_cx, d_poses, d_coords, n_poses, w, d_out, d_nsat, d_redo, parts, split, lds, stream, counts); \
     break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)

@@ -86,6 +86,14 @@ class Complex:
                                       ptr(ns, C.c_int32)), self.ctx.handle, "gdock_score_poses")
         return out, ns
 
+    def count_pairs(self, genes: np.ndarray) -> tuple[int, int]:
+        """(lane-pairs the scoring kernel steps through after culling, those within 15 A)."""
+        g = np.ascontiguousarray(genes, dtype=np.float64).reshape(-1, 6)
+        c = np.zeros(2, np.int64)
+        check(lib().gdock_count_pairs(self.ctx.handle, self.handle, ptr(g), len(g), ptr(c, C.c_int64)),
+              self.ctx.handle, "gdock_count_pairs")
+        return int(c[0]), int(c[1])
+
     def score_coords(self, lig_xyz: np.ndarray, weights: constants.EnergyWeights | None = None):
         """Ligand coordinates (n, n_lig, 3) scored as given (commands/score.rs)."""
         x = np.ascontiguousarray(lig_xyz, dtype=np.float64).reshape(-1, len(self.ligand), 3)

@@ -347,6 +347,14 @@ gdock_status gdock_server_wait(gdock_server* s, int64_t job_id, double timeout_s
                                gdock_job_stats* stats);
 gdock_status gdock_server_stats(gdock_server* s, gdock_server_stats_t* out);
 
+/* Diagnostic (no counterpart in the reference; for measurement): runs the
+ * counting instantiation of the plain scoring kernel over the poses of n gene
+ * vectors and returns counts[0] = receptor x ligand lane-pairs the kernel
+ * steps through after its culling (64 per wave step, padding lanes
+ * included) and counts[1] = those within 15 A, against the brute-force
+ * n_rec x n_lig x n.  Scores are not returned. */
+gdock_status gdock_count_pairs(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n, int64_t* counts);
+
 /* ---- FCC clustering ---------------------------------------------------------- */
 /* Contact bitsets (clustering::calculate_contacts, src/clustering.rs:62-115) of
  * the complexes receptor + (ligand moved by genes i), for any chain layout:

@@ -257,3 +257,29 @@ def test_component_errors_against_fp64_oracle():
             parts.append(f"{name} max abs {err[:, j].max():.2e} max rel {rel[nz, j].max() if nz.any() else 0.0:.2e}")
         print(f"{what} ({len(rec)} x {len(lig)} atoms, {len(genes)} poses): " + "; ".join(parts))
         assert err[:, 1].max() <= 5e-4, (what, err[:, 1].max())
+
+
+def test_culling_keeps_every_pair_within_the_cutoff():
+    """The counting instantiation of the scoring kernel: the lane-pairs it
+    steps through are whole wave steps (multiples of 64) and at most the
+    padded brute force, and the pairs among them within 15 A equal, up to
+    the pairs inside the 15 A guard band, the pairs that brute force finds
+    within 15 A: the culling never drops a pair inside the cutoff."""
+    c = synth.make_complex(41, rec_atoms=1800, lig_atoms=700, n_restraints=4)
+    rec, lig, cx = _pair(c.receptor_pdb, c.ligand_pdb, c.restraints)
+    genes = np.concatenate([synth.random_genes(41, 12, 8.0), synth.ga_genes(42, 4)])
+    stepped, within = cx.count_pairs(genes)
+    r = rec.xyz
+    lo = hi = 0
+    for g in genes:
+        moved = orc.apply_genes(lig.atoms, g)
+        lx = np.stack([moved["x"], moved["y"], moved["z"]], 1)
+        d2 = ((r[:, None, :] - lx[None, :, :]) ** 2).sum(-1)
+        lo += int((d2 <= 225.0 - 0.05).sum())
+        hi += int((d2 <= 225.0 + 0.05).sum())
+    k = -(-len(lig) // 256)
+    padded = (-(-len(rec) // 64) * 64) * (256 * k) * len(genes)
+    print(f"stepped {stepped} ({stepped / (len(rec) * len(lig) * len(genes)):.3f} of brute force), within {within}, "
+          f"brute force within 15 A: {lo}..{hi}")
+    assert stepped % 64 == 0 and 0 < stepped <= padded
+    assert lo <= within <= hi and within <= stepped
