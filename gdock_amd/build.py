@@ -37,21 +37,40 @@ def sources() -> list[str]:
     return [str(PKG / "csrc" / s) for s in SOURCES]
 
 
+def _run(cmd: list[str]) -> subprocess.CompletedProcess:
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
 def build(verbose: bool = True) -> Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS]
+    """One object per source (compiled in parallel into gdock_amd/_build),
+    then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    out = PKG / "_build"
+    out.mkdir(exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
     for s in sources():
-        cmd += ["-x", "hip", s] if s.endswith(".cpp") else [s]
-    cmd += ["-o", str(tmp)]
+        obj = out / (Path(s).name + ".o")
+        jobs.append([hipcc(), *compile_flags, "-c", *(["-x", "hip", s] if s.endswith(".cpp") else [s]),
+                     "-o", str(obj)])
     if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+        print(f"hipcc {' '.join(compile_flags)} -c <{len(jobs)} sources> -> {out}", flush=True)
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4, 16)) as pool:
+        results = list(pool.map(_run, jobs))
+    notes = []
+    for cmd, res in zip(jobs, results):
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed compiling {cmd[-3]}")
+        notes += [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *[j[-1] for j in jobs], "-o", str(tmp)]
+    res = _run(link)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libgdock_hip.so")
-    err = "\n".join(l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
-    if err and verbose:
-        sys.stderr.write(err + "\n")
+        raise RuntimeError("hipcc failed linking libgdock_hip.so")
+    if notes and verbose:
+        sys.stderr.write("\n".join(notes) + "\n")
     os.replace(tmp, LIB)
     return LIB
 
