@@ -471,6 +471,10 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   for (int32_t i = 0; i < n_lig; ++i) lig_perm[i] = i;
   bisect_order(cx->lig.data(), lig_perm, 0, lig_perm.size(), kLigBlock);
   spread_blocks(lig_perm, kLigBlock);
+  // within a block, aligned quarters of kLigSub atoms are compact too (the
+  // kernel culls by the boxes of both)
+  for (size_t b = 0; b < lig_perm.size(); b += kLigBlock)
+    bisect_order(cx->lig.data(), lig_perm, b, std::min(lig_perm.size(), b + kLigBlock), kLigSub);
   cx->lig_slot.assign(n_lig, 0);
   for (int32_t q = 0; q < n_lig; ++q) cx->lig_slot[lig_perm[q]] = q;
   std::unordered_map<int32_t, std::vector<int32_t>> lig_by_serial;  // serial -> device indices
@@ -515,6 +519,25 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
     meta[j] = make_int2(a.serial, hit != lig_by_serial.end() ? 1 : 0);
     if (hit != lig_by_serial.end())
       for (int32_t l : hit->second) coll.push_back(make_int4(j, l, cls, 0));
+  }
+  // Sub-tile boxes: the kernel's own FP32 A coordinates of each receptor slot
+  // (FP32 of the fixed-point integer times the power-of-two unit), so a box
+  // holds every slot's coordinates exactly and a box beyond reach of a
+  // ligand block implies every slot in it is.
+  const int32_t n_sub = (n_pad + kRecSub - 1) / kRecSub;
+  std::vector<float4> sub_box(2 * static_cast<size_t>(std::max(n_sub, 1)));
+  const float unit_f = std::ldexp(1.0f, -qexp);
+  for (int32_t t = 0; t < n_sub; ++t) {
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    for (int32_t j = t * kRecSub; j < std::min(n_pad, (t + 1) * kRecSub); ++j) {
+      if (perm[j] < 0) continue;
+      const float x = static_cast<float>(fixed[j].x) * unit_f, y = static_cast<float>(fixed[j].y) * unit_f,
+                  z = static_cast<float>(fixed[j].z) * unit_f;
+      lo.x = std::min(lo.x, x); lo.y = std::min(lo.y, y); lo.z = std::min(lo.z, z);
+      hi.x = std::max(hi.x, x); hi.y = std::max(hi.y, y); hi.z = std::max(hi.z, z);
+    }
+    sub_box[2 * t] = lo;
+    sub_box[2 * t + 1] = hi;
   }
   std::vector<double> lx(n_lig), ly(n_lig), lz(n_lig), lrm(n_lig), leps(n_lig), lq(n_lig), lasp(n_lig), lrad(n_lig);
   std::vector<int32_t> lser(n_lig);
@@ -561,7 +584,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
                o_lser = L.add<int32_t>(n_lig), o_lkq = L.add<double>(lkq.size()), o_rrx = L.add<double>(rr_xyz.size()),
                o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1),
-               o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig);
+               o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig), o_sbox = L.add<float4>(sub_box.size());
   // Lennard-Jones factors of every (receptor class, ligand atom) pair, with
   // the FP64 operations of fitness.rs:58-63 (R = rmin2_r + rmin2_l;
   // sqrt(eps_r eps_l)), as q^2 / R^2 2^-16 in the kernel's squared units
@@ -598,6 +621,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_lkq, lkq.data(), lkq.size() * 8);
   put(o_rrx, rr_xyz.data(), rr_xyz.size() * 8); put(o_rli, rl_idx.data(), rl_idx.size() * 4);
   put(o_lcls, lcls.data(), lcls.size() * sizeof(float2));
+  put(o_sbox, sub_box.data(), sub_box.size() * sizeof(float4));
   if (!host.empty()) std::memcpy(blob.data() + cx->contacts_layout_begin, host.data(), host.size());
 
   hipError_t e = hipSetDevice(ctx->device);
@@ -631,6 +655,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.restr_rec_xyz = reinterpret_cast<const double*>(m + o_rrx);
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
   d.lig_class = reinterpret_cast<const float2*>(m + o_lcls);
+  d.rec_sub_box = reinterpret_cast<const float4*>(m + o_sbox);
   d.n_rec = n_pad; d.n_rec_atoms = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
   d.n_coll = static_cast<int32_t>(coll.size());
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }

@@ -41,6 +41,8 @@ constexpr int kMaxReceptorAtoms = kMaxPoseParts * kPartSlots - 4 * 64;
 constexpr float kPadCoord = 1.0e4f;      // padding ligand lanes of the contact kernel sit far away
 constexpr int kRecGroup = 4;             // receptor atoms per scalar load
 constexpr int kLigBlock = 64;            // ligand atoms per spatial block (one wave's slot)
+constexpr int kLigSub = 16;              // ligand atoms per spatial sub-block (a quarter of a block)
+constexpr int kRecSub = 16;              // receptor slots per sub-tile box
 // A receptor group and a wave's ligand atoms farther apart than this have no
 // pair within any cutoff (15 A) or its guard band.
 constexpr float kCullReach = 15.01f;
@@ -72,6 +74,7 @@ struct DevComplex {
   const double* class_rmin2;
   const double* class_eps;
   const float2* lig_class;     // n_class x n_lig: {q^2 / R^2 (q: the fixed-point unit), sqrt(eps_r eps_l)} per receptor class and ligand atom
+  const float4* rec_sub_box;   // per kRecSub receptor slots: {lo x, y, z, 0}, {hi x, y, z, 0} of the slots' FP32 A coordinates (padding excluded; empty: lo +inf, hi -inf)
   // ligand (positioned), in spatial block order (lig_orig: index in the input)
   const double* lig_x;
   const double* lig_y;

@@ -151,21 +151,10 @@ __device__ __forceinline__ int lig_index(int base, int k) {
 }
 
 // Axis-aligned boxes of a wave's K ligand blocks (FP32 A, origin frame; empty
-// when a block holds no atom), kept in LDS as 8 floats per block, and the
-// test that a receptor atom is farther from a block's box than every cutoff.
-constexpr int kBoxFloats = 8;
-
-__device__ __forceinline__ int32_t wave_min(int32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-  return __builtin_amdgcn_readfirstlane(v);
-}
-
-__device__ __forceinline__ int32_t wave_max(int32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-  return __builtin_amdgcn_readfirstlane(v);
-}
+// when a block holds no atom), kept in LDS per block as the block's box and
+// the boxes of its four kLigSub-lane quarters (8 floats each), and the test
+// that a receptor atom is farther from a box than every cutoff.
+constexpr int kBoxFloats = 8 * (1 + kLigBlock / kLigSub);
 
 // The distance from a point to any point of a box is at least the
 // point-to-box distance; FP32 rounding of it (and of the fixed-point to A
@@ -177,9 +166,22 @@ __device__ __forceinline__ bool beyond_reach(float rx, float ry, float rz, const
   return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
 }
 
+// The same test for every point of the box [lo, hi]: for a point inside it
+// each difference below is at most the point's (FP32 subtraction is
+// monotone), so a box beyond reach implies every point in it is, as
+// beyond_reach would find it.
+__device__ __forceinline__ bool box_beyond_reach(float4 lo, float4 hi, const float* __restrict__ b) {
+  const float dx = fmaxf(fmaxf(b[0] - hi.x, lo.x - b[3]), 0.0f);
+  const float dy = fmaxf(fmaxf(b[1] - hi.y, lo.y - b[4]), 0.0f);
+  const float dz = fmaxf(fmaxf(b[2] - hi.z, lo.z - b[5]), 0.0f);
+  return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
+}
+
 // Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
 // t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
-// slot against every block; padding slots sit far from every block).
+// slot against every block; padding slots sit far from every block), and,
+// for a block some slot reaches, one of the block's quarters.  A culled pair
+// would only have added exact zeros, so the tests decide work, not results.
 template <int K>
 __device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int g1, const float* __restrict__ boxes,
                                                uint64_t (&keep)[K]) {
@@ -189,8 +191,26 @@ __device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int
   const float rx = static_cast<float>(r.x) * unit, ry = static_cast<float>(r.y) * unit,
               rz = static_cast<float>(r.z) * unit;
 #pragma unroll
-  for (int k = 0; k < K; ++k)
-    keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(rx, ry, rz, boxes + kBoxFloats * k));
+  for (int k = 0; k < K; ++k) {
+    const float* b = boxes + kBoxFloats * k;
+    keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(rx, ry, rz, b));
+    if (keep[k] != 0ull)
+      keep[k] &= __builtin_amdgcn_ballot_w64(!beyond_reach(rx, ry, rz, b + 8) || !beyond_reach(rx, ry, rz, b + 16) ||
+                                             !beyond_reach(rx, ry, rz, b + 24) || !beyond_reach(rx, ry, rz, b + 32));
+  }
+}
+
+// Minimum and maximum over the 16-lane quarter of the wave holding this lane.
+__device__ __forceinline__ int32_t quarter_min(int32_t v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
+__device__ __forceinline__ int32_t quarter_max(int32_t v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
 }
 
 template <int K>
@@ -554,11 +574,11 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const bool in = lig_index<K>(base, k) < cx.n_lig;
-      const int32_t x0 = wave_min(in ? a.x[k] : INT_MAX), y0 = wave_min(in ? a.y[k] : INT_MAX),
-                    z0 = wave_min(in ? a.z[k] : INT_MAX), x1 = wave_max(in ? a.x[k] : INT_MIN),
-                    y1 = wave_max(in ? a.y[k] : INT_MIN), z1 = wave_max(in ? a.z[k] : INT_MIN);
-      if (lane == 0) {
-        float* b = my_box + kBoxFloats * k;
+      // quarter boxes first, then the block's box from them
+      int32_t x0 = quarter_min(in ? a.x[k] : INT_MAX), y0 = quarter_min(in ? a.y[k] : INT_MAX),
+              z0 = quarter_min(in ? a.z[k] : INT_MAX), x1 = quarter_max(in ? a.x[k] : INT_MIN),
+              y1 = quarter_max(in ? a.y[k] : INT_MIN), z1 = quarter_max(in ? a.z[k] : INT_MIN);
+      auto put_box = [unit](float* b, int32_t x0, int32_t y0, int32_t z0, int32_t x1, int32_t y1, int32_t z1) {
         const bool empty = x0 > x1;
         b[0] = empty ? INFINITY : static_cast<float>(x0) * unit;
         b[1] = empty ? INFINITY : static_cast<float>(y0) * unit;
@@ -566,12 +586,36 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         b[3] = empty ? -INFINITY : static_cast<float>(x1) * unit;
         b[4] = empty ? -INFINITY : static_cast<float>(y1) * unit;
         b[5] = empty ? -INFINITY : static_cast<float>(z1) * unit;
+      };
+      if ((lane & (kLigSub - 1)) == 0)
+        put_box(my_box + kBoxFloats * k + 8 * (1 + lane / kLigSub), x0, y0, z0, x1, y1, z1);
+#pragma unroll
+      for (int off = kLigSub; off < 64; off <<= 1) {
+        x0 = min(x0, __shfl_xor(x0, off)); y0 = min(y0, __shfl_xor(y0, off)); z0 = min(z0, __shfl_xor(z0, off));
+        x1 = max(x1, __shfl_xor(x1, off)); y1 = max(y1, __shfl_xor(y1, off)); z1 = max(z1, __shfl_xor(z1, off));
       }
+      if (lane == 0) put_box(my_box + kBoxFloats * k, x0, y0, z0, x1, y1, z1);
     }
     __syncthreads();
     int nev = 0;
     int cls = -1, cend = 0;
-    for (int g0 = pp.r0; g0 < pp.r1; g0 += 64) {
+    // Receptor sub-tiles (kRecSub slots) near any of the wave's blocks, 64
+    // at a time (lane t tests sub-tile s0 + t); a 64-slot tile is walked when
+    // one of its four sub-tiles is.  Parts start on whole tiles.
+    const int s_end = (pp.r1 + kRecSub - 1) / kRecSub;
+    for (int s0 = pp.r0 / kRecSub; s0 < s_end; s0 += 64) {
+      const int si = s0 + lane;
+      bool near = false;
+      if (si < s_end) {
+        const float4 lo = cx.rec_sub_box[2 * si], hi = cx.rec_sub_box[2 * si + 1];
+#pragma unroll
+        for (int k = 0; k < K; ++k) near = near || !box_beyond_reach(lo, hi, my_box + kBoxFloats * k);
+      }
+      uint64_t tiles = __builtin_amdgcn_ballot_w64(near);
+      while (tiles != 0ull) {
+      const int first = (__builtin_ctzll(tiles) / (64 / kRecSub)) * (64 / kRecSub);  // the tile's first sub-tile
+      tiles &= ~(((1ull << (64 / kRecSub)) - 1ull) << first);
+      const int g0 = (s0 + first) * kRecSub;
       const int g1 = min(g0 + 64, pp.r1);
       // A receptor atom beyond every cutoff from all atoms of a ligand block
       // adds nothing and holds no band pair with it: that block skips the
@@ -647,6 +691,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       for (int k = 0; k < K; ++k) {  // fold the tile's FP32 LJ sums into FP64
         sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]) * static_cast<double>(kRhoScale);
         a.acv[k] = 0.0f;
+      }
       }
     }
     if (!EXACT && nev > 0)
