@@ -142,6 +142,7 @@ def lib() -> C.CDLL:
         "gdock_time_lanes": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
         "gdock_ga_default_params": (C.c_int, [C.POINTER(GaParams)]),
         "gdock_ga_run": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
+"gdock_ga_run_device": (C.c_int, [P, P, C.POINTER(Weights), C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_free": (None, [P]),
         "gdock_ga_run_dockq": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(GaParams), C.POINTER(P)]),
         "gdock_ga_info": (C.c_int, [P, I32, I32, I32, I32, I32]),
@@ -150,6 +151,8 @@ def lib() -> C.CDLL:
         "gdock_ga_clusters": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_ga_selected": (C.c_int32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "gdock_rng_stream": (None, [C.c_uint64, C.c_int32, C.POINTER(C.c_uint64)]),
+        "gdock_rng_normals": (None, [C.c_uint64, C.c_int32, C.c_double, C.c_double, D]),
+        "gdock_stream_math": (None, [D, C.c_int32, C.c_int32, D]),
         "gdock_evaluator_create": (C.c_int, [P, P, P, C.c_int32, C.POINTER(P)]),
         "gdock_evaluator_destroy": (None, [P, P]),
         "gdock_evaluator_info": (C.c_int, [P, I32, I32, I32]),

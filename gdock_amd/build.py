@@ -13,7 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libgdock_hip.so"
-SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip", "eval.hip", "batch.cpp", "lanes.cpp", "ga.cpp", "serve.cpp", "output.cpp"]
+SOURCES = ["api.cpp", "pdb.cpp", "score.hip", "fcc.hip", "eval.hip", "batch.cpp", "lanes.cpp", "ga.cpp", "ga_device.hip", "serve.cpp", "output.cpp"]
 
 # -ffp-contract=off: FP64 sums and products round as in the reference (no
 # implicit FMA); the FP32 pair loop spells its FMAs out.  Packed FP32

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "ga_device.h"
 #include "internal.h"
 
 namespace gdock {
@@ -36,56 +37,6 @@ double angular_difference(double a, double b) {  // run.rs:160-169
 
 bool by_fitness(const Individual& a, const Individual& b) { return a.fitness < b.fitness; }
 }  // namespace
-
-// ---- random stream ---------------------------------------------------------
-Xoshiro256::Xoshiro256(uint64_t seed) {
-  uint64_t x = seed;
-  for (uint64_t& v : s_) {  // SplitMix64
-    x += 0x9e3779b97f4a7c15ull;
-    uint64_t z = x;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    v = z ^ (z >> 31);
-  }
-}
-
-uint64_t Xoshiro256::next() {
-  auto rotl = [](uint64_t v, int k) { return (v << k) | (v >> (64 - k)); };
-  const uint64_t out = rotl(s_[1] * 5, 7) * 9;
-  const uint64_t t = s_[1] << 17;
-  s_[2] ^= s_[0];
-  s_[3] ^= s_[1];
-  s_[1] ^= s_[2];
-  s_[0] ^= s_[3];
-  s_[2] ^= t;
-  s_[3] = rotl(s_[3], 45);
-  return out;
-}
-
-double Xoshiro256::unit() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
-
-double Xoshiro256::range_closed(double lo, double hi) {
-  const double v = lo + (hi - lo) * (static_cast<double>(next() >> 11) / 9007199254740991.0);
-  return v > hi ? hi : v;
-}
-
-uint64_t Xoshiro256::below(uint64_t n) {  // Lemire's multiply-shift with rejection
-  unsigned __int128 m = static_cast<unsigned __int128>(next()) * n;
-  uint64_t low = static_cast<uint64_t>(m);
-  if (low < n) {
-    const uint64_t threshold = (0 - n) % n;
-    while (low < threshold) {
-      m = static_cast<unsigned __int128>(next()) * n;
-      low = static_cast<uint64_t>(m);
-    }
-  }
-  return static_cast<uint64_t>(m >> 64);
-}
-
-double Xoshiro256::normal(double mean, double sd) {  // Box-Muller, cosine branch
-  const double u1 = 1.0 - unit(), u2 = unit();
-  return mean + sd * (std::sqrt(-2.0 * std::log(u1)) * std::cos(kTwoPi * u2));
-}
 
 // ---- Hall of Fame -------------------------------------------------------------
 bool HallOfFame::offer(const double* g, double fitness, uint32_t generation) {
@@ -183,6 +134,23 @@ const std::vector<double>& GaJob::final_genes() {
   }
   final_bred_ = true;
   return genes_;
+}
+
+void GaJob::load_device_run(int generations, int converged_at, std::vector<HofEntry> hof,
+                            std::vector<double> final_genes, std::vector<double> trace_g, std::vector<double> trace_f,
+                            std::vector<int32_t> trace_b) {
+  generations_ = generations;
+  generation_ = generations;
+  converged_at_ = converged_at;
+  hof_.assign(std::move(hof));
+  genes_ = std::move(final_genes);
+  pop_.resize(genes_.size() / 6);
+  for (size_t i = 0; i < pop_.size(); ++i) std::memcpy(pop_[i].genes, &genes_[6 * i], 6 * sizeof(double));
+  trace_genes = std::move(trace_g);
+  trace_fitness = std::move(trace_f);
+  trace_best = std::move(trace_b);
+  done_ = true;
+  final_bred_ = true;  // the device bred the final population already
 }
 
 void GaJob::breed() {
@@ -305,6 +273,33 @@ gdock_status gdock_ga_default_params(gdock_ga_params* p) {
   return GDOCK_OK;
 }
 
+namespace {
+// Scores one generation of the host driver: genes up, pose records made on
+// the device (as the device GA makes them, so both drivers score the same
+// records), the plain scoring launch, results down.
+gdock_status score_generation(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
+                              const gdock_weights* w, double* out) {
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "ga: set device");
+  Layout L;
+  const size_t o_g = L.add<double>(static_cast<size_t>(n) * 6), o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
+               o_o = L.add<double>(static_cast<size_t>(n) * kOutDoubles), o_s = L.add<uint8_t>(plain_scratch_bytes(cx->dev, n));
+  if (!ctx->staging.reserve(L.size, &e)) return hip_fail(ctx, e, "ga: staging");
+  char* m = static_cast<char*>(ctx->staging.ptr);
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  e = hipMemcpyAsync(m + o_g, genes, static_cast<size_t>(n) * 6 * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_ga_pose_records(reinterpret_cast<double*>(m + o_g), n, reinterpret_cast<double*>(m + o_p), ctx->stream);
+  if (e == hipSuccess)
+    e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(m + o_p), nullptr, n, ww,
+                     reinterpret_cast<double*>(m + o_o), nullptr, m + o_s, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, m + o_o, static_cast<size_t>(n) * kOutDoubles * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GDOCK_OK : hip_fail(ctx, e, "ga: generation");
+}
+}  // namespace
+
 gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
                           const gdock_ga_params* params, gdock_ga_result** out) {
   if (!ctx || !cx || !w || !out) {
@@ -319,7 +314,9 @@ gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights
   GaJob& job = r->job;
   std::vector<double> res(static_cast<size_t>(job.population()) * 6);
   while (!job.done()) {  // run.rs:361-531, one plain scoring launch per generation
-    st = gdock_score_poses(ctx, cx, job.genes().data(), job.population(), w, res.data(), nullptr);
+    for (double g : job.genes())
+      if (!std::isfinite(g)) { delete r; set_error(ctx, "ga_run: non-finite gene"); return GDOCK_ERR_INVALID_ARG; }
+    st = score_generation(ctx, cx, job.genes().data(), job.population(), w, res.data());
     if (st != GDOCK_OK) { delete r; return st; }
     job.absorb(res.data());
   }
@@ -421,6 +418,16 @@ int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t*
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out) {
   Xoshiro256 r(seed);
   for (int32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+void gdock_rng_normals(uint64_t seed, int32_t n, double mean, double sd, double* out) {
+  Xoshiro256 r(seed);
+  for (int32_t i = 0; i < n; ++i) out[i] = r.normal(mean, sd);
+}
+
+void gdock_stream_math(const double* x, int32_t n, int32_t which, double* out) {
+  for (int32_t i = 0; i < n; ++i)
+    out[i] = which == 0 ? stream::ln(x[i]) : (which == 1 ? stream::cos2pi(x[i]) : stream::sqrt_cr(x[i]));
 }
 
 }  // extern "C"

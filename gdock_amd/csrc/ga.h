@@ -6,25 +6,11 @@
 #include <vector>
 
 #include "../../include/gdock_hip.h"
+#include "stream.h"
 
 namespace gdock {
 
-// Project-defined random stream: state seeded by SplitMix64, xoshiro256**
-// generator, Lemire's unbiased bounded integers, Box-Muller normals (one per
-// call).  The reference draws from rand's StdRng; the algorithms consuming
-// the stream follow the reference step by step.
-class Xoshiro256 {
- public:
-  explicit Xoshiro256(uint64_t seed);
-  uint64_t next();
-  double unit();                              // [0, 1), 53 bits
-  double range_closed(double lo, double hi);  // [lo, hi]
-  uint64_t below(uint64_t n);                 // [0, n)
-  double normal(double mean, double sd);
-
- private:
-  uint64_t s_[4];
-};
+using Xoshiro256 = stream::Xoshiro256;  // the project stream (stream.h)
 
 struct Individual {
   double genes[6];
@@ -44,6 +30,7 @@ class HallOfFame {
   bool offer(const double* genes, double fitness, uint32_t generation);
   void offer_population(const std::vector<Individual>& pop, uint32_t generation);
   const std::vector<HofEntry>& entries() const { return entries_; }
+  void assign(std::vector<HofEntry> e) { entries_ = std::move(e); }
 
  private:
   std::vector<HofEntry> entries_;
@@ -74,6 +61,11 @@ class GaJob {
   // it on first call once the job is done.
   const std::vector<double>& final_genes();
   const std::vector<GaSelected>& selected() const { return selected_; }
+  // The stream's state after the initial population was drawn (a device GA
+  // continues from it), and the end state of a job the device GA ran.
+  const uint64_t* rng_state() const { return rng_.s; }
+  void load_device_run(int generations, int converged_at, std::vector<HofEntry> hof, std::vector<double> final_genes,
+                       std::vector<double> trace_g, std::vector<double> trace_f, std::vector<int32_t> trace_b);
   // per evaluated generation: genes (pop x 6), fitness (pop), best index
   std::vector<double> trace_genes, trace_fitness;
   std::vector<int32_t> trace_best;

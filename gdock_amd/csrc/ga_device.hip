@@ -1,0 +1,511 @@
+// The GA generation on the device: one workgroup per job takes its
+// population's scores, offers the best to the Hall of Fame, tests
+// convergence and breeds the next population (run.rs:381-531,
+// population.rs:74-158, ga.rs:7-39, chromosome.rs:56-77), then writes the
+// next generation's pose records, so consecutive generations need nothing
+// from the host.  Every decision and every draw of the project stream
+// (stream.h) follows the host driver (ga.cpp) step by step: a job run here is
+// bit-identical to the same job run by the host driver.
+//
+// The parts that are sequential by nature - tournaments, the Fisher-Yates
+// shuffle, crossover and mutation, whose draws come one after another from
+// one stream - run on one lane; ranking, the Hall of Fame's similarity tests
+// and sorts, trace copies and pose records run on the whole workgroup.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gdock_hip.h"
+#include "common.h"
+#include "ga.h"
+#include "ga_device.h"
+#include "internal.h"
+#include "stream.h"
+
+namespace gdock {
+
+namespace {
+constexpr double kTwoPi = 2.0 * kPI;
+constexpr int kGaThreads = 256;
+}  // namespace
+
+// structure.rs:96-103 through the pose record of the scoring kernels: genes,
+// then cos and sin of the three angles and the four mixed products (the
+// bracketed sums, no contraction).  The device's sin and cos.
+__device__ __forceinline__ void write_pose_record(const double* __restrict__ g, double* __restrict__ p) {
+  for (int k = 0; k < 6; ++k) p[k] = g[k];
+  double ca, sa, cb, sb, cg, sg;
+  sincos(g[0], &sa, &ca);
+  sincos(g[1], &sb, &cb);
+  sincos(g[2], &sg, &cg);
+  p[6] = ca; p[7] = sa; p[8] = cb; p[9] = sb; p[10] = cg; p[11] = sg;
+  p[12] = (-sa * cg) + ((ca * sb) * sg);
+  p[13] = (sa * sg) + ((ca * sb) * cg);
+  p[14] = (ca * cg) + ((sa * sb) * sg);
+  p[15] = (-ca * sg) + ((sa * sb) * cg);
+}
+
+__global__ void ga_pose_records_kernel(const double* __restrict__ genes, int n, double* __restrict__ poses) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) write_pose_record(genes + 6 * i, poses + static_cast<size_t>(kPoseDoubles) * i);
+}
+
+hipError_t launch_ga_pose_records(const double* d_genes, int n, double* d_poses, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ga_pose_records_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_genes, n, d_poses);
+  return hipGetLastError();
+}
+
+// run.rs:160-169 and 139-158: a pose within 0.2 rad per angle and 2 A per
+// shift of a kept one is the same pose.
+__device__ __forceinline__ bool hof_similar(const double* __restrict__ g, const double* __restrict__ h) {
+  for (int i = 0; i < 3; ++i) {
+    double d = fabs(g[i] - h[i]);
+    d = d > kPI ? kTwoPi - d : d;
+    if (!(d <= 0.2)) return false;
+  }
+  for (int i = 3; i < 6; ++i)
+    if (!(fabs(g[i] - h[i]) <= 2.0)) return false;
+  return true;
+}
+
+// Rust's f64::rem_euclid(a, 2 pi) as the host computes it (fmod, then + 2 pi
+// when negative); fmod is exact, and for |a| < 4 pi it is a, a - 2 pi or
+// a + 2 pi, each exact by Sterbenz's lemma.
+__device__ __forceinline__ double rem_euclid_2pi(double a) {
+  double r;
+  if (fabs(a) < kTwoPi) r = a;
+  else if (a >= kTwoPi && a < 2.0 * kTwoPi) r = a - kTwoPi;
+  else if (a <= -kTwoPi && a > -2.0 * kTwoPi) r = a + kTwoPi;
+  else r = fmod(a, kTwoPi);
+  return r < 0.0 ? r + kTwoPi : r;
+}
+
+__device__ __forceinline__ void copy_entry(const DevGaJob& J, int from, double* tg, double* tf, int32_t* tn, int to) {
+  for (int k = 0; k < 6; ++k) tg[6 * to + k] = J.hof_genes[6 * from + k];
+  tf[to] = J.hof_fit[from];
+  tn[to] = J.hof_gen[from];
+}
+
+// The Hall of Fame holds kGaHofCapacity + 1 entries: the one just appended
+// (run.rs:171-177) makes it sort by fitness, stably, and drop the last.  The
+// first time the entries are in insertion order (a full stable sort through
+// the spare arrays); from then on they stay sorted, and the stable sort of a
+// sorted run plus one entry puts that entry after every entry of lower or
+// equal fitness.
+__device__ void hof_trim(const DevGaJob& J, bool sorted) {
+  const int tid = threadIdx.x, n = kGaHofCapacity + 1;
+  if (!sorted) {
+    for (int e = tid; e < n; e += kGaThreads) {
+      const double fe = J.hof_fit[e];
+      int r = 0;
+      for (int j = 0; j < n; ++j) {
+        const double fj = J.hof_fit[j];
+        r += (fj < fe || (fj == fe && j < e)) ? 1 : 0;
+      }
+      copy_entry(J, e, J.hof_tmp_genes, J.hof_tmp_fit, J.hof_tmp_gen, r);
+    }
+    __syncthreads();
+    for (int e = tid; e < kGaHofCapacity; e += kGaThreads) {
+      for (int k = 0; k < 6; ++k) J.hof_genes[6 * e + k] = J.hof_tmp_genes[6 * e + k];
+      J.hof_fit[e] = J.hof_tmp_fit[e];
+      J.hof_gen[e] = J.hof_tmp_gen[e];
+    }
+    __syncthreads();
+    return;
+  }
+  const double fn = J.hof_fit[kGaHofCapacity];
+  int cnt = 0;
+  for (int e = tid; e < kGaHofCapacity; e += kGaThreads) cnt += J.hof_fit[e] <= fn ? 1 : 0;
+  __shared__ int s_cnt;
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  if (cnt) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  const int p = s_cnt;
+  if (p < kGaHofCapacity) {  // shift [p, capacity - 1) up by one, then place the new entry at p
+    double g[2][6], f[2];
+    int32_t gen[2];
+    int m = 0;
+    for (int e = p + tid; e < kGaHofCapacity - 1; e += kGaThreads, ++m) {
+      for (int k = 0; k < 6; ++k) g[m][k] = J.hof_genes[6 * e + k];
+      f[m] = J.hof_fit[e];
+      gen[m] = J.hof_gen[e];
+    }
+    __syncthreads();
+    m = 0;
+    for (int e = p + tid; e < kGaHofCapacity - 1; e += kGaThreads, ++m) {
+      for (int k = 0; k < 6; ++k) J.hof_genes[6 * (e + 1) + k] = g[m][k];
+      J.hof_fit[e + 1] = f[m];
+      J.hof_gen[e + 1] = gen[m];
+    }
+    __syncthreads();
+    if (tid == 0) copy_entry(J, kGaHofCapacity, J.hof_genes, J.hof_fit, J.hof_gen, p);
+  }
+  __syncthreads();
+}
+
+// One generation of every listed job (one workgroup each); see the file
+// comment.  A job that has ended does nothing.
+__global__ __launch_bounds__(kGaThreads) void ga_generation_kernel(const DevGaJob* __restrict__ jobs) {
+  __shared__ double s_fit[kGaMaxPopulation];    // fitness of the population; then the children's inherited fitness
+  __shared__ double s_cfit[kGaMaxPopulation];
+  __shared__ int32_t s_order[kGaMaxPopulation];  // ascending stable order of the population by fitness
+  __shared__ int32_t s_chosen[kGaMaxPopulation];
+  __shared__ int32_t s_pos[kGaMaxPopulation];    // each child's place in the descending stable order
+  const DevGaJob J = jobs[blockIdx.x];
+  DevGaState* st = J.st;
+  const int tid = threadIdx.x;
+  if (st->done) return;
+  const int n = st->n, cur = st->cur, gen = st->generation;
+  const double* __restrict__ genes = J.genes[cur];
+  const DevGaParams P = J.p;
+  for (int i = tid; i < n; i += kGaThreads) s_fit[i] = J.out[static_cast<size_t>(kOutDoubles) * i + 5];
+  __syncthreads();
+  for (int i = tid; i < n; i += kGaThreads) {
+    const double fi = s_fit[i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double fj = s_fit[j];
+      r += (fj < fi || (fj == fi && j < i)) ? 1 : 0;
+    }
+    s_order[r] = i;
+  }
+  __syncthreads();
+  const int trace_at = st->trace_at;
+  if (P.keep_trace) {
+    for (int i = tid; i < 6 * n; i += kGaThreads) J.trace_genes[6 * static_cast<size_t>(trace_at) + i] = genes[i];
+    for (int i = tid; i < n; i += kGaThreads) J.trace_fit[trace_at + i] = s_fit[i];
+    if (tid == 0) J.trace_best[gen] = s_order[0];
+  }
+  // run.rs:194-216: the population's best, in order, until 10 are kept
+  int hof_n = st->hof_n;
+  bool sorted = st->hof_sorted != 0;
+  int added = 0;
+  for (int r = 0; r < n && added < kGaHofPerGeneration; ++r) {
+    const int i = s_order[r];
+    double g[6];
+    for (int k = 0; k < 6; ++k) g[k] = genes[6 * i + k];
+    bool sim = false;
+    for (int e = tid; e < hof_n && !sim; e += kGaThreads) sim = hof_similar(g, J.hof_genes + 6 * e);
+    if (__syncthreads_or(sim ? 1 : 0)) continue;
+    if (tid == 0) {
+      for (int k = 0; k < 6; ++k) J.hof_genes[6 * hof_n + k] = g[k];
+      J.hof_fit[hof_n] = s_fit[i];
+      J.hof_gen[hof_n] = gen;
+    }
+    ++hof_n;
+    ++added;
+    __syncthreads();
+    if (hof_n > kGaHofCapacity) {
+      hof_trim(J, sorted);
+      sorted = true;
+      hof_n = kGaHofCapacity;
+    }
+  }
+  // run.rs:480-531: the first individual of least fitness; the relative
+  // improvement of the best against the threshold; early stop or the last
+  // generation
+  const int best = s_order[0];
+  const double f = s_fit[best];
+  int stall = st->stall, converged_at = st->converged_at, done = 0;
+  if (gen > 0) {
+    const double last = st->last_best;
+    const double improvement = (last - f) / fabs(last);
+    stall = improvement < P.threshold ? stall + 1 : 0;
+  }
+  int next_gen = gen;
+  bool final_breed = false;
+  if (P.early && stall >= P.window) {
+    converged_at = gen;
+    done = 1;
+  } else {
+    ++next_gen;
+    if (next_gen >= P.max_gen) {
+      done = 1;
+      final_breed = true;  // the loop evolves once more after its last generation (run.rs:530)
+    }
+  }
+  __syncthreads();  // every thread has read the state
+  if (tid == 0) {
+    st->hof_n = hof_n;
+    st->hof_sorted = sorted ? 1 : 0;
+    st->stall = stall;
+    st->last_best = f;
+    st->generations = gen + 1;
+    st->generation = next_gen;
+    st->converged_at = converged_at;
+    st->trace_at = trace_at + n;
+    st->done = done;
+  }
+  if (done && !final_breed) return;
+
+  // population.rs:74-158: breed the next population
+  const int n_child = n & ~1;
+  const int elite = min(P.elitism, n_child);
+  if (tid == 0) {
+    stream::Xoshiro256 rng;
+    for (int k = 0; k < 4; ++k) rng.s[k] = st->rng[k];
+    for (int k = 0; k < n; ++k) {  // tournaments: the first of least fitness wins
+      int b = -1;
+      for (int t = 0; t < P.tsize; ++t) {
+        const int c = static_cast<int>(rng.below(static_cast<uint64_t>(n)));
+        if (b < 0 || s_fit[c] < s_fit[b]) b = c;
+      }
+      s_chosen[k] = b;
+    }
+    for (int i = n; i > 1; --i) {  // Fisher-Yates
+      const int j = static_cast<int>(rng.below(static_cast<uint64_t>(i)));
+      const int t = s_chosen[i - 1];
+      s_chosen[i - 1] = s_chosen[j];
+      s_chosen[j] = t;
+    }
+    for (int m = 0; 2 * m + 1 < n; ++m) {  // uniform crossover of neighbours
+      const int a = s_chosen[2 * m], b = s_chosen[2 * m + 1];
+      double* ca = J.child + 12 * m;
+      double* cb = ca + 6;
+      for (int g = 0; g < 6; ++g) {
+        double x = genes[6 * a + g], y = genes[6 * b + g];
+        if (rng.unit() <= P.cross_rate) {
+          const double t = x;
+          x = y;
+          y = t;
+        }
+        ca[g] = x;
+        cb[g] = y;
+      }
+      s_cfit[2 * m] = s_fit[a];
+      s_cfit[2 * m + 1] = s_fit[b];
+    }
+    for (int c = 0; c < n_child; ++c)  // mutation
+      for (int g = 0; g < 6; ++g) {
+        if (!(rng.unit() <= P.mut_rate)) continue;
+        double& v = J.child[6 * c + g];
+        if (g < 3)
+          v = rem_euclid_2pi(v + rng.normal(0.0, 0.174));
+        else
+          v = fmin(fmax(v + rng.normal(0.0, 1.0), -P.max_disp), P.max_disp);
+      }
+    for (int k = 0; k < 4; ++k) st->rng[k] = rng.s[k];
+  }
+  __syncthreads();
+  // the elite replace the children that inherited the worst fitness: stable
+  // descending order of the children, then the elite over its first places
+  for (int c = tid; c < n_child; c += kGaThreads) {
+    const double fc = s_cfit[c];
+    int r = 0;
+    for (int j = 0; j < n_child; ++j) {
+      const double fj = s_cfit[j];
+      r += (fj > fc || (fj == fc && j < c)) ? 1 : 0;
+    }
+    s_pos[c] = r;
+  }
+  __syncthreads();
+  double* __restrict__ out_genes = J.genes[cur ^ 1];
+  for (int c = tid; c < n_child; c += kGaThreads) {
+    const int q = s_pos[c];
+    if (q >= elite)
+      for (int g = 0; g < 6; ++g) out_genes[6 * q + g] = J.child[6 * c + g];
+  }
+  for (int q = tid; q < elite; q += kGaThreads)
+    for (int g = 0; g < 6; ++g) out_genes[6 * q + g] = genes[6 * s_order[q] + g];
+  __syncthreads();
+  if (!done)
+    for (int q = tid; q < n_child; q += kGaThreads)
+      write_pose_record(out_genes + 6 * q, J.poses + static_cast<size_t>(kPoseDoubles) * q);
+  if (tid == 0) {
+    st->n = n_child;
+    st->cur = cur ^ 1;
+  }
+}
+
+hipError_t launch_ga_generation(const DevGaJob* d_jobs, int n_jobs, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ga_generation_kernel, dim3(n_jobs), dim3(kGaThreads), 0, s, d_jobs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- host side
+size_t ga_job_bytes(int max_pop, int max_gen, bool trace) {
+  Layout L;
+  ga_job_layout(&L, max_pop, max_gen, trace, nullptr);
+  return L.size;
+}
+
+void ga_job_layout(Layout* L, int max_pop, int max_gen, bool trace, GaJobOffsets* o) {
+  GaJobOffsets t{};
+  t.state = L->add<DevGaState>(1);
+  t.genes0 = L->add<double>(static_cast<size_t>(max_pop) * 6);
+  t.genes1 = L->add<double>(static_cast<size_t>(max_pop) * 6);
+  t.child = L->add<double>(static_cast<size_t>(max_pop) * 6);
+  t.hof_genes = L->add<double>((kGaHofCapacity + 1) * 6);
+  t.hof_fit = L->add<double>(kGaHofCapacity + 1);
+  t.hof_gen = L->add<int32_t>(kGaHofCapacity + 1);
+  t.tmp_genes = L->add<double>((kGaHofCapacity + 1) * 6);
+  t.tmp_fit = L->add<double>(kGaHofCapacity + 1);
+  t.tmp_gen = L->add<int32_t>(kGaHofCapacity + 1);
+  if (trace) {
+    t.trace_genes = L->add<double>(static_cast<size_t>(max_gen) * max_pop * 6);
+    t.trace_fit = L->add<double>(static_cast<size_t>(max_gen) * max_pop);
+    t.trace_best = L->add<int32_t>(max_gen);
+  }
+  if (o) *o = t;
+}
+
+DevGaJob ga_job_bind(char* base, const GaJobOffsets& o, const gdock_ga_params& p, double* d_poses,
+                     const double* d_out) {
+  DevGaJob J{};
+  J.st = reinterpret_cast<DevGaState*>(base + o.state);
+  J.genes[0] = reinterpret_cast<double*>(base + o.genes0);
+  J.genes[1] = reinterpret_cast<double*>(base + o.genes1);
+  J.child = reinterpret_cast<double*>(base + o.child);
+  J.hof_genes = reinterpret_cast<double*>(base + o.hof_genes);
+  J.hof_fit = reinterpret_cast<double*>(base + o.hof_fit);
+  J.hof_gen = reinterpret_cast<int32_t*>(base + o.hof_gen);
+  J.hof_tmp_genes = reinterpret_cast<double*>(base + o.tmp_genes);
+  J.hof_tmp_fit = reinterpret_cast<double*>(base + o.tmp_fit);
+  J.hof_tmp_gen = reinterpret_cast<int32_t*>(base + o.tmp_gen);
+  if (p.keep_trace) {
+    J.trace_genes = reinterpret_cast<double*>(base + o.trace_genes);
+    J.trace_fit = reinterpret_cast<double*>(base + o.trace_fit);
+    J.trace_best = reinterpret_cast<int32_t*>(base + o.trace_best);
+  }
+  J.poses = d_poses;
+  J.out = d_out;
+  J.p.pop = p.population_size;
+  J.p.max_gen = p.max_generations;
+  J.p.elitism = p.elitism;
+  J.p.tsize = p.tournament_size;
+  J.p.window = p.convergence_window;
+  J.p.early = p.early_stopping;
+  J.p.keep_trace = p.keep_trace;
+  J.p.mut_rate = p.mutation_rate;
+  J.p.cross_rate = p.crossover_rate;
+  J.p.threshold = p.convergence_threshold;
+  J.p.max_disp = p.max_displacement;
+  return J;
+}
+
+DevGaState ga_initial_state(const GaJob& job) {
+  DevGaState s{};
+  for (int k = 0; k < 4; ++k) s.rng[k] = job.rng_state()[k];
+  s.n = job.population();
+  s.cur = 0;
+  s.generation = 0;
+  s.generations = 0;
+  s.converged_at = -1;
+  s.stall = 0;
+  s.done = 0;
+  s.hof_n = 0;
+  s.hof_sorted = 0;
+  s.trace_at = 0;
+  s.last_best = 1.7976931348623157e308;
+  return s;
+}
+
+hipError_t ga_job_download(const DevGaJob& J, const gdock_ga_params& p, hipStream_t s, GaJob* job) {
+  DevGaState st{};
+  hipError_t e = hipMemcpyAsync(&st, J.st, sizeof st, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  const int hn = st.hof_n, n = st.n;
+  std::vector<double> hg(static_cast<size_t>(hn) * 6), hf(hn), fin(static_cast<size_t>(n) * 6);
+  std::vector<int32_t> hgen(hn);
+  std::vector<double> tg, tf;
+  std::vector<int32_t> tb;
+  if (hn) {
+    e = hipMemcpyAsync(hg.data(), J.hof_genes, hg.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hf.data(), J.hof_fit, hf.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hgen.data(), J.hof_gen, hgen.size() * 4, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess && n) e = hipMemcpyAsync(fin.data(), J.genes[st.cur], fin.size() * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && p.keep_trace) {
+    tg.resize(static_cast<size_t>(st.trace_at) * 6);
+    tf.resize(st.trace_at);
+    tb.resize(st.generations);
+    if (!tg.empty()) e = hipMemcpyAsync(tg.data(), J.trace_genes, tg.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !tf.empty()) e = hipMemcpyAsync(tf.data(), J.trace_fit, tf.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !tb.empty()) e = hipMemcpyAsync(tb.data(), J.trace_best, tb.size() * 4, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  std::vector<HofEntry> hof(hn);
+  for (int i = 0; i < hn; ++i) {
+    std::memcpy(hof[i].genes, &hg[6 * i], 6 * sizeof(double));
+    hof[i].fitness = hf[i];
+    hof[i].generation = static_cast<uint32_t>(hgen[i]);
+  }
+  job->load_device_run(st.generations, st.converged_at, std::move(hof), std::move(fin), std::move(tg), std::move(tf),
+                       std::move(tb));
+  return hipSuccess;
+}
+
+}  // namespace gdock
+
+using namespace gdock;
+
+extern "C" {
+
+gdock_status gdock_ga_run_device(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
+                                 const gdock_ga_params* params, gdock_ga_result** out) {
+  if (!ctx || !cx || !w || !out) {
+    if (ctx) set_error(ctx, "ga_run_device: invalid arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  gdock_ga_params p;
+  if (params) p = *params; else gdock_ga_default_params(&p);
+  gdock_status st = validate_ga_params(ctx, &p);
+  if (st != GDOCK_OK) return st;
+  if (p.population_size > kGaMaxPopulation) {
+    set_error(ctx, "ga_run_device: population %d exceeds the device GA's %d", p.population_size, kGaMaxPopulation);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "ga_run_device: set device");
+  auto* r = new gdock_ga_result(seed, p);  // draws the initial population, as the host driver does
+  const int pop = p.population_size;
+  Layout L;
+  GaJobOffsets o;
+  ga_job_layout(&L, pop, p.max_generations, p.keep_trace != 0, &o);
+  const size_t o_job = L.add<DevGaJob>(1), o_poses = L.add<double>(static_cast<size_t>(pop) * kPoseDoubles),
+               o_out = L.add<double>(static_cast<size_t>(pop) * kOutDoubles),
+               o_scr = L.add<uint8_t>(plain_scratch_bytes(cx->dev, pop));
+  char* m = nullptr;
+  e = hipMalloc(reinterpret_cast<void**>(&m), L.size);
+  if (e != hipSuccess) { delete r; return hip_fail(ctx, e, "ga_run_device: memory"); }
+  const DevGaJob J = ga_job_bind(m, o, p, reinterpret_cast<double*>(m + o_poses), reinterpret_cast<double*>(m + o_out));
+  const DevGaState s0 = ga_initial_state(r->job);
+  const Weights ww{w->vdw, w->elec, w->desolv, w->air};
+  e = hipMemcpyAsync(J.st, &s0, sizeof s0, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(J.genes[0], r->job.genes().data(), static_cast<size_t>(pop) * 6 * 8, hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m + o_job, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_ga_pose_records(J.genes[0], pop, J.poses, ctx->stream);
+  int n = pop;
+  while (e == hipSuccess) {  // one plain scoring launch and one GA launch per generation
+    e = launch_score(cx->dev, cx->d_dev, J.poses, nullptr, n, ww, const_cast<double*>(J.out), nullptr, m + o_scr,
+                     ctx->stream);
+    if (e == hipSuccess) e = launch_ga_generation(reinterpret_cast<const DevGaJob*>(m + o_job), 1, ctx->stream);
+    int32_t dn[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(&dn[0], &J.st->done, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&dn[1], &J.st->n, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess || dn[0]) break;
+    n = dn[1];
+  }
+  if (e == hipSuccess) e = ga_job_download(J, p, ctx->stream, &r->job);
+  (void)hipFree(m);
+  if (e != hipSuccess) { delete r; return hip_fail(ctx, e, "ga_run_device"); }
+  if (p.cluster) {
+    st = finish_job(ctx, cx, &r->job, &r->clusters);
+    if (st != GDOCK_OK) { delete r; return st; }
+  }
+  *out = r;
+  return GDOCK_OK;
+}
+
+}  // extern "C"

@@ -46,6 +46,16 @@ def run(cx: Complex, seed: int = 42, weights: constants.EnergyWeights | None = N
     return take_result(h, p)
 
 
+def run_device(cx: Complex, seed: int = 42, weights: constants.EnergyWeights | None = None, **params) -> GaResult:
+    """The same job with the GA generation on the device (gdock_ga_run_device)."""
+    p = default_params(**params)
+    w = weights_c(weights)
+    h = C.c_void_p()
+    check(lib().gdock_ga_run_device(cx.ctx.handle, cx.handle, C.byref(w), seed, C.byref(p), C.byref(h)),
+          cx.ctx.handle, "gdock_ga_run_device")
+    return take_result(h, p)
+
+
 def run_dockq(cx: Complex, evaluator, seed: int = 42, **params) -> GaResult:
     """Debug mode (chromosome.rs:97-108): fitness = -DockQ against the evaluator's reference."""
     p = default_params(**params)
@@ -75,7 +85,8 @@ def take_result(h, p: GaParams) -> GaResult:
         si, ss = np.zeros(max(ns.value, 1), np.int64), np.zeros(max(ns.value, 1), np.int64)
         L.gdock_ga_selected(h, ptr(si, C.c_int64), ptr(ss, C.c_int64))
         fin = np.zeros((pop, 6))
-        L.gdock_ga_final_population(h, ptr(fin))
+        k = L.gdock_ga_final_population(h, ptr(fin))
+        fin = fin[:max(k, 0)]  # an odd population breeds one child fewer
         return GaResult(G, conv.value, genes, fit, best, hg, hf, hgen, cc, cs,
                         list(zip(si[:ns.value].tolist(), ss[:ns.value].tolist())), fin)
     finally:

@@ -224,6 +224,13 @@ gdock_status gdock_ga_default_params(gdock_ga_params* p);
 /* One job, one plain scoring launch per generation (the drop-in path). */
 gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
                           const gdock_ga_params* params, gdock_ga_result** out);
+/* The same job with every generation after the first drawn on the device:
+ * the GA kernel takes the scores, updates the Hall of Fame, tests
+ * convergence, breeds the next population with the same stream and writes
+ * its pose records, so the host only watches for the end.  The result equals
+ * gdock_ga_run's bit for bit.  population_size at most 1024. */
+gdock_status gdock_ga_run_device(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
+                                 const gdock_ga_params* params, gdock_ga_result** out);
 /* Debug mode (src/chromosome.rs:97-108, `--debug` with a reference): the
  * fitness is -DockQ from gdock_calc_metrics, the energy terms are zero. */
 typedef struct gdock_evaluator gdock_evaluator; /* see the DockQ evaluator section */
@@ -247,6 +254,12 @@ int32_t gdock_ga_selected(const gdock_ga_result* r, int64_t* hof_index, int64_t*
 int32_t gdock_ga_final_population(gdock_ga_result* r, double* genes);
 /* The first n outputs of the GA's random stream for a seed. */
 void gdock_rng_stream(uint64_t seed, int32_t n, uint64_t* out);
+/* The first n normal deviates (mean + sd z) the GA's stream draws for a seed. */
+void gdock_rng_normals(uint64_t seed, int32_t n, double mean, double sd, double* out);
+/* The stream's own functions behind its normals, element-wise: which 0: ln(x)
+ * (x > 0, normal); 1: cos(2 pi x) (0 <= x < 1); 2: the correctly rounded
+ * square root. */
+void gdock_stream_math(const double* x, int32_t n, int32_t which, double* out);
 
 /* ---- output stage ------------------------------------------------------------ */
 /* Atom::to_pdb_string over n atoms, concatenated as write_pdb writes them

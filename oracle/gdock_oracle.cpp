@@ -583,6 +583,51 @@ std::vector<ClusterResult> cluster_structures(const std::vector<Molecule>& struc
 #define ORACLE_MUTANT 0
 #endif
 
+// The project stream's logarithm and cosine (the library's stream.h defines
+// them): IEEE basic operations only, so the library's host and device code
+// and this restatement compute the same bits.  ln: x = m 2^e, m in
+// [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m - 1) / (m + 1), as the odd
+// series to s^25 in Horner form on z = s^2; plus e ln 2 with ln 2 split in a
+// high part exact under multiplication by e and a low part.
+inline double stream_ln(double x) {
+  uint64_t b;
+  std::memcpy(&b, &x, 8);
+  int e = static_cast<int>((b >> 52) & 0x7ff) - 1023;
+  const uint64_t mb = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m;
+  std::memcpy(&m, &mb, 8);
+  if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+  const double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+  double p = 1.0 / 25.0;
+  for (int k = 11; k >= 1; --k) p = 1.0 / static_cast<double>(2 * k + 1) + z * p;  // 1/23, ..., 1/3
+  const double s2 = 2.0 * s;
+  const double de = static_cast<double>(e);
+  return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + (s2 + s2 * (z * p)));
+}
+
+// cos(2 pi u), 0 <= u < 1: nearest quarter k (u - k/4 exact), x = 2 pi r,
+// Taylor series of cos (to x^18) or sin (to x^19) chosen by k mod 4.
+inline double stream_cos2pi(double u) {
+  const double k = static_cast<double>(static_cast<int64_t>(u * 4.0 + 0.5));
+  const double x = (u - k * 0.25) * 6.283185307179586, z = x * x;
+  double fact[20];
+  fact[0] = 1.0;
+  for (int i = 1; i < 20; ++i) fact[i] = fact[i - 1] * static_cast<double>(i);  // exact to 18!; 19! rounds once
+  double c = -1.0 / fact[18];
+  for (int i = 16; i >= 4; i -= 2) c = ((i / 2) % 2 ? -1.0 : 1.0) / fact[i] + z * c;
+  c = -0.5 + z * c;
+  const double cosx = 1.0 + z * c;
+  double sn = -1.0 / fact[19];
+  for (int i = 17; i >= 3; i -= 2) sn = ((i / 2) % 2 ? -1.0 : 1.0) / fact[i] + z * sn;
+  const double sinx = x + x * (z * sn);
+  switch (static_cast<int>(k) & 3) {
+    case 0: return cosx;
+    case 1: return -sinx;
+    case 2: return -cosx;
+    default: return sinx;
+  }
+}
+
 // Project-defined random stream: SplitMix64 seeding, xoshiro256**.  Test
 // hook: with a script, every draw replays the next scripted value instead
 // (unit and unit_incl: the value itself; below(n): the index; normal: the
@@ -640,7 +685,7 @@ struct Rng {
   double normal(double mean, double sd) {  // Box-Muller, one draw per call
     if (script) return mean + sd * scripted(3, 0.0);
     const double u1 = 1.0 - unit(), u2 = unit();
-    return mean + sd * (std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPI * u2));
+    return mean + sd * (std::sqrt(-2.0 * stream_ln(u1)) * stream_cos2pi(u2));
   }
 };
 
@@ -1232,6 +1277,14 @@ void or_hof_free(void* h) { delete static_cast<HallOfFame*>(h); }
 
 // Project random stream, for the product GA's stream test.
 void or_rng_stream(uint64_t seed, int n, uint64_t* out) { Rng r(seed); for (int i = 0; i < n; ++i) out[i] = r.next(); }
+void or_rng_normals(uint64_t seed, int n, double mean, double sd, double* out) {
+  Rng r(seed);
+  for (int i = 0; i < n; ++i) out[i] = r.normal(mean, sd);
+}
+// which: 0 the stream's ln, 1 its cos(2 pi u)
+void or_stream_math(const double* x, int n, int which, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = which == 0 ? stream_ln(x[i]) : stream_cos2pi(x[i]);
+}
 
 // One evolve step on given (genes, fitness), for GA operator tests.
 void or_evolve(const double* genes, const double* fitness, int n, uint64_t seed, double* out_genes, double* out_fitness) {

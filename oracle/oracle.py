@@ -85,6 +85,8 @@ def load(path) -> C.CDLL:
         "or_hof_angular_difference": (C.c_double, [C.c_double, C.c_double]),
         "or_hof_free": (None, [P]),
         "or_rng_stream": (None, [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
+        "or_rng_normals": (None, [C.c_uint64, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double)]),
+        "or_stream_math": (None, [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "or_evolve": (None, [D, D, C.c_int, C.c_uint64, D, D]),
         "or_evaluator_new": (P, [P, C.c_int, P, C.c_int]),
         "or_evaluator_free": (None, [P]),
@@ -284,6 +286,20 @@ class GaRun:
 def rng_stream(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.uint64)
     lib().or_rng_stream(seed, n, _p(out, C.c_uint64))
+    return out
+
+
+def rng_normals(seed: int, n: int, mean=0.0, sd=1.0) -> np.ndarray:
+    out = np.zeros(n)
+    lib().or_rng_normals(seed, n, mean, sd, _p(out))
+    return out
+
+
+def stream_math(x, which: int) -> np.ndarray:
+    """The project stream's ln (which 0) or cos(2 pi u) (which 1)."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().or_stream_math(_p(x), len(x), which, _p(out))
     return out
 
 
