@@ -321,14 +321,14 @@ void host_parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& f
 
 }  // namespace gdock
 
-// Device scratch of gdock_score_poses_device_noalloc: one exact-rescore flag
-// per pose (256 KiB on every device the library runs on, held until the
-// process ends).  Host code never uses the name `scratch` as an address: its
+// Device scratch of gdock_score_poses_device_noalloc: per pose an
+// exact-rescore flag, a place in the launch order and a cost bucket (576 KiB
+// on every device the library runs on, held until the process ends).  Host code never uses the name `scratch` as an address: its
 // device address is resolved once per device (scratch_resolve) and that
 // pointer is what kernels receive.
 namespace {
 constexpr int32_t kNoAllocMaxPoses = 1 << 16;
-static __device__ __align__(256) int32_t scratch[kNoAllocMaxPoses];
+static __device__ __align__(256) int32_t scratch[2 * kNoAllocMaxPoses + kNoAllocMaxPoses / 4];
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
@@ -558,6 +558,23 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   }
   const double com_off = std::max({std::fabs(lcom[0] - origin[0]), std::fabs(lcom[1] - origin[1]),
                                    std::fabs(lcom[2] - origin[2])});
+  // Bounding sphere of each ligand block (centre relative to the COM): the
+  // per-pose cost estimates of the launch order rotate these, not the atoms.
+  const int32_t n_blocks = (n_lig + kLigBlock - 1) / kLigBlock;
+  std::vector<float4> lsph(static_cast<size_t>(std::max(n_blocks, 1)), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  for (int32_t b = 0; b < n_blocks; ++b) {
+    const int32_t i0 = b * kLigBlock, i1 = std::min(n_lig, i0 + kLigBlock);
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = i0; i < i1; ++i) { c[0] += lx[i] - lcom[0]; c[1] += ly[i] - lcom[1]; c[2] += lz[i] - lcom[2]; }
+    for (double& v : c) v /= (i1 - i0);
+    double r = 0.0;
+    for (int32_t i = i0; i < i1; ++i) {
+      const double dx = lx[i] - lcom[0] - c[0], dy = ly[i] - lcom[1] - c[1], dz = lz[i] - lcom[2] - c[2];
+      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    lsph[b] = make_float4(static_cast<float>(c[0]), static_cast<float>(c[1]), static_cast<float>(c[2]),
+                          static_cast<float>(r + 1e-3));
+  }
 
   // restraints.rs:44-77: first "CA" with the residue number, receptor and ligand.
   std::vector<double> rr_xyz;
@@ -584,17 +601,17 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
                o_lq = L.add<double>(n_lig), o_lasp = L.add<double>(n_lig), o_lrad = L.add<double>(n_lig),
                o_lser = L.add<int32_t>(n_lig), o_lkq = L.add<double>(lkq.size()), o_rrx = L.add<double>(rr_xyz.size()),
                o_rli = L.add<int32_t>(rl_idx.size()), o_dev = L.add<DevComplex>(1),
-               o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig), o_sbox = L.add<float4>(sub_box.size());
+               o_lcls = L.add<float2>(static_cast<size_t>(nc) * n_lig), o_sbox = L.add<float4>(sub_box.size()),
+               o_lsph = L.add<float4>(lsph.size());
   // Lennard-Jones factors of every (receptor class, ligand atom) pair, with
   // the FP64 operations of fitness.rs:58-63 (R = rmin2_r + rmin2_l;
-  // sqrt(eps_r eps_l)), as q^2 / R^2 2^-16 in the kernel's squared units
-  // (the 2^-16 keeps the pair step's reciprocal at the scale of A), rounded
-  // once to FP32.  1 / R^2 is capped at 5e4 (R >= 4.5e-3 A): only
+  // sqrt(eps_r eps_l)), as q^2 / R^2 in the kernel's squared units (q =
+  // 2^-qexp A), rounded once to FP32.  1 / R^2 is capped at 5e4 (R >= 4.5e-3 A): only
   // unparameterized pairs (R = 0, eps = 0, a zero term) and nearly
   // coincident atoms notice the cap, and with it every product of the pair
   // step stays finite.
   std::vector<float2> lcls(static_cast<size_t>(nc) * n_lig);
-  const double q2 = std::ldexp(1.0, -2 * qexp - 16);
+  const double q2 = std::ldexp(1.0, -2 * qexp);
   for (int c = 0; c < nc; ++c)
     for (int32_t i = 0; i < n_lig; ++i) {
       const double R = class_rmin[c] + lrm[i];
@@ -622,6 +639,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   put(o_rrx, rr_xyz.data(), rr_xyz.size() * 8); put(o_rli, rl_idx.data(), rl_idx.size() * 4);
   put(o_lcls, lcls.data(), lcls.size() * sizeof(float2));
   put(o_sbox, sub_box.data(), sub_box.size() * sizeof(float4));
+  put(o_lsph, lsph.data(), lsph.size() * sizeof(float4));
   if (!host.empty()) std::memcpy(blob.data() + cx->contacts_layout_begin, host.data(), host.size());
 
   hipError_t e = hipSetDevice(ctx->device);
@@ -656,6 +674,7 @@ gdock_status gdock_complex_create(gdock_ctx* ctx, const gdock_atom* rec, int32_t
   d.restr_lig = reinterpret_cast<const int32_t*>(m + o_rli);
   d.lig_class = reinterpret_cast<const float2*>(m + o_lcls);
   d.rec_sub_box = reinterpret_cast<const float4*>(m + o_sbox);
+  d.lig_block_sphere = reinterpret_cast<const float4*>(m + o_lsph);
   d.n_rec = n_pad; d.n_rec_atoms = n_rec; d.n_lig = n_lig; d.n_class = nc; d.n_restr = cx->n_restr;
   d.n_coll = static_cast<int32_t>(coll.size());
   for (int i = 0; i < 3; ++i) { d.com[i] = lcom[i]; d.origin[i] = origin[i]; }
@@ -729,6 +748,11 @@ gdock_status gdock_score_poses_device_noalloc(gdock_ctx* ctx, gdock_complex* cx,
   if (plain_parts(cx->dev) > 1) {
     set_error(ctx, "score_poses_device_noalloc: a receptor of %d slots needs split-pose state; use "
               "gdock_score_poses_device", cx->dev.n_rec);
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  if (plain_scratch_bytes(cx->dev, n) > sizeof(scratch)) {  // the layout for n <= kNoAllocMaxPoses fits by design
+    set_error(ctx, "score_poses_device_noalloc: %d poses need more than the %zu bytes of device scratch", n,
+              sizeof(scratch));
     return GDOCK_ERR_INVALID_ARG;
   }
   hipError_t e = hipSuccess;

@@ -74,6 +74,7 @@ struct DevComplex {
   const double* class_rmin2;
   const double* class_eps;
   const float2* lig_class;     // n_class x n_lig: {q^2 / R^2 (q: the fixed-point unit), sqrt(eps_r eps_l)} per receptor class and ligand atom
+  const float4* lig_block_sphere;  // per kLigBlock ligand atoms (device order): {centre x, y, z relative to com, radius} in FP32 A
   const float4* rec_sub_box;   // per kRecSub receptor slots: {lo x, y, z, 0}, {hi x, y, z, 0} of the slots' FP32 A coordinates (padding excluded; empty: lo +inf, hi -inf)
   // ligand (positioned), in spatial block order (lig_orig: index in the input)
   const double* lig_x;
@@ -123,6 +124,11 @@ struct Weights { double vdw, elec, desolv, air; };
 // Cost buckets of the per-launch item order (half-octave steps of the
 // estimated pair work).
 constexpr int kCostBuckets = 64;
+// The plain path orders the poses of a launch by cost when it has at most
+// kOrderMaxPoses of them (one workgroup of kOrderThreads places them).
+constexpr int kOrderMaxPoses = 1 << 16;
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBuckets = 256;  // eighth-octave steps of the estimated cost
 // Estimated cost of one work item (receptor slots within reach of the
 // ligand, plus 1/16 of the slots walked for the per-tile tests, times the
 // ligand lane-slots): a pose estimated above it is split by receptor range

@@ -217,44 +217,40 @@ template <int K>
 struct LaneAtoms {
   // x, y, z: fixed point.  acv: sum of rho (rho - 2) over the current
   // receptor class within the tile (times eps when folded into FP64); ace:
-  // sum of q_r / d^2 over the current 16 receptor slots.
+  // sum of q_r / s over the current group of kRecGroup receptor slots.
   int32_t x[K], y[K], z[K];
   float eps[K], ir2[K], acv[K], ace[K];
   int32_t cnt[K];
 };
 
 // Soft-core LJ (fitness.rs:57-70) and the Coulomb factor from s, the squared
-// distance in units^2 (s = d^2 2^48 at the finest unit).  The pair's vdw term
-// is eps (rho^2 - 2 rho), rho = R^6 / (d^6 + R^6/2) = 1 / (t^3 + 1/2), t =
-// d^2/R^2.  rho and 1/s come from one reciprocal: with t' = t 2^-16 = s ir2
-// (ir2: the class factor q^2 / R^2 2^-16 of the lane's ligand atom) and den'
-// = t'^3 + 2^-49 = (t^3 + 1/2) 2^-48, rp = 1 / (den' s) = 1 / ((t^3 + 1/2) d^2)
-// stays a normal float for every pair within the cutoffs (a product at the
-// unit's scale, about 2^48 larger, would reach FP32's denormals for tiny R);
-// then rho' = rp s = rho 2^48 and y = rp den' = 1/s.  The vdw sums carry
-// rho' (u = rho' 2^-48 - 2, the term eps u rho' 2^-48 when folded into FP64).
-// 1/s is refined by one Newton step (e = 1 - s y, y + y e, two FMAs):
-// v_rcp_f32 is within 1 ulp but biased (-0.07 ulp on average), and over
-// ~10^6 Coulomb terms of one sign a bias adds up where rounding errors
-// cancel; after the step the reciprocal is correctly rounded but for rare
-// half-ulp cases.  s = 0 gives rho' = 0, a zero vdw term (the reference's
-// term is 0 there too) and a finite 1/s the elec mask never selects.
-constexpr float kRhoScale = 0x1p-48f;  // rho' to rho
+// distance in units^2.  The pair's vdw term is eps (rho^2 - 2 rho), rho = R^6
+// / (d^6 + R^6/2) = 1 / (t^3 + 1/2), t = d^2/R^2 = s ir2 (ir2: the class
+// factor q^2 / R^2 of the lane's ligand atom, q the fixed-point unit); u =
+// rho - 2.  rho and 1/s come from two reciprocals.  On gfx950 a v_rcp_f32
+// costs two to three FP32 VALU issue slots (measured), so this form times
+// the same as deriving both from one reciprocal of (t^3 + 1/2) d^2 (three
+// more products); it needs no rescaling to keep that product normal.  rho
+// lies in (0, 2] for every pair and t^3 stays finite even for padding lanes
+// (t < 1e10).  1/s is refined by one Newton step (e = 1 - s y,
+// y + y e, two FMAs): v_rcp_f32 is within 1 ulp but biased (-0.07 ulp on
+// average), and over ~10^6 Coulomb terms of one sign a bias adds up where
+// rounding errors cancel; after the step the reciprocal is correctly rounded
+// but for rare half-ulp cases.  s = 0 gives rho = 2 and u = 0, a zero vdw
+// term (the reference's term is 0 there too), and a non-finite 1/s that the
+// elec mask never selects (d <= 1 A).
 __device__ __forceinline__ void pair_terms(float s, float ir2, float& rho, float& u, float& is) {
   const float t = s * ir2;
-  const float den = __builtin_fmaf(t * t, t, 0x1p-49f);
-  const float rp = __builtin_amdgcn_rcpf(__builtin_fmaf(den, s, 1.0e-30f));
-  rho = rp * s;
-  const float y = rp * den;
+  rho = __builtin_amdgcn_rcpf(__builtin_fmaf(t * t, t, 0.5f));
+  u = rho - 2.0f;
+  const float y = __builtin_amdgcn_rcpf(s);
   is = __builtin_fmaf(y, __builtin_fmaf(-s, y, 1.0f), y);
-  u = __builtin_fmaf(rho, kRhoScale, -2.0f);
 }
 
-// The vdw term of a pair as the pair loop counts it: eps (u rho') 2^-48, in
-// FP64 from the FP32 factors.
+// The vdw term of a pair as the pair loop counts it: eps (u rho), in FP64
+// from the FP32 factors.
 __device__ __forceinline__ double vdw_term(float eps, float u, float rho) {
-  return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho)) *
-         static_cast<double>(kRhoScale);
+  return static_cast<double>(eps) * (static_cast<double>(u) * static_cast<double>(rho));
 }
 
 // s (units^2) inside one of the four guard bands (the pair loop's test).
@@ -295,11 +291,14 @@ struct SettleEvent { int32_t j, pad; uint64_t mask; };  // first receptor slot o
 constexpr int kEventsPerWave = 64;
 
 // Fast path of one receptor atom against the lane's ligand atom in slot k;
-// returns this wave's burial count of the receptor atom from that slot and
-// flags band pairs.
+// returns this wave's burial count of the receptor atom from that slot.
+// bmin keeps the least of the pair's four distances above the band floors
+// (bits(s) - c_T, unsigned) over the lane's pairs of the receptor group: the
+// group holds a band pair for the lane when it ends below the band width, one
+// compare per group instead of one per pair.
 template <int K>
 __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, LaneAtoms<K>& a, int k,
-                                             uint64_t& ambm) {
+                                             uint32_t& bmin) {
   const float s = pair_s(r, a.x[k], a.y[k], a.z[k]);
   // FP32 decisions below each band, as the borrows of bits(s) - c_T (pairs
   // of equal serial are included in the vdw sum and taken out afterwards, see
@@ -320,9 +319,8 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
       : "v"(bd), "s"(B.c64));
   const bool mv = __builtin_sub_overflow(bd, B.c144, &q144);
   const bool me0 = __builtin_sub_overflow(bd, B.c225, &q225);
-  // The band test as a lane mask straight from the compare (predicate 36:
-  // unsigned less-than); a ballot of a bool would first widen it to 0/1.
-  ambm |= __builtin_amdgcn_uicmp(min(min(q1, q64), min(q144, q225)), B.w, 36);
+  bmin = min(min(bmin, q1), q64);
+  bmin = min(min(bmin, q144), q225);
   const bool me = me0 && !le1;
   float rho, u, id2;
   pair_terms(s, a.ir2[k], rho, u, id2);
@@ -332,7 +330,7 @@ __device__ __forceinline__ int32_t pair_fast(const RecAtom r, const Bands& B, La
   // Pin this pair's results here (an empty asm, no instruction): otherwise
   // the compiler sinks the math below the group's rare band-pair branch and
   // keeps every pair's lane masks alive across it, spilling SGPRs.
-  asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+s"(ambm));
+  asm volatile("" : "+v"(a.acv[k]), "+v"(a.ace[k]), "+v"(a.cnt[k]), "+v"(bmin));
   return cj;
 }
 
@@ -634,7 +632,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           do { ++cls; cend = cstart[cls + 1]; } while (cend <= j);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]) * static_cast<double>(kRhoScale);
+            sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
             a.acv[k] = 0.0f;
             const int l = lig_index<K>(base, k);
             if (l < cx.n_lig) class_factors(cx, cls, l, a.ir2[k], a.eps[k]);
@@ -642,7 +640,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         }
         const RecAtom r0 = rec_atom(rf, j), r1 = rec_atom(rf, j + 1), r2 = rec_atom(rf, j + 2),
                       r3 = rec_atom(rf, j + 3);
-        uint64_t ambm = 0;  // lanes that saw a band pair in this group
+        uint32_t bmin = 0xffffffffu;  // see pair_fast
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {  // wave-uniform branches
@@ -655,15 +653,15 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
             for (int q = 0; q < 4; ++q)
               if ((mk >> q) & 1u) n_within += pair_s(rr[q], a.x[k], a.y[k], a.z[k]) <= B.lo225 ? 1u : 0u;
           }
-          if (mk & 1u) c0 += pair_fast<K>(r0, B, a, k, ambm);
-          if (mk & 2u) c1 += pair_fast<K>(r1, B, a, k, ambm);
-          if (mk & 4u) c2 += pair_fast<K>(r2, B, a, k, ambm);
-          if (mk & 8u) c3 += pair_fast<K>(r3, B, a, k, ambm);
+          if (mk & 1u) c0 += pair_fast<K>(r0, B, a, k, bmin);
+          if (mk & 2u) c1 += pair_fast<K>(r1, B, a, k, bmin);
+          if (mk & 4u) c2 += pair_fast<K>(r2, B, a, k, bmin);
+          if (mk & 8u) c3 += pair_fast<K>(r3, B, a, k, bmin);
           // Coulomb: the group's FP32 partial (at most 4 terms) goes to FP64
           // at once; a longer FP32 partial carries a close pair's large term
           // and rounds every later term at its scale.
           const int l = lig_index<K>(base, k);
-          sum_e += static_cast<double>(a.ace[k]) * cx.lig_kq[l];  // zero-padded past n_lig
+          sum_e = __builtin_fma(static_cast<double>(a.ace[k]), cx.lig_kq[l], sum_e);  // zero-padded past n_lig
           a.ace[k] = 0.0f;
         }
         const int t = lane - sh;
@@ -671,7 +669,10 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         rc = t == 1 ? c1 : rc;
         rc = t == 2 ? c2 : rc;
         rc = t == 3 ? c3 : rc;
-        const uint64_t amask = ambm;
+        // lanes that saw a band pair in this group, as a lane mask straight
+        // from the compare (predicate 36: unsigned less-than); a ballot of a
+        // bool would first widen it to 0/1
+        const uint64_t amask = __builtin_amdgcn_uicmp(bmin, B.w, 36);
         if (amask != 0ull) {  // rare: this group holds band pairs
           if (EXACT) {
             band_pairs<K, EXACT>(cx, pc, coords, base, j, r0, B, a, sum_e, my_ev, nev, overflow, s_rcnt, pp.r0);
@@ -689,7 +690,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       if (g0 + lane < g1 && rc) atomicAdd(&s_rcnt[g0 + lane - pp.r0], rc);
 #pragma unroll
       for (int k = 0; k < K; ++k) {  // fold the tile's FP32 LJ sums into FP64
-        sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]) * static_cast<double>(kRhoScale);
+        sum_v += static_cast<double>(a.eps[k]) * static_cast<double>(a.acv[k]);
         a.acv[k] = 0.0f;
       }
       }
@@ -832,17 +833,18 @@ __device__ __forceinline__ int2 part_range(int n_rec, int part, int parts) {
   return make_int2(64 * t0, min(64 * t1, n_rec));
 }
 
-// Plain path: pose blockIdx.x / parts, receptor part blockIdx.x % parts (one
-// part unless the receptor exceeds one workgroup's LDS table).  A split pose
-// keeps its shared state at split_* (cleared before the launch; the last part
-// clears it again).
+// Plain path: pose order[blockIdx.x / parts] (blockIdx.x / parts without an
+// order), receptor part blockIdx.x % parts (one part unless the receptor
+// exceeds one workgroup's LDS table).  A split pose keeps its shared state at
+// split_* (cleared before the launch; the last part clears it again).
 template <int K, bool EXACT, bool COUNT = false>
 __global__ __launch_bounds__(kScoreThreads) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 4, EXACT ? 8 : 4))) void score_poses_kernel(
     const DevComplex* __restrict__ d_cx, const double* __restrict__ poses, const double* __restrict__ coords,
     const Weights w, double* __restrict__ out, int32_t* __restrict__ nsat_out, int32_t* __restrict__ redo,
     int parts, int32_t* __restrict__ split_lig, double* __restrict__ split_sums, int32_t* __restrict__ split_sync,
-    unsigned long long* __restrict__ counts) {
-  const int pose = blockIdx.x / parts, part = blockIdx.x % parts;
+    const int32_t* __restrict__ order, unsigned long long* __restrict__ counts) {
+  const int slot = blockIdx.x / parts, part = blockIdx.x % parts;
+  const int pose = order ? order[slot] : slot;
   if (EXACT && redo[pose] == 0) return;  // only poses whose settle buffer overflowed
   extern __shared__ int32_t s_lds[];
   __shared__ int32_t s_flags[2];
@@ -980,26 +982,158 @@ int plain_parts(const DevComplex& cx) {
   return std::max(1, (tiles + per - 1) / per);
 }
 
-size_t plain_scratch_bytes(const DevComplex& cx, int n_poses) {
-  const size_t flags = (static_cast<size_t>(std::max(n_poses, 1)) * sizeof(int32_t) + 255) / 256 * 256;
-  return flags + split_state_bytes(cx, plain_parts(cx), n_poses);
+// Plain-path scratch: the per-pose rescore flags, the launch order of the
+// poses and their cost buckets, then the split state (each piece 256-byte
+// aligned).
+struct PlainScratch { size_t order, keys, split, total; };
+static PlainScratch plain_scratch_layout(const DevComplex& cx, int n_poses) {
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n = static_cast<size_t>(std::max(n_poses, 1));
+  PlainScratch L;
+  L.order = al(n * sizeof(int32_t));
+  L.keys = L.order + al(n * sizeof(int32_t));
+  L.split = L.keys + al(n);
+  L.total = L.split + split_state_bytes(cx, plain_parts(cx), n_poses);
+  return L;
+}
+
+size_t plain_scratch_bytes(const DevComplex& cx, int n_poses) { return plain_scratch_layout(cx, n_poses).total; }
+
+// Estimated work of scoring a pose (pose record p) with k ligand atoms per
+// lane, by one whole wave: the receptor slots within the ligand's radius +
+// kCullReach of the pose's ligand centre (the rotation keeps the centre; the
+// genes move it; every fourth slot is tested), plus 1/16 of the slots for the
+// per-tile tests, times the ligand lane-slots walked.
+__device__ double pose_cost(CxRef cx, const double* __restrict__ p, int k) {
+  const int lane = threadIdx.x & 63;
+  const double ox = cx.com[0] - cx.origin[0], oy = cx.com[1] - cx.origin[1], oz = cx.com[2] - cx.origin[2];
+  const double lig_r = cx.lig_extent - fmax(fabs(ox), fmax(fabs(oy), fabs(oz)));
+  const float px = static_cast<float>(ox + p[3]), py = static_cast<float>(oy + p[4]),
+              pz = static_cast<float>(oz + p[5]);
+  const float reach = static_cast<float>(lig_r) + kCullReach;
+  const float unit = ldexpf(1.0f, -cx.qexp);
+  int32_t near = 0;
+#pragma unroll 4
+  for (int j = lane; j < cx.n_rec; j += 256) {  // every fourth slot: an estimate
+    const int4 r = cx.rec_fixed[j];
+    const float dx = static_cast<float>(r.x) * unit - px, dy = static_cast<float>(r.y) * unit - py,
+                dz = static_cast<float>(r.z) * unit - pz;
+    near += dx * dx + dy * dy + dz * dz <= reach * reach ? 4 : 0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) near += __shfl_xor(near, off);
+  const int per_chunk = kScoreThreads * k;
+  const double slots = static_cast<double>((cx.n_lig + per_chunk - 1) / per_chunk) * per_chunk;
+  return (near + 0.0625 * cx.n_rec) * slots;
+}
+
+// Estimated work of scoring a pose (pose record p) with k ligand atoms per
+// lane, by one whole wave, for the plain path's launch order.  The pose moves
+// each ligand block's bounding sphere, and every receptor sub-tile box within
+// the sphere's radius + kCullReach counts its kRecSub slots (the pairs the
+// kernel's culling keeps, nearly).  A workgroup ends with its slowest wave,
+// and wave w of a chunk holds the chunk's blocks w k .. w k + k - 1, so the
+// estimate sums, over chunks, the largest of the four waves' block counts;
+// plus 1/16 of the slots per block for the per-tile tests.
+__device__ double pose_cost_blocks(CxRef cx, const double* __restrict__ p, int k) {
+  const int lane = threadIdx.x & 63;
+  const float ca = p[6], sa = p[7], cb = p[8], sb = p[9], cg = p[10], sg = p[11];
+  const float m01 = p[12], m02 = p[13], m11 = p[14], m12 = p[15];
+  const float ox = static_cast<float>(cx.com[0] - cx.origin[0] + p[3]),
+              oy = static_cast<float>(cx.com[1] - cx.origin[1] + p[4]),
+              oz = static_cast<float>(cx.com[2] - cx.origin[2] + p[5]);
+  const int n_blocks = (cx.n_lig + kLigBlock - 1) / kLigBlock, n_sub = (cx.n_rec + kRecSub - 1) / kRecSub;
+  const int waves = kScoreThreads / 64;
+  int32_t total = 0, wave_sum = 0, wave_max = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    const float4 S = cx.lig_block_sphere[b];
+    const float x = ((S.x * ca) * cb + S.y * m01) + S.z * m02 + ox;
+    const float y = ((S.x * sa) * cb + S.y * m11) + S.z * m12 + oy;
+    const float z = ((-S.x) * sb + (S.y * cb) * sg) + (S.z * cb) * cg + oz;
+    const float reach = S.w + kCullReach;
+    int32_t near = 0;
+    for (int t = lane; t < n_sub; t += 64) {
+      const float4 lo = cx.rec_sub_box[2 * t], hi = cx.rec_sub_box[2 * t + 1];
+      const float dx = fmaxf(fmaxf(lo.x - x, x - hi.x), 0.0f), dy = fmaxf(fmaxf(lo.y - y, y - hi.y), 0.0f),
+                  dz = fmaxf(fmaxf(lo.z - z, z - hi.z), 0.0f);
+      near += dx * dx + dy * dy + dz * dz <= reach * reach ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) near += __shfl_xor(near, off);
+    wave_sum += near;
+    const int in_chunk = b % (waves * k);
+    if (in_chunk % k == k - 1 || b == n_blocks - 1) {  // a wave's last block
+      wave_max = max(wave_max, wave_sum);
+      wave_sum = 0;
+    }
+    if (in_chunk == waves * k - 1 || b == n_blocks - 1) {  // a chunk's last block
+      total += wave_max;
+      wave_max = 0;
+    }
+  }
+  return static_cast<double>(total) * (kRecSub * kLigBlock) + 0.0625 * cx.n_rec * n_blocks * kLigBlock / waves;
+}
+
+// Half-octave bucket of an estimated cost.
+__device__ __forceinline__ int cost_bucket(double cost) {
+  return cost < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(cost)));
+}
+
+// Plain path, before the scoring launch: each pose's cost bucket, in eighth
+// octaves (one wave per pose).
+__global__ __launch_bounds__(256) void plain_cost_kernel(const DevComplex* __restrict__ d_cx,
+                                                         const double* __restrict__ poses, int n, int k,
+                                                         uint8_t* __restrict__ keys) {
+  CxRef cx = *(const DevComplex GDOCK_AS4*)d_cx;
+  const int i = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (i >= n) return;
+  const double cost = pose_cost_blocks(cx, poses + static_cast<size_t>(kPoseDoubles) * i, k);
+  if ((threadIdx.x & 63) == 0)
+    keys[i] = static_cast<uint8_t>(cost < 16.0 ? 0 : min(kOrderBuckets - 1, static_cast<int>(8.0 * log2(cost)) - 32));
+}
+
+// Plain path: the launch order of the poses, most expensive bucket first, so
+// that a launch of several rounds of workgroups does not end on expensive
+// poses dispatched last (a pose costs up to twentyfold another of the same
+// complex).  One workgroup: counts per bucket, their descending prefix sums,
+// then every pose placed at its bucket's cursor.  Each pose is scored by one
+// workgroup whatever its place, so the order changes no result.
+__global__ __launch_bounds__(kOrderThreads) void plain_order_kernel(const uint8_t* __restrict__ keys, int n,
+                                                                    int32_t* __restrict__ order) {
+  __shared__ int32_t s_count[kOrderBuckets], s_cursor[kOrderBuckets];
+  const int tid = threadIdx.x;
+  if (tid < kOrderBuckets) s_count[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) atomicAdd(&s_count[keys[i]], 1);
+  __syncthreads();
+  if (tid < kOrderBuckets) {
+    int32_t start = 0;
+    for (int c = kOrderBuckets - 1; c > tid; --c) start += s_count[c];
+    s_cursor[tid] = start;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) order[atomicAdd(&s_cursor[keys[i]], 1)] = i;
 }
 
 template <int K>
 static void launch_pair(const DevComplex* d_cx, const double* d_poses, const double* d_coords, int n_poses,
                         const Weights& w, double* d_out, int32_t* d_nsat, int32_t* d_redo, int parts,
-                        const DevComplex& split, size_t lds, hipStream_t stream, unsigned long long* counts) {
+                        const DevComplex& split, const int32_t* d_order, size_t lds, hipStream_t stream,
+                        unsigned long long* counts) {
   const dim3 grid(static_cast<unsigned>(n_poses) * parts);
   if (counts) {  // the counting instantiation of the fast kernel alone
     hipLaunchKernelGGL((score_poses_kernel<K, false, true>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses,
                        d_coords, w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync,
-                       counts);
+                       nullptr, counts);
     return;
   }
   hipLaunchKernelGGL((score_poses_kernel<K, false>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
-                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, nullptr);
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, d_order,
+                     nullptr);
+  // the exact variant returns at once for every pose not flagged: no order
   hipLaunchKernelGGL((score_poses_kernel<K, true>), grid, dim3(kScoreThreads), lds, stream, d_cx, d_poses, d_coords,
-                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, nullptr);
+                     w, d_out, d_nsat, d_redo, parts, split.split_lig, split.split_sums, split.split_sync, nullptr,
+                     nullptr);
 }
 
 // Fast kernel over all poses, then the exact kernel (a no-op for every pose
@@ -1012,20 +1146,32 @@ hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const doub
   if (n_poses <= 0) return hipSuccess;
   const int parts = plain_parts(cx);
   const size_t lds = score_lds_bytes(part_slots(cx.n_rec, parts));
+  const PlainScratch SL = plain_scratch_layout(cx, n_poses);
   char* m = static_cast<char*>(scratch);
   int32_t* d_redo = reinterpret_cast<int32_t*>(m);
   DevComplex split{};
   split.n_lig = cx.n_lig;
   if (parts > 1) {
-    const size_t flags = (static_cast<size_t>(n_poses) * sizeof(int32_t) + 255) / 256 * 256;
-    bind_split_state(&split, parts, n_poses, 0, m + flags);
-    const hipError_t e = hipMemsetAsync(m + flags, 0, split_state_bytes(cx, parts, n_poses), stream);
+    bind_split_state(&split, parts, n_poses, 0, m + SL.split);
+    const hipError_t e = hipMemsetAsync(m + SL.split, 0, split_state_bytes(cx, parts, n_poses), stream);
     if (e != hipSuccess) return e;
   }
-  switch (lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig)) {
+  const int k = lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig);
+  // Poses in order of estimated cost, most expensive first (pose records
+  // only: coordinates as given carry no pose centre to estimate from).
+  int32_t* d_order = nullptr;
+  if (d_poses && !d_coords && !counts && n_poses > 1 && n_poses <= kOrderMaxPoses) {
+    d_order = reinterpret_cast<int32_t*>(m + SL.order);
+    uint8_t* d_keys = reinterpret_cast<uint8_t*>(m + SL.keys);
+    hipLaunchKernelGGL(plain_cost_kernel, dim3((n_poses + 3) / 4), dim3(256), 0, stream, d_cx, d_poses, n_poses, k,
+                       d_keys);
+    hipLaunchKernelGGL(plain_order_kernel, dim3(1), dim3(kOrderThreads), 0, stream, d_keys, n_poses, d_order);
+  }
+  switch (k) {
 #define GDOCK_CASE(KK)                                                                                  \
   case KK:                                                                                              \
-    launch_pair<KK>(d_cx, d_poses, d_coords, n_poses, w, d_out, d_nsat, d_redo, parts, split, lds, stream, counts); \
+    launch_pair<KK>(d_cx, d_poses, d_coords, n_poses, w, d_out, d_nsat, d_redo, parts, split, d_order, lds, stream, \
+                    counts);                                                                            \
     break;
     GDOCK_CASE(1) GDOCK_CASE(2) GDOCK_CASE(3) GDOCK_CASE(4)
     GDOCK_CASE(5) GDOCK_CASE(6) GDOCK_CASE(7) GDOCK_CASE(8)
@@ -1075,29 +1221,10 @@ __global__ __launch_bounds__(256) void pose_plan_kernel(const DevComplex* __rest
     const int2 P = pose_list[i];
     CxRef cx = ((const DevComplex GDOCK_AS4*)table)[P.x];
     const double* p = poses + static_cast<size_t>(kPoseDoubles) * P.y;
-    const double ox = cx.com[0] - cx.origin[0], oy = cx.com[1] - cx.origin[1], oz = cx.com[2] - cx.origin[2];
-    const double lig_r = cx.lig_extent - fmax(fabs(ox), fmax(fabs(oy), fabs(oz)));
-    const float px = static_cast<float>(ox + p[3]), py = static_cast<float>(oy + p[4]),
-                pz = static_cast<float>(oz + p[5]);
-    const float reach = static_cast<float>(lig_r) + kCullReach;
-    const float unit = ldexpf(1.0f, -cx.qexp);
-    int32_t near = 0;
-#pragma unroll 4
-    for (int j = lane; j < cx.n_rec; j += 256) {  // every fourth slot: an estimate
-      const int4 r = cx.rec_fixed[j];
-      const float dx = static_cast<float>(r.x) * unit - px, dy = static_cast<float>(r.y) * unit - py,
-                  dz = static_cast<float>(r.z) * unit - pz;
-      near += dx * dx + dy * dy + dz * dz <= reach * reach ? 4 : 0;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) near += __shfl_xor(near, off);
-    const int k = cx.lanes_k > 0 ? cx.lanes_k : 1;
-    const int per_chunk = kScoreThreads * k;
-    const double slots = static_cast<double>((cx.n_lig + per_chunk - 1) / per_chunk) * per_chunk;
-    const double cost = (near + 0.0625 * cx.n_rec) * slots;
+    const double cost = pose_cost(cx, p, cx.lanes_k > 0 ? cx.lanes_k : 1);
     const int parts = min(max(cx.parts, 1), max(cx.min_parts, static_cast<int>(ceil(cost / kSplitCost))));
     const double share = cost / parts;
-    const int b = share < 1.0 ? 0 : min(kCostBuckets - 1, static_cast<int>(2.0 * log2(share)));
+    const int b = cost_bucket(share);
     if (lane == 0) {
       plan[i] = static_cast<uint16_t>(b | (parts << 8));
       atomicAdd(&s_hist[b], parts);
