@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--workload", default="batched_scoring", choices=["batched_scoring", "uneven_batch", "serve", "fcc", "ga"])
     ap.add_argument("--jobs", type=int, default=160, help="serve: number of docking jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: Poisson arrival rate, jobs/s (0: all at once)")
+    ap.add_argument("--ga-on-device", action="store_true", help="serve: advance the GAs with the device GA")
     ap.add_argument("--poses", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=5.0,
                     help="target wall time of each CPU baseline sample (two thread counts)")
@@ -221,8 +222,12 @@ def main():
                                                    "ms_per_step", "plain_ms_per_step")}
         v = measure_serve(args.extra_jobs, 20.0)
         result["served"] = {k: v[k] for k in ("workload", "jobs_per_min", "latency_p50_s", "latency_p99_s",
-                                              "generations", "peak_active_jobs", "span_s", "plain",
-                                              "speedup_jobs_per_min")}
+                                              "generations", "peak_active_jobs", "span_s", "mean_step_ms",
+                                              "plain", "speedup_jobs_per_min")}
+        d = measure_serve(args.extra_jobs, 20.0, ga_on_device=True, plain=False)
+        result["served"]["ga_on_device"] = {k: d[k] for k in ("jobs_per_min", "latency_p50_s", "latency_p99_s",
+                                                               "mean_step_ms", "mean_step_kernel_ms",
+                                                               "mean_step_ga_ms")}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(rec, lig, c.restraints, genes, args.cpu_seconds)
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
@@ -325,7 +330,7 @@ def measure_uneven(steps, warmup):
 
 
 def bench_serve(args):
-    v = measure_serve(args.jobs, args.rate)
+    v = measure_serve(args.jobs, args.rate, ga_on_device=args.ga_on_device)
     result = {
         "metric": METRIC,
         "value": v["jobs_per_min"],
@@ -340,12 +345,13 @@ def bench_serve(args):
     print(json.dumps(result), flush=True)
 
 
-def measure_serve(n_jobs, rate):
+def measure_serve(n_jobs, rate, ga_on_device=False, plain=True):
     """Served docking stream: jobs (complexes drawn as in the uneven batch,
     per-job seeds 1000+i, 4-12 restraints) arrive as a Poisson process and run
     the full GA (population 150, early stop after 10 flat generations) and the
     FCC clustering of their Hall of Fame.  Server: continuous batching with
-    memory admission.  Plain: the same jobs one at a time through gdock_ga_run
+    memory admission, the GAs advanced on host threads or (ga_on_device) by
+    the device GA.  Plain: the same jobs one at a time through gdock_ga_run
     (one scoring launch per generation); its latencies apply the same arrival
     times to the measured service times (first come, first served)."""
     import torch  # noqa: F401  (one HIP runtime, see gdock_amd._lib)
@@ -367,7 +373,7 @@ def measure_serve(n_jobs, rate):
         arrivals = np.zeros(len(jobs))
     ctx = default_context(0)
 
-    srv = Server(max_jobs=64, max_receptor_atoms=8192)
+    srv = Server(max_jobs=64, max_receptor_atoms=8192, ga_on_device=ga_on_device)
     warm = srv.submit(*jobs[0], seed=1, max_generations=2, cluster=0)
     srv.wait(warm, 300.0)
     t0 = time.perf_counter()
@@ -387,7 +393,7 @@ def measure_serve(n_jobs, rate):
     gens = sum(r.generations for r, _ in res)
 
     service, pgens = [], 0
-    for rec, lig, pairs in jobs:
+    for rec, lig, pairs in (jobs if plain else []):
         t = time.perf_counter()
         cx = Complex(rec, structure.position_ligand(rec, lig), pairs, ctx)
         r = ga.run(cx, seed=42)
@@ -399,8 +405,8 @@ def measure_serve(n_jobs, rate):
         start = max(a, free_at)
         free_at = start + sv
         plat.append(free_at - a)
-    plat = np.array(plat)
-    pspan = free_at - arrivals[0]
+    plat = np.array(plat) if plain else np.zeros(1)
+    pspan = free_at - arrivals[0] if plain else float("nan")
     return {
         "workload": f"serve: {len(jobs)} GA docking jobs (receptor 500-8000, ligand 300-3000 atoms, "
                     f"population 150, early stop 10, FCC of the Hall of Fame), "
@@ -411,11 +417,14 @@ def measure_serve(n_jobs, rate):
         "generations": gens, "peak_active_jobs": st["peak_active"], "steps_run": st["steps"],
         "server_busy_s": st["busy_s"], "mean_step_ms": st["busy_s"] / max(st["steps"], 1) * 1e3,
         "mean_step_kernel_ms": st["kernel_s"] / max(st["steps"], 1) * 1e3,
+        "mean_step_ga_ms": st["ga_s"] / max(st["steps"], 1) * 1e3,
+        "ga_on_device": bool(ga_on_device),
         "server_admit_s": st["admit_s"], "server_ga_s": st["ga_s"], "server_finish_s": st["finish_s"],
         "span_s": span,
-        "plain": {"jobs_per_min": len(jobs) / pspan * 60.0, "latency_p50_s": float(np.percentile(plat, 50)),
-                  "latency_p99_s": float(np.percentile(plat, 99)), "mean_service_s": float(np.mean(service)),
-                  "generations": pgens, "pose_evals_per_s": pgens * 150 / float(np.sum(service))},
+        "plain": ({"jobs_per_min": len(jobs) / pspan * 60.0, "latency_p50_s": float(np.percentile(plat, 50)),
+                   "latency_p99_s": float(np.percentile(plat, 99)), "mean_service_s": float(np.mean(service)),
+                   "generations": pgens, "pose_evals_per_s": pgens * 150 / float(np.sum(service))}
+                  if plain else None),
         "speedup_jobs_per_min": pspan / span,
     }
 

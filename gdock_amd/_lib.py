@@ -58,7 +58,7 @@ class GaParams(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_jobs", C.c_int32), ("max_population", C.c_int32), ("max_receptor_atoms", C.c_int32),
-                ("reserved", C.c_int32), ("memory_budget", C.c_int64), ("weights", Weights)]
+                ("ga_on_device", C.c_int32), ("memory_budget", C.c_int64), ("weights", Weights)]
 
 
 class JobStats(C.Structure):

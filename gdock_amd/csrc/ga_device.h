@@ -42,6 +42,7 @@ struct DevGaJob {
   DevGaState* st;
   double* genes[2];     // population genes, double-buffered (max_pop x 6)
   double* child;        // children before the elite replacement (max_pop x 6)
+  double* mut_u;        // per child gene: the two uniforms of its mutation draw (2 x max_pop x 6)
   double* hof_genes;    // kGaHofCapacity + 1 entries
   double* hof_fit;
   int32_t* hof_gen;
@@ -53,11 +54,13 @@ struct DevGaJob {
   int32_t* trace_best;
   double* poses;        // the generation's pose records (scored), then the next generation's
   const double* out;    // the generation's scores (kOutDoubles per pose)
+  int32_t* done_flag;   // optional: set to the state's done flag after every generation
+  int32_t pad;
   DevGaParams p;
 };
 
 struct GaJobOffsets {
-  size_t state, genes0, genes1, child, hof_genes, hof_fit, hof_gen, tmp_genes, tmp_fit, tmp_gen;
+  size_t state, genes0, genes1, child, mut_u, hof_genes, hof_fit, hof_gen, tmp_genes, tmp_fit, tmp_gen;
   size_t trace_genes, trace_fit, trace_best;
 };
 

@@ -240,12 +240,21 @@ __global__ __launch_bounds__(kGaThreads) void ga_generation_kernel(const DevGaJo
     st->converged_at = converged_at;
     st->trace_at = trace_at + n;
     st->done = done;
+    if (J.done_flag) *J.done_flag = done;
   }
   if (done && !final_breed) return;
 
-  // population.rs:74-158: breed the next population
+  // population.rs:74-158: breed the next population.  One lane takes every
+  // draw of the stream in the host driver's order and keeps only decisions:
+  // tournament winners, the shuffle, a swap bit per crossed gene, a mutation
+  // bit per child gene with the two uniforms of its normal.  The workgroup
+  // then builds the children from those (gene copies from global memory and
+  // the normals' arithmetic in parallel): the same operations on the same
+  // values as the host driver, so the same bits.
   const int n_child = n & ~1;
   const int elite = min(P.elitism, n_child);
+  __shared__ uint8_t s_xbits[kGaMaxPopulation / 2];  // per pair: genes swapped by the crossover
+  __shared__ uint8_t s_mbits[kGaMaxPopulation];      // per child: genes mutated
   if (tid == 0) {
     stream::Xoshiro256 rng;
     for (int k = 0; k < 4; ++k) rng.s[k] = st->rng[k];
@@ -264,32 +273,39 @@ __global__ __launch_bounds__(kGaThreads) void ga_generation_kernel(const DevGaJo
       s_chosen[j] = t;
     }
     for (int m = 0; 2 * m + 1 < n; ++m) {  // uniform crossover of neighbours
-      const int a = s_chosen[2 * m], b = s_chosen[2 * m + 1];
-      double* ca = J.child + 12 * m;
-      double* cb = ca + 6;
-      for (int g = 0; g < 6; ++g) {
-        double x = genes[6 * a + g], y = genes[6 * b + g];
-        if (rng.unit() <= P.cross_rate) {
-          const double t = x;
-          x = y;
-          y = t;
-        }
-        ca[g] = x;
-        cb[g] = y;
-      }
-      s_cfit[2 * m] = s_fit[a];
-      s_cfit[2 * m + 1] = s_fit[b];
+      uint32_t bits = 0;
+      for (int g = 0; g < 6; ++g) bits |= (rng.unit() <= P.cross_rate ? 1u : 0u) << g;
+      s_xbits[m] = static_cast<uint8_t>(bits);
+      s_cfit[2 * m] = s_fit[s_chosen[2 * m]];
+      s_cfit[2 * m + 1] = s_fit[s_chosen[2 * m + 1]];
     }
-    for (int c = 0; c < n_child; ++c)  // mutation
+    for (int c = 0; c < n_child; ++c) {  // mutation: Box-Muller takes 1 - u1 and u2
+      uint32_t bits = 0;
       for (int g = 0; g < 6; ++g) {
         if (!(rng.unit() <= P.mut_rate)) continue;
-        double& v = J.child[6 * c + g];
-        if (g < 3)
-          v = rem_euclid_2pi(v + rng.normal(0.0, 0.174));
-        else
-          v = fmin(fmax(v + rng.normal(0.0, 1.0), -P.max_disp), P.max_disp);
+        bits |= 1u << g;
+        const double u1 = 1.0 - rng.unit(), u2 = rng.unit();
+        J.mut_u[2 * (6 * c + g)] = u1;
+        J.mut_u[2 * (6 * c + g) + 1] = u2;
       }
+      s_mbits[c] = static_cast<uint8_t>(bits);
+    }
     for (int k = 0; k < 4; ++k) st->rng[k] = rng.s[k];
+  }
+  __syncthreads();
+  for (int i = tid; i < 6 * n_child; i += kGaThreads) {
+    const int c = i / 6, g = i - 6 * c, m = c >> 1;
+    const int a = s_chosen[2 * m], b = s_chosen[2 * m + 1];
+    const bool swap = (s_xbits[m] >> g) & 1u;
+    const double x = genes[6 * a + g], y = genes[6 * b + g];
+    double v = (c & 1) == 0 ? (swap ? y : x) : (swap ? x : y);
+    if ((s_mbits[c] >> g) & 1u) {
+      const double u1 = J.mut_u[2 * i], u2 = J.mut_u[2 * i + 1];
+      const double sd = g < 3 ? 0.174 : 1.0;
+      const double z = 0.0 + sd * (stream::sqrt_cr(-2.0 * stream::ln(u1)) * stream::cos2pi(u2));
+      v = g < 3 ? rem_euclid_2pi(v + z) : fmin(fmax(v + z, -P.max_disp), P.max_disp);
+    }
+    J.child[i] = v;
   }
   __syncthreads();
   // the elite replace the children that inherited the worst fitness: stable
@@ -341,6 +357,7 @@ void ga_job_layout(Layout* L, int max_pop, int max_gen, bool trace, GaJobOffsets
   t.genes0 = L->add<double>(static_cast<size_t>(max_pop) * 6);
   t.genes1 = L->add<double>(static_cast<size_t>(max_pop) * 6);
   t.child = L->add<double>(static_cast<size_t>(max_pop) * 6);
+  t.mut_u = L->add<double>(static_cast<size_t>(max_pop) * 12);
   t.hof_genes = L->add<double>((kGaHofCapacity + 1) * 6);
   t.hof_fit = L->add<double>(kGaHofCapacity + 1);
   t.hof_gen = L->add<int32_t>(kGaHofCapacity + 1);
@@ -362,6 +379,7 @@ DevGaJob ga_job_bind(char* base, const GaJobOffsets& o, const gdock_ga_params& p
   J.genes[0] = reinterpret_cast<double*>(base + o.genes0);
   J.genes[1] = reinterpret_cast<double*>(base + o.genes1);
   J.child = reinterpret_cast<double*>(base + o.child);
+  J.mut_u = reinterpret_cast<double*>(base + o.mut_u);
   J.hof_genes = reinterpret_cast<double*>(base + o.hof_genes);
   J.hof_fit = reinterpret_cast<double*>(base + o.hof_fit);
   J.hof_gen = reinterpret_cast<int32_t*>(base + o.hof_gen);

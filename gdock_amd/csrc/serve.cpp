@@ -2,13 +2,20 @@
 //
 // Jobs are queued by gdock_server_submit and admitted by a worker thread as
 // slots and device memory free up.  Each admitted job keeps its complex
-// (receptor, positioned ligand, restraints, contact tables) resident in HBM
-// until it finishes.  Every scheduling step scores the current generation of
-// every active job in one persistent launch (replayed from a graph captured
-// once at fixed capacity), advances each job's GA on the host, and retires
-// finished jobs after clustering their Hall of Fame.  A job's results do not
-// depend on which other jobs share its launches: every pose is scored by the
-// same kernel code with the same per-shape choice of lanes.
+// (receptor, positioned ligand, restraints, contact tables) and its GA state
+// (population, Hall of Fame, random stream) resident in HBM until it
+// finishes.  Every scheduling step scores the current generation of every
+// active job in one persistent launch (replayed from a graph captured once at
+// fixed capacity), then advances every job's GA: on the host threads (the
+// default: a generation's draws are a sequential chain per job, which a host
+// core runs faster than one GPU lane), or, with ga_on_device, in one launch of
+// the device GA (one workgroup per job), which keeps the population resident
+// and writes the next generation's pose records in place, so the host only
+// reads one done flag per slot between steps.  Finished jobs are retired
+// after clustering their Hall of Fame.  A job's results do
+// not depend on which other jobs share its launches: every pose is scored by
+// the same kernel code with the same per-shape choice of lanes, and every
+// job's GA has its own stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +33,7 @@
 #include "../../include/gdock_hip.h"
 #include "common.h"
 #include "ga.h"
+#include "ga_device.h"
 #include "internal.h"
 
 using namespace gdock;
@@ -52,6 +60,9 @@ struct Job {
   void* split_mem = nullptr;  // split-pose state (parts > 1)
   std::unique_ptr<gdock_ga_result> result;
   DevComplex dev{};      // the job's launch-table entry, once prepared
+  void* ga_mem = nullptr;  // the job's device GA state
+  DevGaJob ga{};           // its kernel view (pose records and scores at the slot's rows)
+  int n_cur = 0;           // poses of the generation to score next
   bool ready = false;    // prepared: the worker scores it from its next step
   bool in_table = false; // the worker has uploaded its table entry
 };
@@ -69,10 +80,15 @@ struct gdock_server {
   // device state of the launches
   BatchLaunch L{};
   void* mem = nullptr;
-  double* h_poses = nullptr;  // pinned
-  double* h_out = nullptr;
-  int2* h_list = nullptr;  // {slot, pose row} per pose of a step
+  int2* h_list = nullptr;  // {slot, pose row} per pose of a step (pinned)
   DevComplex* h_table = nullptr;
+  double* h_poses = nullptr;  // host GA: the step's pose records (pinned)
+  double* h_out = nullptr;    // host GA: the step's scores (pinned)
+  DevGaJob* d_ga = nullptr;   // the active jobs' GA views, in step order
+  DevGaJob* h_ga = nullptr;   // (pinned staging)
+  int32_t* d_done = nullptr;  // per slot: the GA's done flag after the last step
+  int32_t* h_done = nullptr;  // (pinned)
+  std::vector<int64_t> step_key;  // the {job, slot, poses} list last uploaded
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   int max_pop = 0;
@@ -111,11 +127,15 @@ struct gdock_server {
   int64_t n_steps = 0, n_poses = 0, n_finished = 0;
   int32_t peak_active = 0;
   double busy_s = 0, kernel_s = 0, admit_s = 0, ga_s = 0, finish_s = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 
   void run();
   void admit_locked(std::vector<Job*>* fresh);
   bool step();
+  // One step's launches and GA advance (jobs prepared and in the table);
+  // returns the launch status.
+  hipError_t step_host_ga(const std::vector<Job*>& active);
+  hipError_t step_device_ga(const std::vector<Job*>& active);
   void finish_loop();
   void prepare_loop();
   // Frees the job's device state, then marks it ended (failed with `st` unless
@@ -185,69 +205,18 @@ bool gdock_server::step() {
     cv.wait_for(lk, std::chrono::milliseconds(2));
     return true;
   }
-  const auto t_step = std::chrono::steady_clock::now();
-
-  // The poses of the step (rows slot * max_pop + p); the launch plans its
-  // work items from them.  Jobs are prepared on the host threads.
-  std::vector<int> list_at(active.size() + 1, 0);
-  int max_row = 0;
-  for (size_t i = 0; i < active.size(); ++i) {
-    const int pop = active[i]->result->job.population();
-    list_at[i + 1] = list_at[i] + pop;
-    max_row = std::max(max_row, active[i]->slot * max_pop + pop);
-  }
-  const int n_list = list_at.back(), step_poses = n_list;
-  host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
-    for (int64_t i = lo; i < hi; ++i) {
-      const Job* j = active[i];
-      const GaJob& ga = j->result->job;
-      const int pop = ga.population(), row0 = j->slot * max_pop;
-      gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
-      for (int p = 0; p < pop; ++p) h_list[list_at[i] + p] = make_int2(j->slot, row0 + p);
-    }
-  });
-  int32_t head = n_list;
   hipError_t e = te;
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
-                                static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
-                                ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(const_cast<int2*>(L.pose_list), h_list, static_cast<size_t>(n_list) * sizeof(int2),
-                       hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(const_cast<int32_t*>(L.n_pose_list), &head, sizeof head, hipMemcpyHostToDevice,
-                       ctx->stream);
-  if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
-  if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
-  if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h_out, L.out, static_cast<size_t>(max_row) * kOutDoubles * sizeof(double),
-                       hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
-  float kms = 0.0f;
-  if (e == hipSuccess && hipEventElapsedTime(&kms, ev0, ev1) == hipSuccess) kernel_s += kms * 1e-3;
-  ++n_steps;
-  n_poses += step_poses;
+  if (e == hipSuccess) e = cfg.ga_on_device ? step_device_ga(active) : step_host_ga(active);
 
-  // Each job's GA takes its generation's results (independent jobs, on the
-  // host threads); finished jobs are then clustered one after another.
-  if (e == hipSuccess) {
-    const auto t_ga = std::chrono::steady_clock::now();
-    host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
-      for (int64_t i = lo; i < hi; ++i)
-        active[i]->result->job.absorb(h_out + static_cast<size_t>(active[i]->slot) * max_pop * kOutDoubles);
-    });
-    ga_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_ga).count();
-  }
   std::vector<Job*> failed, ended;
   const gdock_status launch_st = e != hipSuccess ? hip_fail(ctx, e, "server: generation launch") : GDOCK_OK;
   for (Job* j : active) {
     if (e != hipSuccess) {
       failed.push_back(j);
-    } else if (j->result->job.done()) {
+    } else if (cfg.ga_on_device ? h_done[j->slot] != 0 : j->result->job.done()) {
       ended.push_back(j);
+    } else {
+      j->n_cur &= ~1;  // the children of an odd population are one fewer (population.rs:74-158)
     }
   }
   {
@@ -266,10 +235,116 @@ bool gdock_server::step() {
   return true;
 }
 
+// The host GA: pose records prepared on the host threads, one graph
+// replay, the scores down, then every job's GA advanced on the host threads
+// (the jobs are independent).
+hipError_t gdock_server::step_host_ga(const std::vector<Job*>& active) {
+  const auto t_step = std::chrono::steady_clock::now();
+  std::vector<int> list_at(active.size() + 1, 0);
+  int max_row = 0;
+  for (size_t i = 0; i < active.size(); ++i) {
+    const int pop = active[i]->result->job.population();
+    list_at[i + 1] = list_at[i] + pop;
+    max_row = std::max(max_row, active[i]->slot * max_pop + pop);
+  }
+  const int n_list = list_at.back();
+  host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+      const Job* j = active[i];
+      const GaJob& ga = j->result->job;
+      const int pop = ga.population(), row0 = j->slot * max_pop;
+      gdock_prepare_poses(ga.genes().data(), pop, h_poses + static_cast<size_t>(row0) * kPoseDoubles);
+      for (int p = 0; p < pop; ++p) h_list[list_at[i] + p] = make_int2(j->slot, row0 + p);
+    }
+  });
+  const int32_t head = n_list;
+  hipError_t e = hipMemcpyAsync(const_cast<double*>(L.poses), h_poses,
+                                static_cast<size_t>(max_row) * kPoseDoubles * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<int2*>(L.pose_list), h_list, static_cast<size_t>(n_list) * sizeof(int2),
+                       hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(const_cast<int32_t*>(L.n_pose_list), &head, sizeof head, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_out, L.out, static_cast<size_t>(max_row) * kOutDoubles * sizeof(double),
+                       hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
+  float kms = 0.0f;
+  if (e == hipSuccess && hipEventElapsedTime(&kms, ev0, ev1) == hipSuccess) kernel_s += kms * 1e-3;
+  ++n_steps;
+  n_poses += n_list;
+  if (e == hipSuccess) {
+    const auto t_ga = std::chrono::steady_clock::now();
+    host_parallel_for(static_cast<int64_t>(active.size()), [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i)
+        active[i]->result->job.absorb(h_out + static_cast<size_t>(active[i]->slot) * max_pop * kOutDoubles);
+    });
+    ga_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_ga).count();
+  }
+  return e;
+}
+
+// The device GA: scoring graph, then one GA launch for every job, which
+// writes the next pose records in place; the host reads one done flag per slot.
+hipError_t gdock_server::step_device_ga(const std::vector<Job*>& active) {
+  const auto t_step = std::chrono::steady_clock::now();
+
+  // The poses of the step (rows slot * max_pop + p) and the jobs' GA views,
+  // uploaded when the set of jobs or a population size changed; the launch
+  // plans its work items from the list.
+  std::vector<int64_t> key;  // a job that takes over a slot is another key
+  key.reserve(3 * active.size());
+  for (const Job* j : active) key.insert(key.end(), {j->id, j->slot, j->n_cur});
+  int n_list = 0;
+  for (const Job* j : active) n_list += j->n_cur;
+  hipError_t e = hipSuccess;
+  if (key != step_key) {
+    int at = 0;
+    for (size_t i = 0; i < active.size(); ++i) {
+      const Job* j = active[i];
+      const int row0 = j->slot * max_pop;
+      for (int q = 0; q < j->n_cur; ++q) h_list[at++] = make_int2(j->slot, row0 + q);
+      h_ga[i] = j->ga;
+    }
+    const int32_t head = n_list;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(const_cast<int2*>(L.pose_list), h_list, static_cast<size_t>(n_list) * sizeof(int2),
+                         hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(const_cast<int32_t*>(L.n_pose_list), &head, sizeof head, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_ga, h_ga, active.size() * sizeof(DevGaJob), hipMemcpyHostToDevice, ctx->stream);
+    // the pageable count must be read before it leaves scope
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    step_key = e == hipSuccess ? key : std::vector<int64_t>();
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
+  if (e == hipSuccess) e = launch_ga_generation(d_ga, static_cast<int>(active.size()), ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev2, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_done, d_done, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
+  float kms = 0.0f, gms = 0.0f;
+  if (e == hipSuccess && hipEventElapsedTime(&kms, ev0, ev1) == hipSuccess) kernel_s += kms * 1e-3;
+  if (e == hipSuccess && hipEventElapsedTime(&gms, ev1, ev2) == hipSuccess) ga_s += gms * 1e-3;
+  ++n_steps;
+  n_poses += n_list;
+  return e;
+}
+
 void gdock_server::release(Job* j, gdock_status st, const std::string& message) {
   if (j->cx) gdock_complex_destroy(fctx ? fctx : ctx, j->cx);
   j->cx = nullptr;
   if (j->split_mem) { (void)hipFree(j->split_mem); j->split_mem = nullptr; }
+  if (j->ga_mem) { (void)hipFree(j->ga_mem); j->ga_mem = nullptr; }
   {
     std::lock_guard<std::mutex> g(mu);
     if (st != GDOCK_OK) {
@@ -325,12 +400,40 @@ void gdock_server::prepare_loop() {
         if (he != hipSuccess) st = hip_fail(pctx, he, "server: split state");
       }
     }
+    // The job's GA on the device: the initial population (drawn on the host,
+    // as every driver draws it), the stream state after it, and its pose
+    // records written straight into the slot's rows of the launch.
+    size_t ga_bytes = 0;
+    DevGaState s0{};
+    if (st == GDOCK_OK) j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
+    if (st == GDOCK_OK && cfg.ga_on_device) {
+      const int pop = j->result->job.population();
+      Layout GL;
+      GaJobOffsets go;
+      ga_job_layout(&GL, pop, j->params.max_generations, j->params.keep_trace != 0, &go);
+      ga_bytes = GL.size;
+      he = hipMalloc(&j->ga_mem, ga_bytes);
+      if (he == hipSuccess) {
+        const size_t row0 = static_cast<size_t>(j->slot) * max_pop;
+        j->ga = ga_job_bind(static_cast<char*>(j->ga_mem), go, j->params,
+                            const_cast<double*>(L.poses) + row0 * kPoseDoubles, L.out + row0 * kOutDoubles);
+        j->ga.done_flag = d_done + j->slot;
+        j->n_cur = pop;
+        s0 = ga_initial_state(j->result->job);
+        he = hipMemcpyAsync(j->ga.st, &s0, sizeof s0, hipMemcpyHostToDevice, pctx->stream);
+      }
+      if (he == hipSuccess)
+        he = hipMemcpyAsync(j->ga.genes[0], j->result->job.genes().data(), static_cast<size_t>(pop) * 6 * sizeof(double),
+                            hipMemcpyHostToDevice, pctx->stream);
+      if (he == hipSuccess) he = hipMemsetAsync(j->ga.done_flag, 0, sizeof(int32_t), pctx->stream);
+      if (he == hipSuccess) he = launch_ga_pose_records(j->ga.genes[0], pop, j->ga.poses, pctx->stream);
+      if (he != hipSuccess) st = hip_fail(pctx, he, "server: GA state");
+    }
     if (st == GDOCK_OK) {  // everything on the preparer's stream is done before the worker uses it
       he = hipStreamSynchronize(pctx->stream);
       if (he != hipSuccess) st = hip_fail(pctx, he, "server: job preparation");
     }
     if (st == GDOCK_OK) {
-      j->result = std::make_unique<gdock_ga_result>(j->seed, j->params);
       j->dev = j->cx->dev;
       j->dev.lanes_k = j->lanes;
       j->dev.min_parts = min_batch_parts(j->dev);
@@ -339,7 +442,7 @@ void gdock_server::prepare_loop() {
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_prep).count();
     if (st == GDOCK_OK) {
       std::lock_guard<std::mutex> g(mu);
-      const size_t charged = std::max(j->bytes, j->cx->mem_bytes + split_bytes);  // the larger of estimate and arena
+      const size_t charged = std::max(j->bytes, j->cx->mem_bytes + split_bytes + ga_bytes);  // the larger of estimate and arena
       bytes_in_use += charged - j->bytes;
       j->bytes = charged;
       j->ready = true;
@@ -369,7 +472,13 @@ void gdock_server::finish_loop() {
       to_finish.pop_front();
     }
     gdock_status st = GDOCK_OK;
-    if (j->params.cluster) {
+    if (j->ga_mem) {  // the device GA's results into the job's host state, then its memory is free
+      const hipError_t he = ga_job_download(j->ga, j->params, fctx->stream, &j->result->job);
+      if (he != hipSuccess) st = hip_fail(fctx, he, "server: GA results");
+      (void)hipFree(j->ga_mem);
+      j->ga_mem = nullptr;
+    }
+    if (st == GDOCK_OK && j->params.cluster) {
       const auto t_fcc = std::chrono::steady_clock::now();
       st = finish_job(fctx, j->cx, &j->result->job, &j->result->clusters);
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_fcc).count();
@@ -396,6 +505,7 @@ gdock_status gdock_server_default_config(gdock_server_config* c) {
   c->max_jobs = 64;
   c->max_population = 150;
   c->max_receptor_atoms = 8192;
+  c->ga_on_device = 0;
   c->memory_budget = 0;
   c->weights = gdock_weights{0.4, 0.05, 3.4, 1.0};
   return GDOCK_OK;
@@ -406,7 +516,8 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   *out = nullptr;
   gdock_server_config c;
   if (config) c = *config; else gdock_server_default_config(&c);
-  if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_population > 65536 || c.max_receptor_atoms < 1 ||
+  if (c.max_jobs < 1 || c.max_jobs > 4096 || c.max_population < 2 || c.max_population > 65536 ||
+      (c.ga_on_device && c.max_population > kGaMaxPopulation) || c.max_receptor_atoms < 1 ||
       c.max_receptor_atoms > kMaxReceptorAtoms || c.memory_budget < 0 ||
       static_cast<int64_t>(c.max_jobs) * c.max_population * kMaxPoseParts > (int64_t{1} << 30))
     return GDOCK_ERR_INVALID_ARG;
@@ -429,16 +540,22 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
                o_items = Ly.add<int2>(cap_items), o_n = Ly.add<int32_t>(4),
                o_poses = Ly.add<double>(cap * kPoseDoubles), o_out = Ly.add<double>(cap * kOutDoubles),
                o_ns = Ly.add<int32_t>(cap), o_redo = Ly.add<int32_t>(cap),
-               o_order = Ly.add<uint8_t>(order_bytes(cap));
+               o_order = Ly.add<uint8_t>(order_bytes(cap)), o_ga = Ly.add<DevGaJob>(c.max_jobs),
+               o_done = Ly.add<int32_t>(c.max_jobs);
   hipError_t e = hipMalloc(&s->mem, Ly.size);
   if (e == hipSuccess) e = hipMemsetAsync(s->mem, 0, Ly.size, s->ctx->stream);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), cap * kOutDoubles * sizeof(double));
+  if (e == hipSuccess && !c.ga_on_device)
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->h_poses), cap * kPoseDoubles * sizeof(double));
+  if (e == hipSuccess && !c.ga_on_device)
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), cap * kOutDoubles * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_list), cap * sizeof(int2));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_table), c.max_jobs * sizeof(DevComplex));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_ga), c.max_jobs * sizeof(DevGaJob));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_done), c.max_jobs * sizeof(int32_t));
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
   if (e == hipSuccess) e = hipEventCreate(&s->ev0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev2);
   char* m = static_cast<char*>(s->mem);
   BatchLaunch& L = s->L;
   if (e == hipSuccess) {
@@ -454,6 +571,8 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
     L.nsat = reinterpret_cast<int32_t*>(m + o_ns);
     L.redo = reinterpret_cast<int32_t*>(m + o_redo);
     bind_order(&L, m + o_order, cap);
+    s->d_ga = reinterpret_cast<DevGaJob*>(m + o_ga);
+    s->d_done = reinterpret_cast<int32_t*>(m + o_done);
     // a job's items cover at most min(its receptor slots, kPartSlots) slots;
     // class padding adds at most 3 slots per (rmin2, eps) class
     s->lds_slots = std::min(kPartSlots, (c.max_receptor_atoms + 256 + 63) / 64 * 64);
@@ -516,8 +635,10 @@ void gdock_server_destroy(gdock_server* s) {
       Job* j = kv.second.get();
       if (j->cx) gdock_complex_destroy(s->ctx, j->cx);
       if (j->split_mem) (void)hipFree(j->split_mem);
+      if (j->ga_mem) (void)hipFree(j->ga_mem);
       j->cx = nullptr;
       j->split_mem = nullptr;
+      j->ga_mem = nullptr;
       if (j->state != kFinished && j->state != kFailed) {
         j->state = kFailed;
         j->error = GDOCK_ERR_NOT_FOUND;
@@ -531,12 +652,15 @@ void gdock_server_destroy(gdock_server* s) {
   if (s->ctx) (void)hipSetDevice(s->ctx->device);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev2) (void)hipEventDestroy(s->ev2);
   if (s->exec) (void)hipGraphExecDestroy(s->exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->mem) (void)hipFree(s->mem);
   if (s->h_poses) (void)hipHostFree(s->h_poses);
   if (s->h_out) (void)hipHostFree(s->h_out);
   if (s->h_list) (void)hipHostFree(s->h_list);
+  if (s->h_ga) (void)hipHostFree(s->h_ga);
+  if (s->h_done) (void)hipHostFree(s->h_done);
   if (s->h_table) (void)hipHostFree(s->h_table);
   if (s->ctx) gdock_ctx_destroy(s->ctx);
   if (s->fctx) gdock_ctx_destroy(s->fctx);

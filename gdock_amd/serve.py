@@ -29,10 +29,11 @@ class JobTimes:
 class Server:
     def __init__(self, device: int = 0, max_jobs: int = 64, max_population: int = 150,
                  max_receptor_atoms: int = 8192, memory_budget: int = 0,
-                 weights: constants.EnergyWeights | None = None):
+                 weights: constants.EnergyWeights | None = None, ga_on_device: bool = False):
         c = ServerConfig()
         check(lib().gdock_server_default_config(C.byref(c)), what="server_default_config")
         c.max_jobs, c.max_population, c.max_receptor_atoms = max_jobs, max_population, max_receptor_atoms
+        c.ga_on_device = 1 if ga_on_device else 0
         c.memory_budget = memory_budget
         c.weights = weights_c(weights)
         h = C.c_void_p()

@@ -319,10 +319,12 @@ gdock_status gdock_calc_metrics(gdock_ctx* ctx, gdock_evaluator* ev, const doubl
  * share its launches.  The server owns its own context on `device`. */
 typedef struct gdock_server_config {
   int32_t max_jobs;            /* active job slots (64) */
-  int32_t max_population;      /* largest population a job may use (150; at most 65536; max_jobs x
-                                  max_population x 8 at most 2^30) */
+  int32_t max_population;      /* largest population a job may use (150; at most 65536, 1024 with
+                                  ga_on_device; max_jobs x max_population x 8 at most 2^30) */
   int32_t max_receptor_atoms;  /* sizes the kernel's LDS receptor table (8192) */
-  int32_t reserved;
+  int32_t ga_on_device;        /* 0: the GAs advance on host threads between steps (default, faster);
+                                  1: on the device (gdock_ga_run_device's kernel), population state
+                                  resident in HBM, no genes or scores cross to the host */
   int64_t memory_budget;       /* device bytes for resident jobs; 0: half the free memory */
   gdock_weights weights;       /* one energy weighting for every job */
 } gdock_server_config;
@@ -337,7 +339,8 @@ typedef struct gdock_server_stats_t {
   int64_t bytes_in_use, memory_budget;
   double kernel_s;             /* device time of the scoring graph replays (HIP events) */
   double admit_s;              /* time preparing admitted jobs (positioning, complex, lanes), on a thread of its own */
-  double ga_s;                 /* host time advancing the GAs after each step (on the scoring thread) */
+  double ga_s;                 /* time advancing the GAs after each step: host time on the scoring
+                                  thread, or device time of the GA launches (ga_on_device) */
   double finish_s;             /* time finishing jobs (FCC of the Hall of Fame, model selection), on a thread of its own */
 } gdock_server_stats_t;
 typedef struct gdock_server gdock_server;

@@ -169,3 +169,48 @@ def test_destroy_ends_pending_waits():
     srv.close()
     t.join(60.0)
     assert not t.is_alive() and got["status"] == _lib.NOT_FOUND
+
+
+def test_device_ga_alone_equals_in_mix():
+    """With ga_on_device the populations stay in HBM and advance in one GA
+    launch per step (one workgroup per job, each job its own stream): a job is
+    bit-identical alone and in a mix, trace, Hall of Fame and models too."""
+    jobs = [_job(60 + i, 20 + 7 * i, 18 + 5 * i) for i in range(4)]
+    srv = Server(max_jobs=8, ga_on_device=True)
+    try:
+        alone, _ = srv.wait(srv.submit(*jobs[0], seed=3, **SMALL), 180.0)
+        order = [1, 0, 2, 3]
+        ids = [srv.submit(*jobs[i], seed=3 if i == 0 else 10 + i, **SMALL) for i in order]
+        got = {i: srv.wait(j, 180.0) for i, j in zip(order, ids)}
+        _same(alone, got[0][0])
+        st = srv.stats()
+        assert st["peak_active"] >= 2 and st["ga_s"] > 0
+        for i in order:
+            assert got[i][0].generations >= 1 and got[i][1].generations == got[i][0].generations
+    finally:
+        srv.close()
+
+
+def test_device_ga_served_job_matches_oracle_ga():
+    """The 60 x 76-residue job served with the GA on the device takes the
+    oracle GA's decisions on the same stream (near-ties of the two scorers
+    aside), and equals the same job served with the host GA wherever their
+    scores agree."""
+    rec, lig, pairs = _job(0, 60, 76, 6)
+    srv = Server(max_jobs=4, ga_on_device=True)
+    try:
+        res, t = srv.wait(srv.submit(rec, lig, pairs, seed=42, keep_trace=1), 180.0)
+    finally:
+        srv.close()
+    lig_pos = structure.position_ligand(rec, lig)
+    ora = orc.GaRun(rec.atoms, lig_pos.atoms, pairs, W, seed=42, pop_size=150, max_gen=250, nthreads=16)
+    outcome, compared = _compare(res, ora, 150)
+    print("device-GA served vs oracle:", outcome, f"latency {t.latency_s:.2f} s")
+    assert compared >= min(10, res.generations), outcome
+
+
+def test_device_ga_population_limit():
+    with pytest.raises(_lib.GdockError):
+        Server(max_jobs=2, max_population=2048, ga_on_device=True)
+    srv = Server(max_jobs=2, max_population=2048)  # the host GA has no such limit
+    srv.close()
