@@ -124,8 +124,11 @@ struct Weights { double vdw, elec, desolv, air; };
 // Cost buckets of the per-launch item order (half-octave steps of the
 // estimated pair work).
 constexpr int kCostBuckets = 64;
-// The plain path orders the poses of a launch by cost when it has at most
-// kOrderMaxPoses of them (one workgroup of kOrderThreads places them).
+// The plain path orders the poses of a launch by cost when its workgroups
+// exceed one round of an MI355X's slots (256 CUs x 4 scoring workgroups) and
+// it has at most kOrderMaxPoses poses (one workgroup of kOrderThreads places
+// them).
+constexpr int kOrderMinItems = 1024;
 constexpr int kOrderMaxPoses = 1 << 16;
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderBuckets = 256;  // eighth-octave steps of the estimated cost

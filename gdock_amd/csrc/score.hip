@@ -1157,10 +1157,12 @@ hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const doub
     if (e != hipSuccess) return e;
   }
   const int k = lanes_k > 0 ? lanes_k : score_atoms_per_lane(cx.n_lig);
-  // Poses in order of estimated cost, most expensive first (pose records
+  // Poses in order of estimated cost, most expensive first, when the launch
+  // takes more than one round of the device's workgroup slots (pose records
   // only: coordinates as given carry no pose centre to estimate from).
   int32_t* d_order = nullptr;
-  if (d_poses && !d_coords && !counts && n_poses > 1 && n_poses <= kOrderMaxPoses) {
+  if (d_poses && !d_coords && !counts && static_cast<int64_t>(n_poses) * parts > kOrderMinItems &&
+      n_poses <= kOrderMaxPoses) {
     d_order = reinterpret_cast<int32_t*>(m + SL.order);
     uint8_t* d_keys = reinterpret_cast<uint8_t*>(m + SL.keys);
     hipLaunchKernelGGL(plain_cost_kernel, dim3((n_poses + 3) / 4), dim3(256), 0, stream, d_cx, d_poses, n_poses, k,

@@ -283,3 +283,22 @@ def test_culling_keeps_every_pair_within_the_cutoff():
           f"brute force within 15 A: {lo}..{hi}")
     assert stepped % 64 == 0 and 0 < stepped <= padded
     assert lo <= within <= hi and within <= stepped
+
+
+def test_launch_order_changes_no_result():
+    """A launch of more poses than one round of workgroup slots runs them in
+    order of estimated cost; every pose is still scored by one workgroup, so
+    its results are bit-identical to the same pose in launches small enough to
+    run in pose order."""
+    c = synth.make_complex(23, rec_atoms=1200, lig_atoms=700, n_restraints=5)
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+    cx = Complex(rec, lig, c.restraints)
+    genes = np.concatenate([synth.random_genes(24, 1800, 18.0), synth.ga_genes(25, 600)])
+    whole, ns_whole = cx.score_poses(genes)
+    parts = [cx.score_poses(genes[i:i + 600]) for i in range(0, len(genes), 600)]
+    assert np.array_equal(whole, np.concatenate([p[0] for p in parts]))
+    assert np.array_equal(ns_whole, np.concatenate([p[1] for p in parts]))
+    ref, rns = orc.score_poses(rec.atoms, lig.atoms, c.restraints, genes[::150], W, nthreads=16)
+    assert_energies_close(whole[::150], ref, "ordered launch vs oracle")
+    assert np.array_equal(ns_whole[::150], rns)

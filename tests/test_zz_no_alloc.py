@@ -82,11 +82,13 @@ def test_noalloc_equals_device_entry_twice_second_smaller():
     q = np.pi / 2
     lattice_genes = np.array([[0, 0, 0, 0, 0, 0], [q, 0, 0, 0, 0, 0], [0, q, 0, 0, 0, 0], [0, 0, q, 0, 0, 1.0]])
 
-    g1 = np.concatenate([synth.random_genes(11, 200, 16.0), synth.ga_genes(12, 100)])  # 300 poses
-    g2 = synth.random_genes(13, 90, 12.0)                                               # then 90
+    # 1500 poses: more than one round of workgroups, so launched in cost order
+    # with the order in the scratch; then 90, in pose order
+    g1 = np.concatenate([synth.random_genes(11, 1400, 16.0), synth.ga_genes(12, 100)])
+    g2 = synth.random_genes(13, 90, 12.0)
     stream = torch.cuda.Stream(device=dev)
-    d_out = torch.zeros((400, 6), dtype=torch.float64, device=dev)
-    d_ns = torch.zeros(400, dtype=torch.int32, device=dev)
+    d_out = torch.zeros((1600, 6), dtype=torch.float64, device=dev)
+    d_ns = torch.zeros(1600, dtype=torch.int32, device=dev)
     d_p1 = torch.from_numpy(_poses(g1)).to(dev)
     d_p2 = torch.from_numpy(_poses(g2)).to(dev)
     d_pl = torch.from_numpy(_poses(lattice_genes)).to(dev)
@@ -106,7 +108,7 @@ def test_noalloc_equals_device_entry_twice_second_smaller():
     free1 = _free_bytes(hip)
     assert free1 == free0, (free0, free1)
 
-    for got, ref, what in ((got1, ref1, "first call (300 poses)"), (got2, ref2, "second call (90 poses)"),
+    for got, ref, what in ((got1, ref1, "first call (1500 poses)"), (got2, ref2, "second call (90 poses)"),
                            (gotl, refl, "lattice poses (exact path)")):
         assert np.array_equal(got[0], ref[0]), what
         assert np.array_equal(got[1], ref[1]), what
