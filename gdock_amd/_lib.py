@@ -160,6 +160,7 @@ def lib() -> C.CDLL:
         "gdock_format_pdb": (C.c_int, [P, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "gdock_apply_genes": (C.c_int, [P, C.c_int32, D, P]),
         "gdock_write_best_outputs": (C.c_int, [P, P, P, C.POINTER(Weights), D, C.c_int32, C.c_char_p]),
+        "gdock_write_best_outputs_dockq": (C.c_int, [P, P, P, D, C.c_int32, C.c_char_p]),
         "gdock_ga_final_population": (C.c_int32, [P, D]),
         "gdock_write_run_outputs": (C.c_int, [P, P, P, D, D, C.c_int32, C.POINTER(C.c_int64),
                                               C.POINTER(C.c_int64), C.c_int32, C.c_char_p]),

@@ -227,28 +227,28 @@ gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_ev
   return GDOCK_OK;
 }
 
-gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const gdock_weights* w,
-                                      const double* genes, int32_t n, const char* out_dir) {
-  if (!ctx || !cx || !w || !out_dir || n <= 0 || !genes) {
-    if (ctx) set_error(ctx, "write_best_outputs: bad arguments");
-    return GDOCK_ERR_INVALID_ARG;
-  }
+namespace {
+// run.rs:549-637 from the final population's fitness (kOutDoubles per pose,
+// the fitness at 5) and, with an evaluator, its metrics: best_by_score.pdb is
+// the first pose of least fitness (Iterator::min_by), best_by_dockq.pdb the
+// last pose of greatest DockQ (Iterator::max_by), and metrics.tsv both rows.
+gdock_status write_best(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const double* genes, int32_t n,
+                        const std::vector<double>& out, const std::vector<double>* metrics, const char* out_dir) {
   const std::filesystem::path dir(out_dir);
   if (!make_dir(ctx, out_dir)) return GDOCK_ERR_INVALID_ARG;
-  // run.rs:553-565: score the final population, the first individual of least fitness
-  std::vector<double> out(static_cast<size_t>(n) * kOutDoubles);
-  gdock_status st = gdock_score_poses(ctx, cx, genes, n, w, out.data(), nullptr);
-  if (st != GDOCK_OK) return st;
   int32_t best = 0;
   for (int32_t i = 1; i < n; ++i)
     if (out[kOutDoubles * i + 5] < out[kOutDoubles * best + 5]) best = i;
   std::vector<gdock_atom> complex;
   if (!write_complex(ctx, cx, dir, "best_by_score", genes + 6 * best, &complex)) return GDOCK_ERR_INVALID_ARG;
   if (!ev) return GDOCK_OK;
-  // run.rs:577-634: the last individual of greatest DockQ, and metrics.tsv
   std::vector<double> m(static_cast<size_t>(n) * 4);  // {rmsd, irmsd, fnat, dockq}
-  st = gdock_calc_metrics(ctx, ev, genes, n, m.data());
-  if (st != GDOCK_OK) return st;
+  if (metrics) {
+    m = *metrics;
+  } else {
+    const gdock_status st = gdock_calc_metrics(ctx, ev, genes, n, m.data());
+    if (st != GDOCK_OK) return st;
+  }
   int32_t top = 0;
   for (int32_t i = 1; i < n; ++i)
     if (m[4 * i + 3] >= m[4 * top + 3]) top = i;
@@ -264,6 +264,36 @@ gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_e
     return GDOCK_ERR_INVALID_ARG;
   }
   return GDOCK_OK;
+}
+}  // namespace
+
+gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const gdock_weights* w,
+                                      const double* genes, int32_t n, const char* out_dir) {
+  if (!ctx || !cx || !w || !out_dir || n <= 0 || !genes) {
+    if (ctx) set_error(ctx, "write_best_outputs: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  // run.rs:553-565: the final population scored with the energy weights
+  std::vector<double> out(static_cast<size_t>(n) * kOutDoubles);
+  const gdock_status st = gdock_score_poses(ctx, cx, genes, n, w, out.data(), nullptr);
+  if (st != GDOCK_OK) return st;
+  return write_best(ctx, cx, ev, genes, n, out, nullptr, out_dir);
+}
+
+gdock_status gdock_write_best_outputs_dockq(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev,
+                                            const double* genes, int32_t n, const char* out_dir) {
+  if (!ctx || !cx || !ev || !out_dir || n <= 0 || !genes) {
+    if (ctx) set_error(ctx, "write_best_outputs_dockq: bad arguments");
+    return GDOCK_ERR_INVALID_ARG;
+  }
+  // debug mode (chromosome.rs:97-108): the fitness of a pose is -DockQ and
+  // its energies are zero
+  std::vector<double> m(static_cast<size_t>(n) * 4);
+  const gdock_status st = gdock_calc_metrics(ctx, ev, genes, n, m.data());
+  if (st != GDOCK_OK) return st;
+  std::vector<double> out(static_cast<size_t>(n) * kOutDoubles, 0.0);
+  for (int32_t i = 0; i < n; ++i) out[kOutDoubles * static_cast<size_t>(i) + 5] = -m[4 * static_cast<size_t>(i) + 3];
+  return write_best(ctx, cx, ev, genes, n, out, &m, out_dir);
 }
 
 }  // extern "C"

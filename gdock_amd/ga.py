@@ -107,11 +107,19 @@ def write_outputs(cx: Complex, result: GaResult, out_dir, evaluator=None) -> Non
 
 
 def write_best_outputs(cx: Complex, result: GaResult, out_dir, evaluator=None,
-                       weights: constants.EnergyWeights | None = None) -> None:
+                       weights: constants.EnergyWeights | None = None, debug: bool = False) -> None:
     """commands::run's `--no-clust` output (run.rs:549-637): best_by_score.pdb,
     and with an evaluator best_by_dockq.pdb and metrics.tsv, from the final
-    population."""
+    population.  debug: the reference's debug mode, where a pose's fitness is
+    -DockQ (chromosome.rs:97-108); it needs the evaluator."""
     g = np.ascontiguousarray(result.final_genes, dtype=np.float64).reshape(-1, 6)
+    if debug:
+        if evaluator is None:
+            raise ValueError("debug outputs rank poses by DockQ: an evaluator is required")
+        check(lib().gdock_write_best_outputs_dockq(cx.ctx.handle, cx.handle, evaluator.handle, ptr(g), len(g),
+                                                   str(out_dir).encode()),
+              cx.ctx.handle, "gdock_write_best_outputs_dockq")
+        return
     w = weights_c(weights)
     check(lib().gdock_write_best_outputs(cx.ctx.handle, cx.handle, evaluator.handle if evaluator else None,
                                          C.byref(w), ptr(g), len(g), str(out_dir).encode()),

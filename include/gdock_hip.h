@@ -290,6 +290,13 @@ gdock_status gdock_write_run_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_ev
  * score for the two models, four decimals). */
 gdock_status gdock_write_best_outputs(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev, const gdock_weights* w,
                                       const double* genes, int32_t n, const char* out_dir);
+/* The same outputs in the reference's debug mode (`--debug --no-clust`,
+ * src/commands/run.rs:549-637 with the debug fitness of
+ * src/chromosome.rs:97-108): each pose's fitness is -DockQ (energies zero),
+ * so best_by_score.pdb is the first pose of greatest DockQ and the score
+ * column of metrics.tsv is -DockQ.  The evaluator is required. */
+gdock_status gdock_write_best_outputs_dockq(gdock_ctx* ctx, gdock_complex* cx, gdock_evaluator* ev,
+                                            const double* genes, int32_t n, const char* out_dir);
 
 /* ---- DockQ evaluator --------------------------------------------------------- */
 /* Evaluator::new (src/evaluator.rs:50-66) for the complex's receptor and a

@@ -84,3 +84,42 @@ def test_ga_debug_mode_dockq_matches_oracle():
     assert compared >= min(10, res.generations), outcome
     assert (res.fitness <= 0).all() and res.fitness.min() >= -1.0 - 1e-12
     print("debug-mode GA vs oracle:", outcome, res.generations, "generations, best DockQ", -res.fitness.min())
+
+
+def test_debug_no_clust_outputs_rank_by_dockq(tmp_path):
+    """`--debug --no-clust` (run.rs:549-637 with the debug fitness of
+    chromosome.rs:97-108): the final population's fitness is -DockQ, so
+    best_by_score.pdb is its first pose of greatest DockQ, best_by_dockq.pdb
+    its last such pose, and metrics.tsv carries -DockQ as the score."""
+    from gdock_amd import ga
+    rec = structure.read_pdb_text(read_text("2oob_A.pdb"))[0]
+    native = structure.read_pdb_text(read_text("2oob_B.pdb"))[0]
+    lig = structure.position_ligand(rec, native)
+    cx = Complex(rec, lig)
+    ev = Evaluator(cx, native)
+    res = ga.run_dockq(cx, ev, seed=5, population_size=60, max_generations=12, cluster=0)
+    fin = res.final_genes
+    # two copies of the best pose: the first and the last of greatest DockQ differ
+    met0 = ev.calc_metrics(fin)
+    k = int(np.argmax(met0[:, 3]))
+    fin = np.concatenate([fin, fin[k:k + 1]])
+    res.final_genes = fin
+    out = tmp_path / "debug"
+    ga.write_best_outputs(cx, res, out, evaluator=ev, debug=True)
+    met = ev.calc_metrics(fin)
+    ref = orc.Evaluator(rec.atoms, native.atoms).calc_metrics([orc.apply_genes(lig.atoms, g) for g in fin])
+    assert np.allclose(met, ref, rtol=1e-12, atol=1e-12)
+    fit = -met[:, 3]
+    best = int(np.argmin(fit))                                   # first minimum of -DockQ
+    top = len(met) - 1 - int(np.argmax(met[::-1, 3]))            # last maximum of DockQ
+    assert best == k and top == len(fin) - 1
+    for name, i in (("best_by_score", best), ("best_by_dockq", top)):
+        moved = structure.apply_genes(lig, fin[i])
+        assert (out / f"{name}.pdb").read_text() == structure.to_pdb_string(structure.combine_molecules(rec, moved))
+    rows = (out / "metrics.tsv").read_text().splitlines()
+    assert rows[0] == "model\tdockq\trmsd\tirmsd\tfnat\tscore"
+    for row, (name, i) in zip(rows[1:], (("best_by_score", best), ("best_by_dockq", top))):
+        rmsd, irmsd, fnat, dockq = met[i]
+        assert row.split("\t") == [name, f"{dockq:.4f}", f"{rmsd:.4f}", f"{irmsd:.4f}", f"{fnat:.4f}", f"{-dockq:.4f}"]
+    with pytest.raises(ValueError):
+        ga.write_best_outputs(cx, res, tmp_path / "x", debug=True)
