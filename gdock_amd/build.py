@@ -41,13 +41,14 @@ def _run(cmd: list[str]) -> subprocess.CompletedProcess:
     return subprocess.run(cmd, capture_output=True, text=True)
 
 
-def build(verbose: bool = True) -> Path:
-    """One object per source (compiled in parallel into gdock_amd/_build),
-    then one link."""
+def build(verbose: bool = True, lib: Path = LIB, objdir: Path | None = None, extra_flags=()) -> Path:
+    """One object per source (compiled in parallel into gdock_amd/_build, or
+    objdir), then one link into lib.  extra_flags go to every compile and to
+    the link (the host-sanitizer test builds a copy with them)."""
     from concurrent.futures import ThreadPoolExecutor
-    out = PKG / "_build"
-    out.mkdir(exist_ok=True)
-    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    out = objdir or PKG / "_build"
+    out.mkdir(parents=True, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"] + list(extra_flags)
     jobs = []
     for s in sources():
         obj = out / (Path(s).name + ".o")
@@ -63,16 +64,16 @@ def build(verbose: bool = True) -> Path:
             sys.stderr.write(res.stdout + res.stderr)
             raise RuntimeError(f"hipcc failed compiling {cmd[-3]}")
         notes += [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
-    tmp = LIB.with_suffix(".so.tmp")
-    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *[j[-1] for j in jobs], "-o", str(tmp)]
+    tmp = lib.with_suffix(".so.tmp")
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *extra_flags, *[j[-1] for j in jobs], "-o", str(tmp)]
     res = _run(link)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed linking libgdock_hip.so")
     if notes and verbose:
         sys.stderr.write("\n".join(notes) + "\n")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
