@@ -4,9 +4,11 @@ Run on a GPU as ``python -m gdock_amd.tune [--out PATH]``.  For every
 receptor size class (half octaves of receptor atoms) and every 128-atom
 ligand interval with more than one candidate K, it builds a seeded synthetic
 complex of that shape (receptor atoms 2^(class/2), ligand atoms the
-interval's midpoint), times each candidate (gdock_time_lanes: 128 probe
-poses near the receptor, one warm-up launch and 3 timed ones) and writes the
-fastest candidate's chunk count to gdock_amd/csrc/lanes_table.inc.  The
+interval's midpoint), times each candidate (gdock_time_lanes: 512 probe
+poses near the receptor, one warm-up launch and 5 timed ones) and writes the
+chunk count of the fewest-chunk candidate, or of the fastest one when it is
+more than 3% faster (near-ties would flip from run to run), to
+gdock_amd/csrc/lanes_table.inc.  The
 library reads that table when it is built; nothing is measured at run time,
 so a complex's K, and with it the summation order of its scores, depends on
 its shape alone.
@@ -50,8 +52,10 @@ def main(argv=None) -> None:
     ap.add_argument("--out", default=str(TABLE))
     ap.add_argument("--classes", default="17-26", help="receptor size classes, e.g. 17-26")
     ap.add_argument("--max-lig", type=int, default=3072)
-    ap.add_argument("--poses", type=int, default=128)
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--poses", type=int, default=512)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--margin", type=float, default=0.03,
+                    help="a candidate with more chunks must be this much faster to be chosen")
     args = ap.parse_args(argv)
     from . import _lib, structure, synth
     from .device import Complex, default_context
@@ -79,7 +83,10 @@ def main(argv=None) -> None:
                            "gdock_time_lanes")
                 times[k] = ms.value
             cx.close()
-            best = min(ks, key=lambda k: (times[k], k))
+            # the fewest chunks (largest K) unless another candidate is faster
+            # by more than the margin: near-ties would flip from run to run
+            fastest = min(ks, key=lambda k: (times[k], k))
+            best = fastest if times[fastest] < (1.0 - args.margin) * times[ks[0]] else ks[0]
             rows.append((n_lig, cls, chunks_of(n_lig, best)))
             print(f"class {cls} (receptor {n_rec}) ligand {n_lig}: " +
                   ", ".join(f"K={k} {times[k]:.3f} ms" for k in ks) + f" -> K={best} ({time.time() - t0:.0f} s)",
