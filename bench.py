@@ -202,6 +202,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": measured_traffic(),
+            "hbm_gb_per_s": (measured_traffic() or 0.0) / (kern_ms * 1e-3) / 1e9 if measured_traffic() else None,
+            "hbm_frac": (measured_traffic() or 0.0) / (kern_ms * 1e-3) / PEAK_HBM if measured_traffic() else None,
             "valu_issue_frac": measured_valu_issue(),
             "kernel": "score_poses_kernel",
             "flop_per_launch": flop,
