@@ -114,6 +114,49 @@ def test_receptor_larger_than_one_workgroup():
     assert b"split" in _lib.lib().gdock_last_error(cx.ctx.handle)
 
 
+def _tiled_receptor(rec, copies, step):
+    """copies of the receptor side by side along x, step A apart, serials
+    renumbered from 1 (PDB text cannot hold this many atoms)."""
+    parts = []
+    for c in range(copies):
+        a = rec.atoms.copy()
+        a["x"] = a["x"] + c * step
+        parts.append(a)
+    atoms = np.concatenate(parts)
+    atoms["serial"] = np.arange(1, len(atoms) + 1, dtype=atoms["serial"].dtype)
+    return structure.Molecule(atoms)
+
+
+def test_maximum_sizes():
+    """A receptor of 100,000 atoms (five copies of a 20,000-atom chain 30 A
+    apart along x, so a pose sees several; seven receptor ranges per pose,
+    the limit is 122,624 atoms) and a ligand of about 6,000 atoms (three
+    chunks of eight atoms per lane): the plain path against the oracle on a
+    pose at the centre, two random and one GA-initial pose; one receptor atom
+    past the limit is refused with a clear error."""
+    from gdock_amd import _lib
+    base = synth.make_complex(31, rec_atoms=20000, lig_atoms=6000, n_restraints=6)
+    rec = _tiled_receptor(structure.read_pdb_text(base.receptor_pdb)[0], 5, 30.0)
+    lig0 = structure.read_pdb_text(base.ligand_pdb)[0]
+    atoms = lig0.atoms.copy()
+    atoms["serial"] = np.arange(len(rec) + 1, len(rec) + 1 + len(atoms), dtype=atoms["serial"].dtype)
+    lig = structure.position_ligand(rec, structure.Molecule(atoms))
+    pairs = base.restraints
+    assert len(rec) == 100000 and len(lig) > 5000, (len(rec), len(lig))
+    cx = Complex(rec, lig, pairs)
+    genes = np.concatenate([np.zeros((1, 6)), synth.random_genes(32, 2, 40.0), synth.ga_genes(33, 1)])
+    got, ns = cx.score_poses(genes)
+    ref, rns = orc.score_poses(rec.atoms, lig.atoms, pairs, genes, W, nthreads=16)
+    assert_energies_close(got, ref, f"{len(rec)} x {len(lig)} atoms")
+    assert np.array_equal(ns, rns)
+    assert (np.abs(ref[:, 1]) > 0).all()
+    n_over = 122624 + 1 - len(rec)
+    too_big = structure.Molecule(np.concatenate([rec.atoms, rec.atoms[:n_over]]))
+    with pytest.raises(_lib.GdockError) as e:
+        Complex(too_big, lig, pairs)
+    assert "122624" in str(e.value)
+
+
 _CHILD = r"""
 import json, sys
 import numpy as np
