@@ -511,6 +511,17 @@ def bench_ga(args):
         "job_wall_s": wall, "generations": res.generations, "hof": len(res.hof_fitness),
         "clusters": len(res.cluster_center), "selected": res.selected,
     }
+    # the same job with the GA generation on the device (gdock_ga_run_device)
+    for _ in range(args.warmup):
+        ga.run_device(cx, seed=42)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dres = ga.run_device(cx, seed=42)
+    dwall = (time.perf_counter() - t0) / args.steps
+    result["device_ga"] = {"job_wall_s": dwall, "pose_evals_per_s": dres.generations * 150 / dwall,
+                           "generations": dres.generations, "selected": dres.selected,
+                           "same_run_as_host_driver": bool(dres.generations == res.generations and
+                                                           dres.selected == res.selected)}
     if not args.no_cpu:
         import oracle.oracle as orc
         cpu = host_cpus()
