@@ -8,9 +8,11 @@
 #include "../../include/gdock_hip.h"
 #include "ga.h"
 #include "internal.h"
+#include "stream.h"
 
 namespace gdock {
 
+constexpr int kGaThreads = 256;          // one workgroup per job
 constexpr int kGaMaxPopulation = 1024;   // a job's population in one workgroup's LDS
 constexpr int kGaHofCapacity = 500;      // run.rs:84-99
 constexpr int kGaHofPerGeneration = 10;  // run.rs:194-216
@@ -33,7 +35,8 @@ struct DevGaState {
   int32_t done;         // 1 once the GA has ended (and, after its last generation, bred once more)
   int32_t hof_n, hof_sorted;
   int32_t trace_at;     // individuals traced so far
-  int32_t pad[2];
+  int32_t base_ok;      // 1 when the job's rng_base holds T^i rng for i < kGaBaseStates
+  int32_t pad;
 };
 
 // Kernel view of one job: its state, buffers, parameters, and where its
@@ -55,18 +58,45 @@ struct DevGaJob {
   double* poses;        // the generation's pose records (scored), then the next generation's
   const double* out;    // the generation's scores (kOutDoubles per pose)
   int32_t* done_flag;   // optional: set to the state's done flag after every generation
+  uint64_t* rng_base;   // kGaBaseStates x 4: the stream's next kGaBaseStates states
+  uint64_t* draws;      // kGaThreads x draw_span raw outputs of the generation's stream
+  int32_t draw_span;    // outputs per lane the job's buffers allow (0: one lane draws them all)
   int32_t pad;
   DevGaParams p;
 };
 
 struct GaJobOffsets {
   size_t state, genes0, genes1, child, mut_u, hof_genes, hof_fit, hof_gen, tmp_genes, tmp_fit, tmp_gen;
+  size_t rng_base, draws;
   size_t trace_genes, trace_fit, trace_best;
+  int32_t draw_span;
 };
 
+// The generation's draws in the host driver's order, for a population of n:
+// n x tsize tournament draws, n - 1 for the shuffle, 6 per crossed pair, then
+// per child gene one mutation draw and, when it mutates, two more.  The
+// workgroup takes them as kGaThreads lanes of draw_span consecutive outputs
+// each, every lane jumping to its first (kGaBaseStates states past the last
+// possible draw are also covered: the next generation's base states).  0
+// when a population needs more than the jump table or the index buffer holds.
+constexpr int kGaBaseStates = 256;
+constexpr int kGaMaxIndexDraws = 8192;  // tournament and shuffle draws held as indices in LDS
+__host__ __device__ inline int ga_draw_span(int n, int tsize) {
+  if (n < 1 || tsize < 1) return 0;
+  const long long idx = static_cast<long long>(n) * tsize + (n - 1);
+  const long long worst = idx + 6LL * (n / 2) + 18LL * (n & ~1);
+  const long long per = (worst + kGaBaseStates + kGaThreads - 1) / kGaThreads;
+  const long long span = (per + stream::kJumpStride - 1) / stream::kJumpStride * stream::kJumpStride;
+  if (idx > kGaMaxIndexDraws || (span / stream::kJumpStride) * (kGaThreads - 1) >= stream::kJumpEntries) return 0;
+  return static_cast<int>(span);
+}
+
 // Device bytes and layout of one job's GA buffers (appended to L).
-size_t ga_job_bytes(int max_pop, int max_gen, bool trace);
-void ga_job_layout(Layout* L, int max_pop, int max_gen, bool trace, GaJobOffsets* o);
+size_t ga_job_bytes(int max_pop, const gdock_ga_params& p);
+void ga_job_layout(Layout* L, int max_pop, const gdock_ga_params& p, GaJobOffsets* o);
+// The stream's jump table on the current device (once per device; call it
+// before the first GA launch, outside graph capture).
+hipError_t ga_device_init(hipStream_t s);
 DevGaJob ga_job_bind(char* base, const GaJobOffsets& o, const gdock_ga_params& p, double* d_poses,
                      const double* d_out);
 // The state after the host drew the initial population (job.genes()).

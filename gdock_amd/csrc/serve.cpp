@@ -410,7 +410,7 @@ void gdock_server::prepare_loop() {
       const int pop = j->result->job.population();
       Layout GL;
       GaJobOffsets go;
-      ga_job_layout(&GL, pop, j->params.max_generations, j->params.keep_trace != 0, &go);
+      ga_job_layout(&GL, pop, j->params, &go);
       ga_bytes = GL.size;
       he = hipMalloc(&j->ga_mem, ga_bytes);
       if (he == hipSuccess) {
@@ -553,6 +553,7 @@ gdock_status gdock_server_create(int device, const gdock_server_config* config, 
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_ga), c.max_jobs * sizeof(DevGaJob));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_done), c.max_jobs * sizeof(int32_t));
   if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (e == hipSuccess && c.ga_on_device) e = ga_device_init(s->ctx->stream);
   if (e == hipSuccess) e = hipEventCreate(&s->ev0);
   if (e == hipSuccess) e = hipEventCreate(&s->ev1);
   if (e == hipSuccess) e = hipEventCreate(&s->ev2);

@@ -176,5 +176,94 @@ struct Xoshiro256 {
   }
 };
 
+// Jump-ahead.  xoshiro256's state transition T is linear over GF(2) (shifts,
+// a rotation and XORs of the four words), so the state after j steps is
+// q(T) s with q = x^j mod P, P the characteristic polynomial of T (degree
+// 256): the XOR of the states T^i s over the set bits i of q.  P comes from
+// Berlekamp-Massey on the sequence of one state bit, and is checked by
+// P(T) s = 0; jump_polys() tabulates q for j = kJumpStride k.
+constexpr int kJumpStride = 16;
+constexpr int kJumpEntries = 2048;  // jumps of up to 16 x 2047 steps
+
+// XOR of base[i] (the states T^i s, i < 256) over the set bits of q.
+__host__ __device__ inline void jump_combine(const uint64_t (*base)[4], const uint64_t* q, uint64_t (&out)[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0;
+  for (int w = 0; w < 4; ++w) {
+    uint64_t bits = q[w];
+    while (bits) {
+      const int i = 64 * w + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      out[0] ^= base[i][0];
+      out[1] ^= base[i][1];
+      out[2] ^= base[i][2];
+      out[3] ^= base[i][3];
+    }
+  }
+}
+
+// P without its x^256 term (bit i: coefficient of x^i); false unless the
+// sequence's linear complexity is 256 and P(T) annihilates the state.
+inline bool char_poly(uint64_t (&p)[4]) {
+  constexpr int N = 512, L_MAX = 256;
+  Xoshiro256 r(0x5eed5eed5eedull);
+  const Xoshiro256 s0 = r;
+  uint8_t seq[N];
+  for (int i = 0; i < N; ++i) {  // bit 0 of word 0 along the orbit
+    seq[i] = static_cast<uint8_t>(r.s[0] & 1u);
+    r.next();
+  }
+  uint8_t c[N + 1] = {1}, b[N + 1] = {1}, t[N + 1];  // Berlekamp-Massey over GF(2)
+  int L = 0, m = 1;
+  for (int n = 0; n < N; ++n) {
+    uint8_t d = seq[n];
+    for (int i = 1; i <= L; ++i) d ^= static_cast<uint8_t>(c[i] & seq[n - i]);
+    if (!d) {
+      ++m;
+      continue;
+    }
+    memcpy(t, c, sizeof t);
+    for (int i = 0; i + m <= N; ++i) c[i + m] ^= b[i];
+    if (2 * L <= n) {
+      L = n + 1 - L;
+      memcpy(b, t, sizeof b);
+      m = 1;
+    } else {
+      ++m;
+    }
+  }
+  if (L != L_MAX) return false;
+  p[0] = p[1] = p[2] = p[3] = 0;
+  for (int k = 0; k < L_MAX; ++k)  // P_k = c_(L-k)
+    if (c[L_MAX - k]) p[k / 64] |= 1ull << (k % 64);
+  Xoshiro256 w = s0;  // P(T) s = sum of P_k T^k s, with P_256 = 1
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k <= L_MAX; ++k) {
+    if (k == L_MAX || ((p[k / 64] >> (k % 64)) & 1u))
+      for (int q = 0; q < 4; ++q) acc[q] ^= w.s[q];
+    w.next();
+  }
+  return (acc[0] | acc[1] | acc[2] | acc[3]) == 0;
+}
+
+// table[4 k .. 4 k + 3] = x^(kJumpStride k) mod P, for k < kJumpEntries.
+inline bool jump_polys(uint64_t* table) {
+  uint64_t p[4];
+  if (!char_poly(p)) return false;
+  uint64_t q[4] = {1, 0, 0, 0};
+  for (int k = 0; k < kJumpEntries; ++k) {
+    memcpy(table + 4 * k, q, sizeof q);
+    for (int s = 0; s < kJumpStride; ++s) {  // q <- x q mod P
+      const bool top = (q[3] >> 63) != 0;
+      q[3] = (q[3] << 1) | (q[2] >> 63);
+      q[2] = (q[2] << 1) | (q[1] >> 63);
+      q[1] = (q[1] << 1) | (q[0] >> 63);
+      q[0] <<= 1;
+      if (top)
+        for (int w = 0; w < 4; ++w) q[w] ^= p[w];
+    }
+  }
+  return true;
+}
+
 }  // namespace stream
 }  // namespace gdock

@@ -3,10 +3,10 @@
 // the sanitizers on the host side only and runs this program).  It touches
 // no GPU: PDB parsing (every fixture, plus damaged and edge-case text),
 // record formatting, ligand positioning, the host pose transform, pose
-// records over the host thread pool, the project stream, and the GA's host
-// logic (tournaments, crossover, mutation, the Hall of Fame past its
-// capacity, early stop, the final breed, model selection) over whole runs
-// with synthetic fitness.  Any sanitizer report aborts the program; it prints
+// records over the host thread pool, the project stream and its jump table,
+// and the GA's host logic (tournaments, crossover, mutation, the Hall of Fame
+// past its capacity, early stop, the final breed, model selection) over whole
+// runs with synthetic fitness.  Any sanitizer report aborts the program; it prints
 // "host driver ok" at the end.
 #include <cmath>
 #include <cstdint>
@@ -152,6 +152,34 @@ static void ga_runs() {
   CHECK(overflowed >= 1);
 }
 
+// The stream's jump table (x^(16 k) mod P, P the characteristic polynomial
+// of its transition, found by Berlekamp-Massey): the state it gives for 16 k
+// outputs ahead continues the stream exactly as stepping one output at a time.
+static void jump_table() {
+  using namespace gdock::stream;
+  std::vector<uint64_t> tab(4 * static_cast<size_t>(kJumpEntries));
+  CHECK(jump_polys(tab.data()));
+  for (uint64_t seed : {1ull, 42ull, 0xfeedull}) {
+    const Xoshiro256 r(seed);
+    Xoshiro256 w = r;
+    uint64_t base[256][4];
+    for (int i = 0; i < 256; ++i) {
+      for (int k = 0; k < 4; ++k) base[i][k] = w.s[k];
+      w.next();
+    }
+    std::vector<uint64_t> seq(16 * static_cast<size_t>(kJumpEntries) + 16);
+    w = r;
+    for (uint64_t& v : seq) v = w.next();
+    for (int k : {0, 1, 2, 15, 16, 17, 255, 256, 1000, 1785, 2047}) {
+      uint64_t st[4];
+      jump_combine(base, &tab[4 * static_cast<size_t>(k)], st);
+      Xoshiro256 j;
+      for (int q = 0; q < 4; ++q) j.s[q] = st[q];
+      for (int t = 0; t < 16; ++t) CHECK(j.next() == seq[16 * static_cast<size_t>(k) + t]);
+    }
+  }
+}
+
 int main(int argc, char** argv) {
   std::vector<gdock_atom> rec, lig;
   for (int i = 1; i < argc; ++i) {  // the fixtures: receptor first, then the ligand
@@ -190,6 +218,7 @@ int main(int argc, char** argv) {
   }
 
   ga_runs();
+  jump_table();
   std::printf("host driver ok (%zu receptor and %zu ligand atoms)\n", rec.size(), lig.size());
   return 0;
 }

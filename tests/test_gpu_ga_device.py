@@ -67,3 +67,23 @@ def test_device_ga_hall_of_fame_overflow():
     _identical(host, dev)
     assert len(dev.hof_fitness) == 500
     assert np.all(np.diff(dev.hof_fitness) >= 0)
+
+
+def test_device_ga_one_lane_draws_equal_host_driver():
+    """Tournaments of 140 draw more indices (60 x 140 + 59) than the lanes'
+    index buffer holds: one lane takes the generation's draws in order."""
+    c = synth.make_complex(5, rec_res=30, lig_res=25, n_restraints=4)
+    rec = structure.read_pdb_text(c.receptor_pdb)[0]
+    lig = structure.position_ligand(rec, structure.read_pdb_text(c.ligand_pdb)[0])
+    cx = Complex(rec, lig, c.restraints)
+    kw = dict(population_size=60, max_generations=12, early_stopping=0, keep_trace=1, tournament_size=140)
+    _identical(ga.run(cx, seed=5, **kw), ga.run_device(cx, seed=5, **kw))
+
+
+@pytest.mark.parametrize("pop", [1024, 3])
+def test_device_ga_population_extremes(pop):
+    """The largest device population (each lane takes 112 consecutive draws)
+    and a population of 3 (one crossed pair, one individual left over)."""
+    rec, lig, pairs, cx = _cfg0()
+    kw = dict(population_size=pop, max_generations=6, early_stopping=0, keep_trace=1, elitism=min(5, pop - 1))
+    _identical(ga.run(cx, seed=9, **kw), ga.run_device(cx, seed=9, **kw))
