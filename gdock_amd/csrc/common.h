@@ -30,6 +30,10 @@ constexpr int kOutDoubles = 6;    // vdw, elec, desolv, air, restraint_penalty, 
 constexpr int kScoreThreads = 256;  // 4 wave64 per pose workgroup
 constexpr int kMaxAtomsPerLane = 8;
 constexpr int kMaxPoseParts = 15;  // workgroups one pose can be split over (a 4-bit field of a work item)
+// The GA drivers' plain launches hold at least this many workgroups (two per
+// CU of a 256-CU device), splitting a small population's poses over receptor
+// ranges; a fixed count, so a run gives the same bits on any device.
+constexpr int kGaFillItems = 512;
 // Receptor slots one workgroup covers: the whole receptor in the plain
 // one-workgroup-per-pose path up to kMaxLdsReceptor slots (larger receptors
 // are split over receptor ranges), at most kPartSlots slots per workgroup in

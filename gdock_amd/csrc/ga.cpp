@@ -276,14 +276,16 @@ gdock_status gdock_ga_default_params(gdock_ga_params* p) {
 namespace {
 // Scores one generation of the host driver: genes up, pose records made on
 // the device (as the device GA makes them, so both drivers score the same
-// records), the plain scoring launch, results down.
+// records), the plain scoring launch filled to kGaFillItems workgroups (as
+// the device GA launches it), results down.
 gdock_status score_generation(gdock_ctx* ctx, gdock_complex* cx, const double* genes, int32_t n,
                               const gdock_weights* w, double* out) {
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "ga: set device");
   Layout L;
   const size_t o_g = L.add<double>(static_cast<size_t>(n) * 6), o_p = L.add<double>(static_cast<size_t>(n) * kPoseDoubles),
-               o_o = L.add<double>(static_cast<size_t>(n) * kOutDoubles), o_s = L.add<uint8_t>(plain_scratch_bytes(cx->dev, n));
+               o_o = L.add<double>(static_cast<size_t>(n) * kOutDoubles),
+               o_s = L.add<uint8_t>(plain_scratch_bytes(cx->dev, n, kGaFillItems));
   if (!ctx->staging.reserve(L.size, &e)) return hip_fail(ctx, e, "ga: staging");
   char* m = static_cast<char*>(ctx->staging.ptr);
   const Weights ww{w->vdw, w->elec, w->desolv, w->air};
@@ -292,7 +294,7 @@ gdock_status score_generation(gdock_ctx* ctx, gdock_complex* cx, const double* g
     e = launch_ga_pose_records(reinterpret_cast<double*>(m + o_g), n, reinterpret_cast<double*>(m + o_p), ctx->stream);
   if (e == hipSuccess)
     e = launch_score(cx->dev, cx->d_dev, reinterpret_cast<double*>(m + o_p), nullptr, n, ww,
-                     reinterpret_cast<double*>(m + o_o), nullptr, m + o_s, ctx->stream);
+                     reinterpret_cast<double*>(m + o_o), nullptr, m + o_s, ctx->stream, 0, nullptr, kGaFillItems);
   if (e == hipSuccess)
     e = hipMemcpyAsync(out, m + o_o, static_cast<size_t>(n) * kOutDoubles * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -845,7 +845,8 @@ gdock_status gdock_ga_run_device(gdock_ctx* ctx, gdock_complex* cx, const gdock_
   ga_job_layout(&L, pop, p, &o);
   const size_t o_job = L.add<DevGaJob>(1), o_poses = L.add<double>(static_cast<size_t>(pop) * kPoseDoubles),
                o_out = L.add<double>(static_cast<size_t>(pop) * kOutDoubles),
-               o_scr = L.add<uint8_t>(plain_scratch_bytes(cx->dev, pop));
+               o_scr = L.add<uint8_t>(std::max(plain_scratch_bytes(cx->dev, pop, kGaFillItems),
+                                               plain_scratch_bytes(cx->dev, pop & ~1, kGaFillItems)));
   char* m = nullptr;
   e = hipMalloc(reinterpret_cast<void**>(&m), L.size);
   if (e != hipSuccess) { delete r; return hip_fail(ctx, e, "ga_run_device: memory"); }
@@ -858,17 +859,17 @@ gdock_status gdock_ga_run_device(gdock_ctx* ctx, gdock_complex* cx, const gdock_
                        ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m + o_job, &J, sizeof J, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = launch_ga_pose_records(J.genes[0], pop, J.poses, ctx->stream);
-  // One plain scoring launch and one GA launch per generation, kGaAhead
-  // generations between looks at the done flag (the first alone: an odd
-  // population breeds one child fewer).  Launches past the end change
-  // nothing the result reads: the GA launch returns at once and the scoring
-  // launch rescores the last pose records.
+  // One plain scoring launch (filled as the host driver fills it) and one GA
+  // launch per generation, kGaAhead generations between looks at the done
+  // flag (the first alone: an odd population breeds one child fewer).
+  // Launches past the end change nothing the result reads: the GA launch
+  // returns at once and the scoring launch rescores the last pose records.
   constexpr int kGaAhead = 4;
   int n = pop, ahead = 1;
   while (e == hipSuccess) {
     for (int g = 0; g < ahead && e == hipSuccess; ++g) {
       e = launch_score(cx->dev, cx->d_dev, J.poses, nullptr, n, ww, const_cast<double*>(J.out), nullptr, m + o_scr,
-                       ctx->stream);
+                       ctx->stream, 0, nullptr, kGaFillItems);
       if (e == hipSuccess) e = launch_ga_generation(reinterpret_cast<const DevGaJob*>(m + o_job), 1, ctx->stream);
     }
     ahead = kGaAhead;

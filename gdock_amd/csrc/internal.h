@@ -118,14 +118,24 @@ void read_pdb_text(std::string_view text, std::vector<gdock_atom>* atoms, std::v
 // ceil(n_lig / 256) capped at 8.  counts (diagnostic): when set, only the
 // counting instantiation of the fast kernel runs and adds {pair steps x 64,
 // pairs within 15 A} to counts[0..1] (the outputs are then not valid).
+// min_items > 0 splits the poses of a small launch over receptor ranges until
+// the launch holds that many workgroups: a pose's sums then associate by
+// range, so its last bits depend on the launch's pose count (the GA drivers,
+// whose launches are one population each, pass kGaFillItems; the scoring
+// entry points pass 0, and a pose scores the same alone and in any launch).
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
-                        hipStream_t stream, int lanes_k = 0, unsigned long long* counts = nullptr);
+                        hipStream_t stream, int lanes_k = 0, unsigned long long* counts = nullptr,
+                        int min_items = 0);
 int score_atoms_per_lane(int n_lig);
 // Receptor-range parts of the plain path (1 unless the receptor exceeds one
-// workgroup's LDS table) and its scratch bytes (flags, split state).
-int plain_parts(const DevComplex& cx);
-size_t plain_scratch_bytes(const DevComplex& cx, int n_poses);
+// workgroup's LDS table; see launch_score for min_items) and its scratch
+// bytes (flags, split state).
+int plain_parts(const DevComplex& cx, int n_poses = 0, int min_items = 0);
+// Equal whole-tile receptor ranges (none empty, at most kMaxPoseParts) that
+// bring n_poses poses to min_items workgroups; 1 when min_items is 0.
+int fill_parts(const DevComplex& cx, int n_poses, int min_items);
+size_t plain_scratch_bytes(const DevComplex& cx, int n_poses, int min_items = 0);
 // Most receptor slots one of `parts` equal tile ranges covers.
 int part_slots(int n_rec, int parts);
 size_t score_lds_bytes(int slots);

@@ -976,28 +976,40 @@ int part_slots(int n_rec, int parts) {
 }
 
 // Parts of the plain path: one workgroup per pose while the receptor fits one
-// workgroup's LDS table, else the fewest equal receptor ranges that fit.
-int plain_parts(const DevComplex& cx) {
+// workgroup's LDS table, else the fewest equal receptor ranges that fit; with
+// min_items > 0, at least enough ranges (whole tiles, each part non-empty)
+// that the launch holds min_items workgroups.
+int fill_parts(const DevComplex& cx, int n_poses, int min_items) {
+  if (min_items <= 0 || n_poses <= 0) return 1;
+  const int tiles = (cx.n_rec + 63) / 64;
+  int want = std::max(1, std::min({(min_items + n_poses - 1) / n_poses, tiles, kMaxPoseParts}));
+  while (want > 1 && (want - 1) * ((tiles + want - 1) / want) >= tiles) --want;  // no empty part
+  return want;
+}
+
+int plain_parts(const DevComplex& cx, int n_poses, int min_items) {
   const int tiles = (cx.n_rec + 63) / 64, per = kMaxLdsReceptor / 64;
-  return std::max(1, (tiles + per - 1) / per);
+  return std::max({1, (tiles + per - 1) / per, fill_parts(cx, n_poses, min_items)});
 }
 
 // Plain-path scratch: the per-pose rescore flags, the launch order of the
 // poses and their cost buckets, then the split state (each piece 256-byte
 // aligned).
 struct PlainScratch { size_t order, keys, split, total; };
-static PlainScratch plain_scratch_layout(const DevComplex& cx, int n_poses) {
+static PlainScratch plain_scratch_layout(const DevComplex& cx, int n_poses, int min_items) {
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t n = static_cast<size_t>(std::max(n_poses, 1));
   PlainScratch L;
   L.order = al(n * sizeof(int32_t));
   L.keys = L.order + al(n * sizeof(int32_t));
   L.split = L.keys + al(n);
-  L.total = L.split + split_state_bytes(cx, plain_parts(cx), n_poses);
+  L.total = L.split + split_state_bytes(cx, plain_parts(cx, n_poses, min_items), n_poses);
   return L;
 }
 
-size_t plain_scratch_bytes(const DevComplex& cx, int n_poses) { return plain_scratch_layout(cx, n_poses).total; }
+size_t plain_scratch_bytes(const DevComplex& cx, int n_poses, int min_items) {
+  return plain_scratch_layout(cx, n_poses, min_items).total;
+}
 
 // Estimated work of scoring a pose (pose record p) with k ligand atoms per
 // lane, by one whole wave: the receptor slots within the ligand's radius +
@@ -1142,11 +1154,11 @@ static void launch_pair(const DevComplex* d_cx, const double* d_poses, const dou
 // larger than one workgroup's LDS table, cleared here).
 hipError_t launch_score(const DevComplex& cx, const DevComplex* d_cx, const double* d_poses, const double* d_coords,
                         int n_poses, const Weights& w, double* d_out, int32_t* d_nsat, void* scratch,
-                        hipStream_t stream, int lanes_k, unsigned long long* counts) {
+                        hipStream_t stream, int lanes_k, unsigned long long* counts, int min_items) {
   if (n_poses <= 0) return hipSuccess;
-  const int parts = plain_parts(cx);
+  const int parts = plain_parts(cx, n_poses, min_items);
   const size_t lds = score_lds_bytes(part_slots(cx.n_rec, parts));
-  const PlainScratch SL = plain_scratch_layout(cx, n_poses);
+  const PlainScratch SL = plain_scratch_layout(cx, n_poses, min_items);
   char* m = static_cast<char*>(scratch);
   int32_t* d_redo = reinterpret_cast<int32_t*>(m);
   DevComplex split{};

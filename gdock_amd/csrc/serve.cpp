@@ -391,8 +391,13 @@ void gdock_server::prepare_loop() {
     hipError_t he = hipSuccess;
     if (st == GDOCK_OK) j->lanes = choose_lanes(j->cx->dev);
     size_t split_bytes = 0;
+    // A job's poses take at least the parts that would fill the device with
+    // its population alone (as the GA drivers fill their plain launches): a
+    // function of the job, so its results stay the same in any mix.
+    int fill = 1;
     if (st == GDOCK_OK) {
-      j->parts = pose_parts(j->cx->dev, j->lanes);
+      fill = fill_parts(j->cx->dev, j->params.population_size, kGaFillItems);
+      j->parts = std::max(pose_parts(j->cx->dev, j->lanes), fill);
       split_bytes = split_state_bytes(j->cx->dev, j->parts, max_pop);
       if (split_bytes) {
         he = hipMalloc(&j->split_mem, split_bytes);
@@ -436,7 +441,7 @@ void gdock_server::prepare_loop() {
     if (st == GDOCK_OK) {
       j->dev = j->cx->dev;
       j->dev.lanes_k = j->lanes;
-      j->dev.min_parts = min_batch_parts(j->dev);
+      j->dev.min_parts = std::max(min_batch_parts(j->dev), fill);
       bind_split_state(&j->dev, j->parts, max_pop, j->slot * max_pop, static_cast<char*>(j->split_mem));
     }
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_prep).count();

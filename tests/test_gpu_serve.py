@@ -147,13 +147,14 @@ def test_job_failing_in_preparation_leaves_the_server_working():
 def test_destroy_ends_pending_waits():
     """Destroying the server while another thread waits on a job that has not
     ended: the job ends as failed, the wait returns NOT_FOUND, and destroy
-    returns once the waiter has left.  One job slot and three long jobs: the
-    third is still queued when destroy begins."""
+    returns once the waiter has left.  One job slot and three jobs of 5000
+    generations each (seconds apiece): the third is still queued when destroy
+    begins."""
     import threading
     import time
     jobs = [_job(95 + i, 120, 90) for i in range(3)]
     srv = Server(max_jobs=1)
-    ids = [srv.submit(*j, seed=1 + k, max_generations=250, early_stopping=0) for k, j in enumerate(jobs)]
+    ids = [srv.submit(*j, seed=1 + k, max_generations=5000, early_stopping=0) for k, j in enumerate(jobs)]
     got = {}
 
     def waiter():
