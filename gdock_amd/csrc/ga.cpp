@@ -36,16 +36,62 @@ double angular_difference(double a, double b) {  // run.rs:160-169
 }
 
 bool by_fitness(const Individual& a, const Individual& b) { return a.fitness < b.fitness; }
+
+// run.rs:129-158: a pose within 0.2 rad per angle and 2 A per shift of a kept one.
+bool similar(const double* g, const HofEntry& e) {
+  bool sim = true;
+  for (int i = 0; i < 3 && sim; ++i) sim = angular_difference(g[i], e.genes[i]) <= kHofRotation;
+  for (int i = 3; i < 6 && sim; ++i) sim = std::fabs(g[i] - e.genes[i]) <= kHofTranslation;
+  return sim;
+}
+
+// Entries by their first angle, in bins of 0.2 rad over [0, 2 pi] (the last
+// one shorter): an entry within 0.2 rad of a pose's first angle, wrapping at
+// 2 pi, lies at most two bins from the pose's, so "similar to some entry" is
+// decided on five bins.  Angles outside [0, 2 pi] (the GA never makes them)
+// are kept aside and always tested.
+constexpr int kHofBins = 32;
+int hof_bin(double a) {
+  if (!(a >= 0.0 && a <= kTwoPi)) return -1;
+  return std::min(static_cast<int>(a / kHofRotation), kHofBins - 1);
+}
+
+struct HofIndex {
+  std::vector<int> bin[kHofBins], aside;
+  void add(const HofEntry& e, int i) {
+    const int b = hof_bin(e.genes[0]);
+    (b < 0 ? aside : bin[b]).push_back(i);
+  }
+  void build(const std::vector<HofEntry>& entries) {
+    for (auto& v : bin) v.clear();
+    aside.clear();
+    for (size_t i = 0; i < entries.size(); ++i) add(entries[i], static_cast<int>(i));
+  }
+  bool any_similar(const double* g, const std::vector<HofEntry>& entries) const {
+    const int b = hof_bin(g[0]);
+    if (b < 0) {
+      for (const HofEntry& e : entries)
+        if (similar(g, e)) return true;
+      return false;
+    }
+    for (int d = -2; d <= 2; ++d)
+      for (int i : bin[(b + d + kHofBins) % kHofBins])
+        if (similar(g, entries[i])) return true;
+    for (int i : aside)
+      if (similar(g, entries[i])) return true;
+    return false;
+  }
+};
 }  // namespace
 
 // ---- Hall of Fame -------------------------------------------------------------
 bool HallOfFame::offer(const double* g, double fitness, uint32_t generation) {
-  for (const HofEntry& e : entries_) {  // run.rs:129-158: similar to a kept pose?
-    bool similar = true;
-    for (int i = 0; i < 3 && similar; ++i) similar = angular_difference(g[i], e.genes[i]) <= kHofRotation;
-    for (int i = 3; i < 6 && similar; ++i) similar = std::fabs(g[i] - e.genes[i]) <= kHofTranslation;
-    if (similar) return false;
-  }
+  for (const HofEntry& e : entries_)  // run.rs:129-158: similar to a kept pose?
+    if (similar(g, e)) return false;
+  return keep(g, fitness, generation);
+}
+
+bool HallOfFame::keep(const double* g, double fitness, uint32_t generation) {
   HofEntry e;
   std::memcpy(e.genes, g, sizeof e.genes);
   e.fitness = fitness;
@@ -63,10 +109,17 @@ void HallOfFame::offer_population(const std::vector<Individual>& pop, uint32_t g
   std::vector<size_t> order(pop.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return pop[a].fitness < pop[b].fitness; });
+  HofIndex index;
+  index.build(entries_);
   int added = 0;
   for (size_t i : order) {
     if (added >= kHofPerGeneration) break;
-    added += offer(pop[i].genes, pop[i].fitness, generation) ? 1 : 0;
+    if (index.any_similar(pop[i].genes, entries_)) continue;
+    const size_t before = entries_.size();
+    keep(pop[i].genes, pop[i].fitness, generation);
+    if (entries_.size() > before) index.add(entries_.back(), static_cast<int>(before));
+    else index.build(entries_);  // trimmed: sorted anew
+    ++added;
   }
 }
 

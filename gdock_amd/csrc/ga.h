@@ -29,6 +29,8 @@ class HallOfFame {
  public:
   bool offer(const double* genes, double fitness, uint32_t generation);
   void offer_population(const std::vector<Individual>& pop, uint32_t generation);
+  // Appends the pose (then trims to the capacity), without the similarity test.
+  bool keep(const double* genes, double fitness, uint32_t generation);
   const std::vector<HofEntry>& entries() const { return entries_; }
   void assign(std::vector<HofEntry> e) { entries_ = std::move(e); }
 
