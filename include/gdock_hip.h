@@ -221,7 +221,11 @@ typedef struct gdock_ga_params {
 } gdock_ga_params;
 typedef struct gdock_ga_result gdock_ga_result;
 gdock_status gdock_ga_default_params(gdock_ga_params* p);
-/* One job, one plain scoring launch per generation (the drop-in path). */
+/* One job, one plain scoring launch per generation (the drop-in path).  A
+ * population under 512 poses is split over receptor ranges so that each
+ * launch holds 512 workgroups: its energies then sum in another order than
+ * gdock_score_poses's for the same pose (the same tolerance; a run gives the
+ * same bits every time, and gdock_ga_run_device splits alike). */
 gdock_status gdock_ga_run(gdock_ctx* ctx, gdock_complex* cx, const gdock_weights* w, uint64_t seed,
                           const gdock_ga_params* params, gdock_ga_result** out);
 /* The same job with every generation after the first drawn on the device:
