@@ -533,19 +533,35 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
 #pragma unroll
       for (int k = 0; k < K; ++k) mn[k] = INFINITY;
       const int j0 = ct.rec_res_start[ri], j1 = ct.rec_res_start[ri + 1];
-      for (int j = j0; j < j1; ++j) {
-        const float4 r = ct.rec_c4[j];
+      auto expanded = [&](const float4& r, int k) {
+        return __builtin_fmaf(r.x, lx[k], __builtin_fmaf(r.y, ly[k], __builtin_fmaf(r.z, lz[k], r.w + ll[k])));
+      };
+      int j = j0;
+      // four atoms per step: their loads issue together and each block's
+      // mask is tested once (the minimum of exact values in any order)
+      for (; j + 4 <= j1; j += 4) {
+        const float4 r0 = ct.rec_c4[j], r1 = ct.rec_c4[j + 1], r2 = ct.rec_c4[j + 2], r3 = ct.rec_c4[j + 3];
 #pragma unroll
         for (int k = 0; k < K; ++k)
           if ((mk >> k) & 1u)
-            mn[k] = fminf(mn[k], __builtin_fmaf(r.x, lx[k], __builtin_fmaf(r.y, ly[k], __builtin_fmaf(r.z, lz[k], r.w + ll[k]))));
+            mn[k] = fminf(fminf(mn[k], fminf(expanded(r0, k), expanded(r1, k))), fminf(expanded(r2, k), expanded(r3, k)));
       }
-      bool hit[K];
+      for (; j < j1; ++j) {
+        const float4 r = ct.rec_c4[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if ((mk >> k) & 1u) mn[k] = fminf(mn[k], expanded(r, k));
+      }
+      bool hit[K], any_amb = false;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         hit[k] = mn[k] <= lo;
-        const bool amb = !hit[k] && mn[k] < hi;
-        if (__any(amb)) {  // rare: decide this residue in FP64 (clustering.rs:98-99)
+        any_amb = any_amb || (!hit[k] && mn[k] < hi);
+      }
+      if (__any(any_amb)) {  // rare: decide this residue in FP64 (clustering.rs:98-99)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const bool amb = !hit[k] && mn[k] < hi;
           const int h = contact_slot(base, k, wave, lane);
           if (amb && h < ct.n_lig_heavy) {
             double px, py, pz;
@@ -557,13 +573,18 @@ __global__ __launch_bounds__(256) void contacts_kernel(const DevComplex cx, cons
           }
         }
       }
+      bool any_hit = false;
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (hit[k]) {
-          const int32_t at = ri * ct.n_lig_res + lres[k];
-          const int32_t bit = ct.pair_bit ? ct.pair_bit[at] : at;
-          if (bit >= 0) atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
-        }
+      for (int k = 0; k < K; ++k) any_hit = any_hit || hit[k];
+      if (__any(any_hit)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (hit[k]) {
+            const int32_t at = ri * ct.n_lig_res + lres[k];
+            const int32_t bit = ct.pair_bit ? ct.pair_bit[at] : at;
+            if (bit >= 0) atomicOr(&s_bits[bit >> 5], 1u << (bit & 31));
+          }
+      }
     }
   }
   __syncthreads();
