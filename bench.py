@@ -375,7 +375,7 @@ def measure_serve(n_jobs, rate, ga_on_device=False, plain=True):
         arrivals = np.zeros(len(jobs))
     ctx = default_context(0)
 
-    srv = Server(max_jobs=64, max_receptor_atoms=8192, ga_on_device=ga_on_device)
+    srv = Server(max_jobs=16, max_receptor_atoms=8192, ga_on_device=ga_on_device)
     warm = srv.submit(*jobs[0], seed=1, max_generations=2, cluster=0)
     srv.wait(warm, 300.0)
     t0 = time.perf_counter()

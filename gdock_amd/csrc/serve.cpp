@@ -507,7 +507,10 @@ extern "C" {
 gdock_status gdock_server_default_config(gdock_server_config* c) {
   if (!c) return GDOCK_ERR_INVALID_ARG;
   *c = gdock_server_config{};
-  c->max_jobs = 64;
+  // A job's launches already fill the device (kGaFillItems), so more active
+  // jobs only share the steps: 160 jobs at once ran at 8,241 jobs/min with
+  // 16 slots and 7,615 with 64 (lower median latency too).
+  c->max_jobs = 16;
   c->max_population = 150;
   c->max_receptor_atoms = 8192;
   c->ga_on_device = 0;

@@ -27,7 +27,7 @@ class JobTimes:
 
 
 class Server:
-    def __init__(self, device: int = 0, max_jobs: int = 64, max_population: int = 150,
+    def __init__(self, device: int = 0, max_jobs: int = 16, max_population: int = 150,
                  max_receptor_atoms: int = 8192, memory_budget: int = 0,
                  weights: constants.EnergyWeights | None = None, ga_on_device: bool = False):
         c = ServerConfig()

@@ -329,7 +329,7 @@ gdock_status gdock_calc_metrics(gdock_ctx* ctx, gdock_evaluator* ev, const doubl
  * finished job's Hall of Fame.  A job's result is the same whichever jobs
  * share its launches.  The server owns its own context on `device`. */
 typedef struct gdock_server_config {
-  int32_t max_jobs;            /* active job slots (64) */
+  int32_t max_jobs;            /* active job slots (16) */
   int32_t max_population;      /* largest population a job may use (150; at most 65536, 1024 with
                                   ga_on_device; max_jobs x max_population x 8 at most 2^30) */
   int32_t max_receptor_atoms;  /* sizes the kernel's LDS receptor table (8192) */
