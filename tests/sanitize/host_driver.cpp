@@ -8,6 +8,7 @@
 // past its capacity, early stop, the final breed, model selection) over whole
 // runs with synthetic fitness.  Any sanitizer report aborts the program; it prints
 // "host driver ok" at the end.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -152,6 +153,52 @@ static void ga_runs() {
   CHECK(overflowed >= 1);
 }
 
+// Hall of Fame offers decided on first-angle bins equal offers tested
+// against every entry: populations crowded at the 0 / 2 pi wrap and at the
+// bin edges (multiples of 0.2 rad), offered to a Hall of Fame grown past its
+// capacity, against the same offers made one by one through offer().
+static void hof_bins() {
+  int kept_total = 0;
+  for (int trial = 0; trial < 40; ++trial) {
+    gdock::HallOfFame binned, plain;
+    for (int gen = 0; gen < 80; ++gen) {
+      std::vector<gdock::Individual> pop(150);
+      gdock::Xoshiro256 rng(static_cast<uint64_t>(7919 * trial + gen));
+      for (auto& c : pop) {
+        const double u = rng.unit();
+        double a;
+        if (u < 0.35) a = 6.283185307179586 - 0.25 * rng.unit();   // just below 2 pi
+        else if (u < 0.7) a = 0.25 * rng.unit();                    // just above 0
+        else a = 0.2 * static_cast<double>(rng.below(32)) + (rng.unit() - 0.5) * 1e-9;  // bin edges
+        if (a < 0.0) a = 0.0;
+        c.genes[0] = a;
+        c.genes[1] = rng.unit() * 0.6;
+        c.genes[2] = rng.unit() * 0.6;
+        for (int k = 3; k < 6; ++k) c.genes[k] = (rng.unit() - 0.5) * 6.0;
+        c.fitness = rng.unit();
+      }
+      binned.offer_population(pop, static_cast<uint32_t>(gen));
+      std::vector<size_t> order(pop.size());
+      for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+      std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return pop[x].fitness < pop[y].fitness; });
+      int added = 0;
+      for (size_t i : order) {
+        if (added >= 10) break;
+        added += plain.offer(pop[i].genes, pop[i].fitness, static_cast<uint32_t>(gen)) ? 1 : 0;
+      }
+      const auto& a = binned.entries();
+      const auto& b = plain.entries();
+      CHECK(a.size() == b.size());
+      for (size_t i = 0; i < a.size(); ++i) {
+        CHECK(std::memcmp(a[i].genes, b[i].genes, sizeof a[i].genes) == 0);
+        CHECK(a[i].fitness == b[i].fitness && a[i].generation == b[i].generation);
+      }
+    }
+    kept_total += static_cast<int>(binned.entries().size());
+  }
+  CHECK(kept_total > 40 * 100);  // the offers did fill the Hall of Fame
+}
+
 // The stream's jump table (x^(16 k) mod P, P the characteristic polynomial
 // of its transition, found by Berlekamp-Massey): the state it gives for 16 k
 // outputs ahead continues the stream exactly as stepping one output at a time.
@@ -218,6 +265,7 @@ int main(int argc, char** argv) {
   }
 
   ga_runs();
+  hof_bins();
   jump_table();
   std::printf("host driver ok (%zu receptor and %zu ligand atoms)\n", rec.size(), lig.size());
   return 0;
