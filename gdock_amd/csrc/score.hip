@@ -221,6 +221,7 @@ struct LaneAtoms {
   int32_t x[K], y[K], z[K];
   float eps[K], ir2[K], acv[K], ace[K];
   int32_t cnt[K];
+  double kq[K];  // the atom's Coulomb factor (lig_kq), loaded once per chunk
 };
 
 // Soft-core LJ (fitness.rs:57-70) and the Coulomb factor from s, the squared
@@ -566,6 +567,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         a.x[k] = a.y[k] = a.z[k] = kFixedClamp;
       }
       a.eps[k] = 0.0f; a.ir2[k] = 0.0f; a.acv[k] = 0.0f; a.ace[k] = 0.0f; a.cnt[k] = 0;
+      a.kq[k] = cx.lig_kq[l];  // zero-padded past n_lig
     }
     float* my_box = s_box + wave * (kMaxAtomsPerLane * kBoxFloats);
     __syncthreads();  // the previous chunk's boxes are no longer read
@@ -660,8 +662,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
           // Coulomb: the group's FP32 partial (at most 4 terms) goes to FP64
           // at once; a longer FP32 partial carries a close pair's large term
           // and rounds every later term at its scale.
-          const int l = lig_index<K>(base, k);
-          sum_e = __builtin_fma(static_cast<double>(a.ace[k]), cx.lig_kq[l], sum_e);  // zero-padded past n_lig
+          sum_e = __builtin_fma(static_cast<double>(a.ace[k]), a.kq[k], sum_e);
           a.ace[k] = 0.0f;
         }
         const int t = lane - sh;
