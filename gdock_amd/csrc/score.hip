@@ -182,12 +182,18 @@ __device__ __forceinline__ bool box_beyond_reach(float4 lo, float4 hi, const flo
 // slot against every block; padding slots sit far from every block), and,
 // for a block some slot reaches, one of the block's quarters.  A culled pair
 // would only have added exact zeros, so the tests decide work, not results.
+// Lane t's receptor slot g0 + t of the tile [g0, g1) (slot g0 past its end).
+__device__ __forceinline__ int4 tile_slot(CxRef cx, int g0, int g1) {
+  const int lane = threadIdx.x & 63;
+  return cx.rec_fixed[g0 + lane < g1 ? g0 + lane : g0];
+}
+
+// r: the lane's slot of the tile, as tile_slot loads it.
 template <int K>
 __device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int g1, const float* __restrict__ boxes,
-                                               uint64_t (&keep)[K]) {
+                                               const int4 r, uint64_t (&keep)[K]) {
   const int lane = threadIdx.x & 63;
   const bool valid = g0 + lane < g1;
-  const int4 r = cx.rec_fixed[valid ? g0 + lane : g0];
   const float rx = static_cast<float>(r.x) * unit, ry = static_cast<float>(r.y) * unit,
               rz = static_cast<float>(r.z) * unit;
 #pragma unroll
@@ -612,16 +618,29 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
         for (int k = 0; k < K; ++k) near = near || !box_beyond_reach(lo, hi, my_box + kBoxFloats * k);
       }
       uint64_t tiles = __builtin_amdgcn_ballot_w64(near);
+      // a tile's first sub-tile; the next near tile's slots load while this
+      // one is tested and walked
+      auto tile_first = [](uint64_t t) { return (__builtin_ctzll(t) / (64 / kRecSub)) * (64 / kRecSub); };
+      int4 r_next = make_int4(0, 0, 0, 0);
+      if (tiles != 0ull) {
+        const int g = (s0 + tile_first(tiles)) * kRecSub;
+        r_next = tile_slot(cx, g, min(g + 64, pp.r1));
+      }
       while (tiles != 0ull) {
-      const int first = (__builtin_ctzll(tiles) / (64 / kRecSub)) * (64 / kRecSub);  // the tile's first sub-tile
+      const int first = tile_first(tiles);
       tiles &= ~(((1ull << (64 / kRecSub)) - 1ull) << first);
       const int g0 = (s0 + first) * kRecSub;
       const int g1 = min(g0 + 64, pp.r1);
+      const int4 r_tile = r_next;
+      if (tiles != 0ull) {
+        const int g = (s0 + tile_first(tiles)) * kRecSub;
+        r_next = tile_slot(cx, g, min(g + 64, pp.r1));
+      }
       // A receptor atom beyond every cutoff from all atoms of a ligand block
       // adds nothing and holds no band pair with it: that block skips the
       // atom (a group when no block is near, the tile when nothing is).
       uint64_t keep[K];
-      atoms_in_reach<K>(cx, unit, g0, g1, my_box, keep);
+      atoms_in_reach<K>(cx, unit, g0, g1, my_box, r_tile, keep);
       uint64_t any_keep = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) any_keep |= keep[k];
