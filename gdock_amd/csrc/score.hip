@@ -150,31 +150,35 @@ __device__ __forceinline__ int lig_index(int base, int k) {
          static_cast<int>(threadIdx.x & 63);
 }
 
-// Axis-aligned boxes of a wave's K ligand blocks (FP32 A, origin frame; empty
-// when a block holds no atom), kept in LDS per block as the block's box and
-// the boxes of its four kLigSub-lane quarters (8 floats each), and the test
-// that a receptor atom is farther from a box than every cutoff.
+// Axis-aligned boxes of a wave's K ligand blocks (FP32 A, origin frame),
+// kept in LDS per block as the block's box and the boxes of its four
+// kLigSub-lane quarters (8 floats each), and the test that a receptor atom is
+// farther from a box than every cutoff.  A box with no atom is the single
+// point (kFarBox, kFarBox, kFarBox): every distance to it overflows to +inf.
 constexpr int kBoxFloats = 8 * (1 + kLigBlock / kLigSub);
+constexpr float kFarBox = 1.0e30f;
 
 // The distance from a point to any point of a box is at least the
-// point-to-box distance; FP32 rounding of it (and of the fixed-point to A
+// point-to-box distance, whose axis components are the point minus its clamp
+// to the box (v_med3_f32; the same values, up to sign, as max(lo - x, x - hi,
+// 0)); ten VALU operations.  FP32 rounding of it (and of the fixed-point to A
 // conversions) is far below the 0.01 A margin of kCullReach over 15 A.
 __device__ __forceinline__ bool beyond_reach(float rx, float ry, float rz, const float* __restrict__ b) {
-  const float dx = fmaxf(fmaxf(b[0] - rx, rx - b[3]), 0.0f);
-  const float dy = fmaxf(fmaxf(b[1] - ry, ry - b[4]), 0.0f);
-  const float dz = fmaxf(fmaxf(b[2] - rz, rz - b[5]), 0.0f);
-  return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
+  const float dx = rx - __builtin_amdgcn_fmed3f(rx, b[0], b[3]);
+  const float dy = ry - __builtin_amdgcn_fmed3f(ry, b[1], b[4]);
+  const float dz = rz - __builtin_amdgcn_fmed3f(rz, b[2], b[5]);
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) > kCullReach * kCullReach;
 }
 
 // The same test for every point of the box [lo, hi]: for a point inside it
-// each difference below is at most the point's (FP32 subtraction is
-// monotone), so a box beyond reach implies every point in it is, as
-// beyond_reach would find it.
+// each difference below is at most the point's magnitude in beyond_reach
+// (FP32 subtraction, products and fma are monotone), so a box beyond reach
+// implies every point in it is, as beyond_reach would find it.
 __device__ __forceinline__ bool box_beyond_reach(float4 lo, float4 hi, const float* __restrict__ b) {
   const float dx = fmaxf(fmaxf(b[0] - hi.x, lo.x - b[3]), 0.0f);
   const float dy = fmaxf(fmaxf(b[1] - hi.y, lo.y - b[4]), 0.0f);
   const float dz = fmaxf(fmaxf(b[2] - hi.z, lo.z - b[5]), 0.0f);
-  return dx * dx + dy * dy + dz * dz > kCullReach * kCullReach;
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) > kCullReach * kCullReach;
 }
 
 // Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
@@ -586,12 +590,12 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
               y1 = quarter_max(in ? a.y[k] : INT_MIN), z1 = quarter_max(in ? a.z[k] : INT_MIN);
       auto put_box = [unit](float* b, int32_t x0, int32_t y0, int32_t z0, int32_t x1, int32_t y1, int32_t z1) {
         const bool empty = x0 > x1;
-        b[0] = empty ? INFINITY : static_cast<float>(x0) * unit;
-        b[1] = empty ? INFINITY : static_cast<float>(y0) * unit;
-        b[2] = empty ? INFINITY : static_cast<float>(z0) * unit;
-        b[3] = empty ? -INFINITY : static_cast<float>(x1) * unit;
-        b[4] = empty ? -INFINITY : static_cast<float>(y1) * unit;
-        b[5] = empty ? -INFINITY : static_cast<float>(z1) * unit;
+        b[0] = empty ? kFarBox : static_cast<float>(x0) * unit;
+        b[1] = empty ? kFarBox : static_cast<float>(y0) * unit;
+        b[2] = empty ? kFarBox : static_cast<float>(z0) * unit;
+        b[3] = empty ? kFarBox : static_cast<float>(x1) * unit;
+        b[4] = empty ? kFarBox : static_cast<float>(y1) * unit;
+        b[5] = empty ? kFarBox : static_cast<float>(z1) * unit;
       };
       if ((lane & (kLigSub - 1)) == 0)
         put_box(my_box + kBoxFloats * k + 8 * (1 + lane / kLigSub), x0, y0, z0, x1, y1, z1);
