@@ -184,8 +184,10 @@ __device__ __forceinline__ bool box_beyond_reach(float4 lo, float4 hi, const flo
 // Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
 // t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
 // slot against every block; padding slots sit far from every block), and,
-// for a block some slot reaches, one of the block's quarters.  A culled pair
-// would only have added exact zeros, so the tests decide work, not results.
+// for a block some slot reaches, one of the block's quarters.  Blocks whose
+// bit in kmask is clear are beyond reach of every sub-tile box of the tile
+// (so of every slot in it) and are not tested.  A culled pair would only have
+// added exact zeros, so the tests decide work, not results.
 // Lane t's receptor slot g0 + t of the tile [g0, g1) (slot g0 past its end).
 __device__ __forceinline__ int4 tile_slot(CxRef cx, int g0, int g1) {
   const int lane = threadIdx.x & 63;
@@ -195,13 +197,15 @@ __device__ __forceinline__ int4 tile_slot(CxRef cx, int g0, int g1) {
 // r: the lane's slot of the tile, as tile_slot loads it.
 template <int K>
 __device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int g1, const float* __restrict__ boxes,
-                                               const int4 r, uint64_t (&keep)[K]) {
+                                               const int4 r, uint32_t kmask, uint64_t (&keep)[K]) {
   const int lane = threadIdx.x & 63;
   const bool valid = g0 + lane < g1;
   const float rx = static_cast<float>(r.x) * unit, ry = static_cast<float>(r.y) * unit,
               rz = static_cast<float>(r.z) * unit;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    keep[k] = 0ull;
+    if (((kmask >> k) & 1u) == 0u) continue;
     const float* b = boxes + kBoxFloats * k;
     keep[k] = __builtin_amdgcn_ballot_w64(valid && !beyond_reach(rx, ry, rz, b));
     if (keep[k] != 0ull)
@@ -611,17 +615,21 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
     int cls = -1, cend = 0;
     // Receptor sub-tiles (kRecSub slots) near any of the wave's blocks, 64
     // at a time (lane t tests sub-tile s0 + t); a 64-slot tile is walked when
-    // one of its four sub-tiles is.  Parts start on whole tiles.
+    // one of its four sub-tiles is, and tests its slots against the blocks
+    // that one of its sub-tiles reaches.  Parts start on whole tiles.
     const int s_end = (pp.r1 + kRecSub - 1) / kRecSub;
     for (int s0 = pp.r0 / kRecSub; s0 < s_end; s0 += 64) {
       const int si = s0 + lane;
-      bool near = false;
+      uint32_t near_k = 0;  // bit k: sub-tile si within reach of block k
       if (si < s_end) {
         const float4 lo = cx.rec_sub_box[2 * si], hi = cx.rec_sub_box[2 * si + 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) near = near || !box_beyond_reach(lo, hi, my_box + kBoxFloats * k);
+        for (int k = 0; k < K; ++k) near_k |= box_beyond_reach(lo, hi, my_box + kBoxFloats * k) ? 0u : 1u << k;
       }
-      uint64_t tiles = __builtin_amdgcn_ballot_w64(near);
+      uint64_t tiles = __builtin_amdgcn_ballot_w64(near_k != 0u);
+      // each tile's blocks, in the four lanes of its sub-tiles
+      near_k |= __shfl_xor(near_k, 1);
+      near_k |= __shfl_xor(near_k, 2);
       // a tile's first sub-tile; the next near tile's slots load while this
       // one is tested and walked
       auto tile_first = [](uint64_t t) { return (__builtin_ctzll(t) / (64 / kRecSub)) * (64 / kRecSub); };
@@ -644,7 +652,7 @@ __device__ __forceinline__ void score_pose(CxRef cx, const double* __restrict__ 
       // adds nothing and holds no band pair with it: that block skips the
       // atom (a group when no block is near, the tile when nothing is).
       uint64_t keep[K];
-      atoms_in_reach<K>(cx, unit, g0, g1, my_box, r_tile, keep);
+      atoms_in_reach<K>(cx, unit, g0, g1, my_box, r_tile, __builtin_amdgcn_readlane(near_k, first), keep);
       uint64_t any_keep = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) any_keep |= keep[k];
