@@ -181,20 +181,20 @@ __device__ __forceinline__ bool box_beyond_reach(float4 lo, float4 hi, const flo
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) > kCullReach * kCullReach;
 }
 
-// Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
-// t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
-// slot against every block; padding slots sit far from every block), and,
-// for a block some slot reaches, one of the block's quarters.  Blocks whose
-// bit in kmask is clear are beyond reach of every sub-tile box of the tile
-// (so of every slot in it) and are not tested.  A culled pair would only have
-// added exact zeros, so the tests decide work, not results.
 // Lane t's receptor slot g0 + t of the tile [g0, g1) (slot g0 past its end).
 __device__ __forceinline__ int4 tile_slot(CxRef cx, int g0, int g1) {
   const int lane = threadIdx.x & 63;
   return cx.rec_fixed[g0 + lane < g1 ? g0 + lane : g0];
 }
 
-// r: the lane's slot of the tile, as tile_slot loads it.
+// Receptor atoms of the tile [g0, g1) against the wave's ligand blocks: bit
+// t of keep[k] is set when slot g0 + t can reach block k (lane t tests its
+// slot against every block; padding slots sit far from every block), and,
+// for a block some slot reaches, one of the block's quarters.  Blocks whose
+// bit in kmask is clear are beyond reach of every sub-tile box of the tile
+// (so of every slot in it) and are not tested.  A culled pair would only have
+// added exact zeros, so the tests decide work, not results.  r: the lane's
+// slot of the tile, as tile_slot loads it.
 template <int K>
 __device__ __forceinline__ void atoms_in_reach(CxRef cx, float unit, int g0, int g1, const float* __restrict__ boxes,
                                                const int4 r, uint32_t kmask, uint64_t (&keep)[K]) {
